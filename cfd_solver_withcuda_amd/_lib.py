@@ -1,0 +1,88 @@
+"""Loader of libcfdhip.so (the HIP implementation) with ctypes signatures of its C-ABI.
+
+There is no CPU fallback: if the library is missing the import of any solver object raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libcfdhip.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_up = ctypes.POINTER(ctypes.c_ushort)
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_d = ctypes.c_double
+
+_lib = None
+
+
+class CfdError(RuntimeError):
+    pass
+
+
+class Config(ctypes.Structure):
+    """cfd_config_t (include/cfd_hip.h)."""
+    _fields_ = [("flux", _i), ("order", _i), ("limiter", _i), ("gradient", _i), ("roe_bdiss", _i),
+                ("cfl", _d), ("bc", _i * 6), ("fs", _d * 4), ("rk", _i), ("lts", _i),
+                ("ref_mode", _i), ("block", _i), ("xcd_remap", _i)]
+
+
+def _sig(lib, name, res, args):
+    fn = getattr(lib, name)
+    fn.restype = res
+    fn.argtypes = args
+
+
+def lib() -> ctypes.CDLL:
+    """The loaded library; raises CfdError when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise CfdError(f"{LIB_PATH} is missing: build it with `python -m cfd_solver_withcuda_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    cfgp = ctypes.POINTER(Config)
+    _sig(L, "cfd_last_error", ctypes.c_char_p, [])
+    _sig(L, "cfd_version", _i, [])
+    for nm in ("cfd_mesh_create", "cfd_mesh_create_host"):
+        _sig(L, nm, _i, [_i, _dp, _i, _ip, _i, _i, ctypes.POINTER(_vp)])
+    _sig(L, "cfd_mesh_destroy", None, [_vp])
+    _sig(L, "cfd_mesh_counts", _i, [_vp, _ip])
+    _sig(L, "cfd_mesh_export", _i, [_vp, _ip, _ip, _ip, _dp, _dp, _dp, _dp, _dp, _dp, _ip, _ip, _ip, _ip])
+    _sig(L, "cfd_mesh_permutation", _i, [_vp, _ip])
+    _sig(L, "cfd_mesh_layout_export", _i, [_vp, _ip, _ip, _ip, _up])
+    _sig(L, "cfd_solver_create", _i, [_vp, cfgp, ctypes.POINTER(_vp)])
+    _sig(L, "cfd_solver_destroy", None, [_vp])
+    _sig(L, "cfd_solver_configure", _i, [_vp, cfgp])
+    _sig(L, "cfd_set_state", _i, [_vp, _dp])
+    _sig(L, "cfd_get_state", _i, [_vp, _dp])
+    _sig(L, "cfd_state_device", _i, [_vp, ctypes.POINTER(_vp)])
+    _sig(L, "cfd_residual", _i, [_vp, _dp, _dp, _dp, _vp])
+    _sig(L, "cfd_step", _i, [_vp, _i, _dp, _dp, _vp])
+    _sig(L, "cfd_run_steady", _i, [_vp, _d, _i, _i, _ip, _dp, _vp])
+    _sig(L, "cfd_time_pass", _i, [_vp, _i, _i, ctypes.POINTER(ctypes.c_float), _vp])
+    _lib = L
+    return L
+
+
+def check(rc: int, what: str = "") -> int:
+    if rc != 0:
+        msg = lib().cfd_last_error().decode(errors="replace")
+        raise CfdError(f"{what}: error {rc}: {msg}")
+    return rc
+
+
+def dptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def iptr(a):
+    return a.ctypes.data_as(_ip)
+
+
+def uptr(a):
+    return a.ctypes.data_as(_up)

@@ -1,0 +1,507 @@
+// C-ABI of the native path (include/cfd_hip.h): mesh handles, solver state, residual, steps.
+#include "../../include/cfd_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "fv_kernels.h"
+#include "mesh_host.h"
+
+using namespace cfd;
+
+namespace {
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *where) {
+    g_err = std::string(where) + ": " + hipGetErrorString(e);
+    return (int)e;
+}
+
+#define HIP_TRY(expr)                                  \
+    do {                                               \
+        hipError_t _e = (expr);                        \
+        if (_e != hipSuccess) return hip_fail(_e, #expr); \
+    } while (0)
+
+template <typename T>
+hipError_t upload(T **d, const std::vector<T> &h) {
+    *d = nullptr;
+    if (h.empty()) return hipSuccess;
+    hipError_t e = hipMalloc((void **)d, h.size() * sizeof(T));
+    if (e != hipSuccess) return e;
+    return hipMemcpy(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice);
+}
+
+const double kDblMax = 1.7976931348623157e308;
+}  // namespace
+
+struct cfd_mesh_s {
+    RefMesh ref;
+    Layout lay;
+    bool on_device = false;
+    int *d_tile_fstart = nullptr, *d_fL = nullptr, *d_fR = nullptr;
+    double *d_fnx = nullptr, *d_fny = nullptr, *d_fdl = nullptr, *d_fdx = nullptr, *d_fdy = nullptr;
+    double *d_fwL = nullptr, *d_fwR = nullptr;
+    uint16_t *d_slots = nullptr;
+    double *d_area = nullptr, *d_inv_area = nullptr;
+    DevMesh dev{};
+    ~cfd_mesh_s() {
+        void *ps[] = {d_tile_fstart, d_fL, d_fR, d_fnx, d_fny, d_fdl, d_fdx, d_fdy, d_fwL, d_fwR, d_slots, d_area, d_inv_area};
+        for (void *p : ps)
+            if (p) (void)hipFree(p);
+    }
+};
+
+struct cfd_solver_s {
+    cfd_mesh_s *m = nullptr;
+    cfd_config_t cfg{};
+    DevCfg dcfg{};
+    double4 *Un = nullptr, *Ua = nullptr, *Ub = nullptr, *acc = nullptr, *R = nullptr;
+    double *dt = nullptr, *grad = nullptr, *hbuf = nullptr, *err_partial = nullptr;
+    double *scal = nullptr;  // [0],[1] global dt slots, [2] stage minimum, [3..6] error norm
+    int cur = 0;
+    bool dt_valid = false;
+    bool has_state = false;
+    ~cfd_solver_s() {
+        void *ps[] = {Un, Ua, Ub, acc, R, dt, grad, hbuf, err_partial, scal};
+        for (void *p : ps)
+            if (p) (void)hipFree(p);
+    }
+};
+
+extern "C" {
+
+const char *cfd_last_error(void) { return g_err.c_str(); }
+int cfd_version(void) { return 1; }
+
+int cfd_mesh_create_host(int npts, const double *pts3, int ncells, const int *cells4, int tile,
+                         int renumber, cfd_mesh_t **out) {
+    if (!out || !pts3 || !cells4 || npts <= 0 || ncells <= 0) return fail(CFD_E_ARG, "cfd_mesh_create: bad arguments");
+    *out = nullptr;
+    cfd_mesh_s *m = new cfd_mesh_s();
+    try {
+        build_ref_mesh(npts, pts3, ncells, cells4, m->ref);
+        build_layout(m->ref, tile, renumber, m->lay);
+    } catch (const std::exception &e) {
+        delete m;
+        return fail(CFD_E_MESH, std::string("cfd_mesh_create: ") + e.what());
+    }
+    *out = m;
+    return 0;
+}
+
+int cfd_mesh_create(int npts, const double *pts3, int ncells, const int *cells4, int tile,
+                    int renumber, cfd_mesh_t **out) {
+    int rc = cfd_mesh_create_host(npts, pts3, ncells, cells4, tile, renumber, out);
+    if (rc) return rc;
+    cfd_mesh_s *m = *out;
+    *out = nullptr;
+    const Layout &L = m->lay;
+    hipError_t e = hipSuccess;
+    if ((e = upload(&m->d_tile_fstart, L.tile_fstart)) != hipSuccess ||
+        (e = upload(&m->d_fL, L.fL)) != hipSuccess || (e = upload(&m->d_fR, L.fR)) != hipSuccess ||
+        (e = upload(&m->d_fnx, L.fnx)) != hipSuccess || (e = upload(&m->d_fny, L.fny)) != hipSuccess ||
+        (e = upload(&m->d_fdl, L.fdl)) != hipSuccess || (e = upload(&m->d_fdx, L.fdx)) != hipSuccess ||
+        (e = upload(&m->d_fdy, L.fdy)) != hipSuccess || (e = upload(&m->d_fwL, L.fwL)) != hipSuccess ||
+        (e = upload(&m->d_fwR, L.fwR)) != hipSuccess || (e = upload(&m->d_slots, L.slots)) != hipSuccess ||
+        (e = upload(&m->d_area, L.area)) != hipSuccess || (e = upload(&m->d_inv_area, L.inv_area)) != hipSuccess) {
+        delete m;
+        return hip_fail(e, "cfd_mesh_create upload");
+    }
+    DevMesh &d = m->dev;
+    d.nc = m->ref.nc;
+    d.ntiles = L.ntiles;
+    d.tile = L.tile;
+    d.max_tile_faces = L.max_tile_faces;
+    d.tile_fstart = m->d_tile_fstart;
+    d.fL = m->d_fL;
+    d.fR = m->d_fR;
+    d.fnx = m->d_fnx;
+    d.fny = m->d_fny;
+    d.fdl = m->d_fdl;
+    d.fdx = m->d_fdx;
+    d.fdy = m->d_fdy;
+    d.fwL = reinterpret_cast<const double2 *>(m->d_fwL);
+    d.fwR = reinterpret_cast<const double2 *>(m->d_fwR);
+    d.slots = m->d_slots;
+    d.area = m->d_area;
+    d.inv_area = m->d_inv_area;
+    m->on_device = true;
+    *out = m;
+    return 0;
+}
+
+void cfd_mesh_destroy(cfd_mesh_t *m) { delete m; }
+
+int cfd_mesh_counts(const cfd_mesh_t *m, int *o) {
+    if (!m || !o) return fail(CFD_E_ARG, "cfd_mesh_counts: null argument");
+    o[0] = m->ref.nc;
+    o[1] = m->ref.ng;
+    o[2] = (int)m->ref.inl.size() / 3;
+    o[3] = (int)m->ref.exl.size() / 3;
+    o[4] = (int)m->ref.wal.size() / 3;
+    o[5] = m->lay.nfaces;
+    o[6] = m->lay.ntiles;
+    o[7] = (int)m->lay.fL.size();
+    o[8] = m->lay.max_tile_faces;
+    return 0;
+}
+
+int cfd_mesh_export(const cfd_mesh_t *m, int *nf, int *nodes4, int *nbr4, double *nrm8,
+                    double *len4, double *ctr3_all, double *area_all, double *inv_area_all,
+                    double *dist4, int *kind4, int *inlet3, int *exit3, int *wall3) {
+    if (!m) return fail(CFD_E_ARG, "cfd_mesh_export: null mesh");
+    const RefMesh &r = m->ref;
+    const size_t nc = r.nc, nt = (size_t)r.nc + r.ng;
+    auto cp = [](void *dst, const void *src, size_t bytes) {
+        if (dst && bytes) std::memcpy(dst, src, bytes);
+    };
+    cp(nf, r.nf.data(), nc * sizeof(int));
+    for (size_t c = 0; c < nc; c++)
+        for (int f = 0; f < r.nf[c]; f++) {
+            if (nodes4) nodes4[4 * c + f] = r.nodes[4 * c + f];
+            if (nbr4) nbr4[4 * c + f] = r.nbr[4 * c + f];
+            if (nrm8) {
+                nrm8[8 * c + 2 * f] = r.nrm[8 * c + 2 * f];
+                nrm8[8 * c + 2 * f + 1] = r.nrm[8 * c + 2 * f + 1];
+            }
+            if (len4) len4[4 * c + f] = r.len[4 * c + f];
+            if (dist4) dist4[4 * c + f] = r.dist[4 * c + f];
+            if (kind4) kind4[4 * c + f] = r.kind[4 * c + f];
+        }
+    cp(ctr3_all, r.ctr.data(), 3 * nt * sizeof(double));
+    cp(area_all, r.area.data(), nt * sizeof(double));
+    cp(inv_area_all, r.inv_area.data(), nt * sizeof(double));
+    cp(inlet3, r.inl.data(), r.inl.size() * sizeof(int));
+    cp(exit3, r.exl.data(), r.exl.size() * sizeof(int));
+    cp(wall3, r.wal.data(), r.wal.size() * sizeof(int));
+    return 0;
+}
+
+int cfd_mesh_permutation(const cfd_mesh_t *m, int *new_to_old) {
+    if (!m || !new_to_old) return fail(CFD_E_ARG, "cfd_mesh_permutation: null argument");
+    std::memcpy(new_to_old, m->lay.perm.data(), m->lay.perm.size() * sizeof(int));
+    return 0;
+}
+
+int cfd_mesh_layout_export(const cfd_mesh_t *m, int *tile_fstart, int *fL, int *fR,
+                           unsigned short *slots4) {
+    if (!m) return fail(CFD_E_ARG, "cfd_mesh_layout_export: null mesh");
+    const Layout &L = m->lay;
+    if (tile_fstart) std::memcpy(tile_fstart, L.tile_fstart.data(), L.tile_fstart.size() * sizeof(int));
+    if (fL) std::memcpy(fL, L.fL.data(), L.fL.size() * sizeof(int));
+    if (fR) std::memcpy(fR, L.fR.data(), L.fR.size() * sizeof(int));
+    if (slots4) std::memcpy(slots4, L.slots.data(), L.slots.size() * sizeof(uint16_t));
+    return 0;
+}
+
+static int check_config(const cfd_mesh_s *m, const cfd_config_t *c) {
+    if (c->flux < 0 || c->flux > 3) return fail(CFD_E_ARG, "config: flux must be 0..3");
+    if (c->order != 1 && c->order != 2) return fail(CFD_E_ARG, "config: order must be 1 or 2");
+    if (c->limiter < 0 || c->limiter > 3) return fail(CFD_E_ARG, "config: limiter must be 0..3");
+    if (c->gradient < 0 || c->gradient > 1) return fail(CFD_E_ARG, "config: gradient must be 0 or 1");
+    if (!(c->cfl > 0.0)) return fail(CFD_E_ARG, "config: cfl must be positive");
+    if (c->rk != 1 && c->rk != 3 && c->rk != 4) return fail(CFD_E_ARG, "config: rk must be 1, 3 or 4");
+    if (c->rk == 1 && !c->ref_mode) return fail(CFD_E_ARG, "config: forward Euler (rk 1) exists in reference mode only");
+    if (c->ref_mode && c->rk == 4) return fail(CFD_E_ARG, "config: the reference iteration has no RK4");
+    if (c->block != 64 && c->block != 128 && c->block != 256) return fail(CFD_E_ARG, "config: block must be 64, 128 or 256");
+    for (int k = 1; k <= 5; k++)
+        if (c->bc[k] < 1 || c->bc[k] > 7) return fail(CFD_E_ARG, "config: bc types must be 1..7");
+    const size_t sm = flux_smem_bytes(m->dev, c->order);
+    if (sm > 65536 || grad_smem_bytes(m->dev) > 65536)
+        return fail(CFD_E_ARG, "config: tile too large for 64 KiB of LDS per workgroup; use a smaller tile");
+    return 0;
+}
+
+int cfd_solver_configure(cfd_solver_t *s, const cfd_config_t *cfg) {
+    if (!s || !cfg) return fail(CFD_E_ARG, "cfd_solver_configure: null argument");
+    int rc = check_config(s->m, cfg);
+    if (rc) return rc;
+    s->cfg = *cfg;
+    for (int k = 0; k < 6; k++) s->dcfg.bc[k] = cfg->bc[k];
+    for (int k = 0; k < 4; k++) s->dcfg.fs[k] = cfg->fs[k];
+    s->dcfg.cfl = cfg->cfl;
+    s->dcfg.roe_bdiss = cfg->roe_bdiss;
+    s->dcfg.xcd_remap = cfg->xcd_remap;
+    s->dt_valid = false;
+    return 0;
+}
+
+int cfd_solver_create(cfd_mesh_t *m, const cfd_config_t *cfg, cfd_solver_t **out) {
+    if (!m || !cfg || !out) return fail(CFD_E_ARG, "cfd_solver_create: null argument");
+    *out = nullptr;
+    if (!m->on_device) return fail(CFD_E_MESH, "cfd_solver_create: mesh was built with cfd_mesh_create_host");
+    cfd_solver_s *s = new cfd_solver_s();
+    s->m = m;
+    int rc = cfd_solver_configure(s, cfg);
+    if (rc) {
+        delete s;
+        return rc;
+    }
+    const size_t nc = m->ref.nc;
+    const size_t nparts = std::max<size_t>(m->lay.ntiles, (nc + 63) / 64);
+    hipError_t e;
+    if ((e = hipMalloc((void **)&s->Un, nc * sizeof(double4))) != hipSuccess ||
+        (e = hipMalloc((void **)&s->Ua, nc * sizeof(double4))) != hipSuccess ||
+        (e = hipMalloc((void **)&s->Ub, nc * sizeof(double4))) != hipSuccess ||
+        (e = hipMalloc((void **)&s->acc, nc * sizeof(double4))) != hipSuccess ||
+        (e = hipMalloc((void **)&s->R, nc * sizeof(double4))) != hipSuccess ||
+        (e = hipMalloc((void **)&s->dt, nc * sizeof(double))) != hipSuccess ||
+        (e = hipMalloc((void **)&s->grad, 8 * nc * sizeof(double))) != hipSuccess ||
+        (e = hipMalloc((void **)&s->hbuf, nc * sizeof(double))) != hipSuccess ||
+        (e = hipMalloc((void **)&s->err_partial, 4 * nparts * sizeof(double))) != hipSuccess ||
+        (e = hipMalloc((void **)&s->scal, 8 * sizeof(double))) != hipSuccess) {
+        delete s;
+        return hip_fail(e, "cfd_solver_create hipMalloc");
+    }
+    *out = s;
+    return 0;
+}
+
+void cfd_solver_destroy(cfd_solver_t *s) { delete s; }
+
+int cfd_set_state(cfd_solver_t *s, const double *U) {
+    if (!s || !U) return fail(CFD_E_ARG, "cfd_set_state: null argument");
+    const int nc = s->m->ref.nc;
+    std::vector<double> h(4 * (size_t)nc);
+    for (int k = 0; k < nc; k++) std::memcpy(&h[4 * (size_t)k], &U[4 * (size_t)s->m->lay.perm[k]], 4 * sizeof(double));
+    HIP_TRY(hipMemcpy(s->Un, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    s->has_state = true;
+    s->dt_valid = false;
+    return 0;
+}
+
+int cfd_get_state(cfd_solver_t *s, double *U) {
+    if (!s || !U) return fail(CFD_E_ARG, "cfd_get_state: null argument");
+    const int nc = s->m->ref.nc;
+    std::vector<double> h(4 * (size_t)nc);
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(h.data(), s->Un, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int k = 0; k < nc; k++) std::memcpy(&U[4 * (size_t)s->m->lay.perm[k]], &h[4 * (size_t)k], 4 * sizeof(double));
+    return 0;
+}
+
+int cfd_state_device(cfd_solver_t *s, void **d_U) {
+    if (!s || !d_U) return fail(CFD_E_ARG, "cfd_state_device: null argument");
+    *d_U = s->Un;
+    return 0;
+}
+
+static StageArgs resid_args(cfd_solver_s *s, double *dtmin, const double4 *uexit) {
+    StageArgs st{};
+    st.R_out = reinterpret_cast<double *>(s->R);
+    st.dt_out = s->dt;
+    st.dtmin_out = dtmin;
+    st.Uexit = uexit;
+    return st;
+}
+
+// residual of state `in` into s->R, s->dt (+ minimum dt into *dtmin when set)
+static int residual_pass(cfd_solver_s *s, const double4 *in, double *dtmin, const double4 *uexit,
+                         hipStream_t st, int order_override = 0) {
+    const cfd_config_t &c = s->cfg;
+    const int order = order_override ? order_override : c.order;
+    if (order == 2) HIP_TRY(launch_grad(s->m->dev, s->dcfg, c.gradient, in, s->grad, c.block, st));
+    if (dtmin) HIP_TRY(launch_fill(dtmin, kDblMax, 1, st));
+    StageArgs a = resid_args(s, dtmin, uexit);
+    HIP_TRY(launch_flux(s->m->dev, s->dcfg, c.flux, order, c.limiter, MODE_RESID, in, s->grad, a, c.block, st));
+    return 0;
+}
+
+int cfd_residual(cfd_solver_t *s, double *R, double *dt, double *grad, void *stream) {
+    if (!s) return fail(CFD_E_ARG, "cfd_residual: null solver");
+    if (!s->has_state) return fail(CFD_E_STATE, "cfd_residual: state not set");
+    hipStream_t st = (hipStream_t)stream;
+    int rc = residual_pass(s, s->Un, nullptr, nullptr, st);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(st));
+    const int nc = s->m->ref.nc;
+    const std::vector<int> &perm = s->m->lay.perm;
+    if (R) {
+        std::vector<double> h(4 * (size_t)nc);
+        HIP_TRY(hipMemcpy(h.data(), s->R, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (int k = 0; k < nc; k++) std::memcpy(&R[4 * (size_t)perm[k]], &h[4 * (size_t)k], 4 * sizeof(double));
+    }
+    if (dt) {
+        std::vector<double> h(nc);
+        HIP_TRY(hipMemcpy(h.data(), s->dt, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (int k = 0; k < nc; k++) dt[perm[k]] = h[k];
+    }
+    if (grad && s->cfg.order == 2) {
+        std::vector<double> h(8 * (size_t)nc);
+        HIP_TRY(hipMemcpy(h.data(), s->grad, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (int k = 0; k < nc; k++) std::memcpy(&grad[8 * (size_t)perm[k]], &h[8 * (size_t)k], 8 * sizeof(double));
+    }
+    return 0;
+}
+
+// One native step (fused stages). want_err: emit the error norm into scal[3..6].
+static int native_step(cfd_solver_s *s, bool want_err, hipStream_t st) {
+    const cfd_config_t &c = s->cfg;
+    const DevMesh &dm = s->m->dev;
+    if (!c.lts && !s->dt_valid) {
+        int rc = residual_pass(s, s->Un, s->scal + s->cur, nullptr, st, 1);
+        if (rc) return rc;
+        s->dt_valid = true;
+    }
+    if (!c.lts) HIP_TRY(launch_fill(s->scal + (1 - s->cur), kDblMax, 1, st));
+    const int nst = (c.rk == 4) ? 4 : 3;
+    const double4 *ins3[3] = {s->Un, s->Ua, s->Ub};
+    double4 *outs3[3] = {s->Ua, s->Ub, s->Un};
+    const double4 *ins4[4] = {s->Un, s->Ua, s->Ub, s->Ua};
+    double4 *outs4[4] = {s->Ua, s->Ub, s->Ua, s->Un};
+    for (int k = 0; k < nst; k++) {
+        const double4 *in = (nst == 4) ? ins4[k] : ins3[k];
+        double4 *out = (nst == 4) ? outs4[k] : outs3[k];
+        if (c.order == 2) HIP_TRY(launch_grad(dm, s->dcfg, c.gradient, in, s->grad, c.block, st));
+        StageArgs a{};
+        a.rk = nst;
+        a.stage = k;
+        a.lts = c.lts;
+        a.last = (k == nst - 1);
+        a.Un = s->Un;
+        a.Uout = out;
+        a.acc = s->acc;
+        a.hbuf = s->hbuf;
+        a.dt_global = s->scal + s->cur;
+        a.err_partial = (a.last && want_err) ? s->err_partial : nullptr;
+        a.dtmin_out = (a.last && !c.lts) ? s->scal + (1 - s->cur) : nullptr;
+        HIP_TRY(launch_flux(dm, s->dcfg, c.flux, c.order, c.limiter, MODE_STAGE, in, s->grad, a, c.block, st));
+    }
+    if (want_err) HIP_TRY(launch_err_finalize(s->err_partial, dm.ntiles, s->scal + 3, st));
+    return 0;
+}
+
+// One iteration of the reference's Inviscid_Solver loop body.
+static int ref_step(cfd_solver_s *s, hipStream_t st) {
+    const cfd_config_t &c = s->cfg;
+    const int nc = s->m->ref.nc;
+    const int nb = (nc + 255) / 256;
+    double *dmin = s->scal + 2;
+    const double4 *R = s->R;
+    int rc;
+    if (c.rk == 1) {
+        if ((rc = residual_pass(s, s->Un, dmin, s->Un, st))) return rc;
+        HIP_TRY(launch_ref_update(nc, 3, 0, s->m->d_inv_area, s->Un, s->Un, R, s->dt, dmin, s->Un, s->err_partial, st));
+    } else {
+        if ((rc = residual_pass(s, s->Un, dmin, s->Un, st))) return rc;
+        HIP_TRY(launch_ref_update(nc, 0, c.lts, s->m->d_inv_area, s->Un, s->Un, R, s->dt, dmin, s->Ua, nullptr, st));
+        if ((rc = residual_pass(s, s->Ua, dmin, s->Un, st))) return rc;
+        HIP_TRY(launch_ref_update(nc, 1, 0, s->m->d_inv_area, s->Un, s->Ua, R, s->dt, dmin, s->Ub, nullptr, st));
+        if ((rc = residual_pass(s, s->Ub, dmin, s->Un, st))) return rc;
+        HIP_TRY(launch_ref_update(nc, 2, 0, s->m->d_inv_area, s->Un, s->Ub, R, s->dt, dmin, s->Un, s->err_partial, st));
+    }
+    HIP_TRY(launch_err_finalize(s->err_partial, nb, s->scal + 3, st));
+    return 0;
+}
+
+int cfd_step(cfd_solver_t *s, int nsteps, double *err_hist, double *dt_hist, void *stream) {
+    if (!s) return fail(CFD_E_ARG, "cfd_step: null solver");
+    if (!s->has_state) return fail(CFD_E_STATE, "cfd_step: state not set");
+    if (nsteps < 0) return fail(CFD_E_ARG, "cfd_step: negative step count");
+    hipStream_t st = (hipStream_t)stream;
+    const bool sync = err_hist || dt_hist;
+    for (int n = 0; n < nsteps; n++) {
+        const int cur_before = s->cur;
+        int rc = s->cfg.ref_mode ? ref_step(s, st) : native_step(s, err_hist != nullptr, st);
+        if (rc) return rc;
+        if (sync) {
+            double h[8];
+            HIP_TRY(hipMemcpyAsync(h, s->scal, sizeof(h), hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipStreamSynchronize(st));
+            if (err_hist)
+                for (int q = 0; q < 4; q++) err_hist[4 * n + q] = h[3 + q];
+            if (dt_hist) dt_hist[n] = s->cfg.ref_mode ? h[2] : (s->cfg.lts ? NAN : h[cur_before]);
+        }
+        if (!s->cfg.ref_mode && !s->cfg.lts) s->cur = 1 - s->cur;
+    }
+    return 0;
+}
+
+int cfd_time_pass(cfd_solver_t *s, int which, int reps, float *ms, void *stream) {
+    if (!s || reps <= 0 || !ms) return fail(CFD_E_ARG, "cfd_time_pass: bad arguments");
+    if (!s->has_state) return fail(CFD_E_STATE, "cfd_time_pass: state not set");
+    if (which < 0 || which > 2) return fail(CFD_E_ARG, "cfd_time_pass: which must be 0, 1 or 2");
+    if (which == 0 && s->cfg.order != 2) return fail(CFD_E_ARG, "cfd_time_pass: no gradient pass at order 1");
+    if (s->cfg.ref_mode) return fail(CFD_E_ARG, "cfd_time_pass: native mode only");
+    hipStream_t st = (hipStream_t)stream;
+    const cfd_config_t &c = s->cfg;
+    const DevMesh &dm = s->m->dev;
+    if (!c.lts && !s->dt_valid) {
+        int rc = residual_pass(s, s->Un, s->scal + s->cur, nullptr, st, 1);
+        if (rc) return rc;
+        s->dt_valid = true;
+    }
+    if (c.order == 2) HIP_TRY(launch_grad(dm, s->dcfg, c.gradient, s->Un, s->grad, c.block, st));
+    StageArgs a{};
+    a.rk = (c.rk == 4) ? 4 : 3;
+    a.stage = 0;
+    a.lts = c.lts;
+    a.last = 0;
+    a.Un = s->Un;
+    a.Uout = s->Ua;
+    a.acc = s->acc;
+    a.hbuf = s->hbuf;
+    a.dt_global = s->scal + s->cur;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, st));
+    for (int r = 0; r < reps; r++) {
+        if (which == 0) HIP_TRY(launch_grad(dm, s->dcfg, c.gradient, s->Un, s->grad, c.block, st));
+        else if (which == 1) HIP_TRY(launch_flux(dm, s->dcfg, c.flux, c.order, c.limiter, MODE_STAGE, s->Un, s->grad, a, c.block, st));
+        else {
+            int rc = native_step(s, false, st);
+            if (rc) return rc;
+            if (!c.lts) s->cur = 1 - s->cur;
+        }
+    }
+    HIP_TRY(hipEventRecord(e1, st));
+    HIP_TRY(hipEventSynchronize(e1));
+    HIP_TRY(hipEventElapsedTime(ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return 0;
+}
+
+int cfd_run_steady(cfd_solver_t *s, double tol, int max_steps, int check_every, int *steps,
+                   double *err4, void *stream) {
+    if (!s || check_every <= 0 || max_steps < 0) return fail(CFD_E_ARG, "cfd_run_steady: bad arguments");
+    if (!s->has_state) return fail(CFD_E_STATE, "cfd_run_steady: state not set");
+    hipStream_t st = (hipStream_t)stream;
+    int done = 0;
+    double e[4] = {NAN, NAN, NAN, NAN};
+    while (done < max_steps) {
+        const int chunk = std::min(check_every, max_steps - done);
+        for (int n = 0; n < chunk; n++) {
+            const bool want = (n == chunk - 1);
+            int rc = s->cfg.ref_mode ? ref_step(s, st) : native_step(s, want, st);
+            if (rc) return rc;
+            if (!s->cfg.ref_mode && !s->cfg.lts) s->cur = 1 - s->cur;
+        }
+        done += chunk;
+        double h[8];
+        HIP_TRY(hipMemcpyAsync(h, s->scal, sizeof(h), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        for (int q = 0; q < 4; q++) e[q] = h[3 + q];
+        if (!std::isfinite(e[0])) break;
+        if (e[0] <= tol) break;
+    }
+    if (steps) *steps = done;
+    if (err4)
+        for (int q = 0; q < 4; q++) err4[q] = e[q];
+    if (!(e[0] <= tol)) return fail(CFD_E_NOTCONV, "cfd_run_steady: not converged");
+    return 0;
+}
+
+}  // extern "C"

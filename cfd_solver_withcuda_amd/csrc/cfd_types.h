@@ -1,0 +1,51 @@
+// Plain data shared by the host library and the HIP kernels of the finite-volume path.
+#pragma once
+#include <stdint.h>
+
+namespace cfd {
+
+// Boundary-condition types (ghost states), assigned per reference face kind.
+enum BcType {
+    BC_NONE = 0,
+    BC_INLET_REF = 1,           // rho, P prescribed; interior velocity
+    BC_EXIT = 2,                // copy of the interior state
+    BC_WALL = 3,                // slip wall, mirrored velocity
+    BC_SYMMETRY = 4,            // same arithmetic as the wall
+    BC_FREESTREAM = 5,          // all primitives prescribed
+    BC_FARFIELD = 6,            // Riemann-invariant far field
+    BC_SUBSONIC_INLET_REF = 7   // the reference's subsonic inlet
+};
+// Reference face kinds (src/Read_Gmsh_File.cpp:318-323).
+enum FaceKind { KIND_INTERNAL = 0, KIND_LEFT = 1, KIND_RIGHT = 2, KIND_TOP = 3, KIND_BOTTOM = 4, KIND_WALL = 5 };
+enum FluxType { FLUX_ROE = 0, FLUX_HLLC = 1, FLUX_LLF = 2, FLUX_AUSM = 3 };
+enum LimiterType { LIM_NONE = 0, LIM_MINMOD = 1, LIM_VANLEER = 2, LIM_SUPERBEE = 3 };
+enum GradType { GRAD_GG = 0, GRAD_WLSQ = 1 };
+
+// Device view of a tiled mesh. Cells are renumbered so that tiles are contiguous ranges of
+// `tile` cells; each tile lists every face touching one of its cells (faces between two tiles
+// appear in both lists with identical data, so both tiles compute identical fluxes).
+struct DevMesh {
+    int nc;        // physical cells
+    int ntiles;
+    int tile;      // cells per tile
+    int max_tile_faces;
+    const int *tile_fstart;   // [ntiles+1] first face entry of each tile
+    const int *fL;            // [nfe] left cell (renumbered)
+    const int *fR;            // [nfe] right cell, or -(face kind) for a boundary face
+    const double *fnx, *fny, *fdl;  // unit normal (from L) and length
+    const double *fdx, *fdy;  // centre offset x_R - x_L (ghost: reflected centre)
+    const double2 *fwL, *fwR; // least-squares coefficient vectors of L and R for this face
+    const uint16_t *slots;    // [4*nc] (local face << 1) | (cell is R); 0xFFFF = none
+    const double *area, *inv_area;  // [nc]
+};
+
+// Per-launch scheme settings.
+struct DevCfg {
+    int bc[6];       // bc[kind] for kinds 1..5
+    double fs[4];    // prescribed state (rho, u, v, p)
+    double cfl;
+    int roe_bdiss;
+    int xcd_remap;   // map consecutive tiles onto one XCD
+};
+
+}  // namespace cfd

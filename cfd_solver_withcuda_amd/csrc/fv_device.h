@@ -1,0 +1,374 @@
+// Device-side FP64 physics of the explicit finite-volume path: primitive recovery, ghost
+// states, face fluxes (Roe + entropy fix, HLLC, LLF, AUSM), limited slopes.
+//
+// Every function keeps the operation order of the reference expression it restates, so that
+// with floating-point contraction disabled (-ffp-contract=off) each kernel reproduces the CPU
+// restatement bit for bit. References are to rameshkolluru43/CFD_Solver_withCUDA.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "cfd_types.h"
+
+namespace cfd {
+
+constexpr double kGamma = 1.4;
+constexpr double kGm1 = (kGamma - 1.0);
+constexpr double kGamma1 = (kGamma / kGm1);  // include/definitions.h:65 (gamma / gamma_M_1)
+
+// Face state: primitives (rho, u, v, p, a) and conserved (rho, rho u, rho v, rho E).
+struct FState {
+    double W[5];
+    double U[4];
+};
+
+// Calculate_Primitive_Variables (src/Basic_Functions.cpp:85-142): velocity flush |v|<=1e-10.
+__device__ __forceinline__ void state_from_U(double U0, double U1, double U2, double U3, FState &s) {
+    double Rho = U0;
+    double inv = 1.0 / Rho;
+    double v1 = U1 * inv;
+    double v2 = U2 * inv;
+    if (fabs(v1) <= 1e-10) v1 = 0.0;
+    if (fabs(v2) <= 1e-10) v2 = 0.0;
+    double vmag = (v1 * v1 + v2 * v2);
+    double P = kGm1 * (U3 - 0.5 * Rho * vmag);
+    s.W[0] = Rho;
+    s.W[1] = v1;
+    s.W[2] = v2;
+    s.W[3] = P;
+    s.W[4] = sqrt(kGamma * P * inv);
+    s.U[0] = U0;
+    s.U[1] = U1;
+    s.U[2] = U2;
+    s.U[3] = U3;
+}
+
+// Calculate_Computational_Variables case 2 (src/Basic_Functions.cpp:171-206).
+__device__ __forceinline__ void cons_from_prim(double rho, double u, double v, double p, double *U) {
+    double vmag = 0.5 * (u * u + v * v);
+    U[0] = rho;
+    U[1] = rho * u;
+    U[2] = rho * v;
+    U[3] = ((p / kGm1) + rho * vmag);
+}
+
+// Ghost state across a boundary face with outward unit normal (n1, n2).
+__device__ __forceinline__ void ghost_state(int bc, const FState &in, double n1, double n2,
+                                            const double *fs, FState &g) {
+    double Ug[4];
+    switch (bc) {
+    case BC_INLET_REF: {  // Supersonic_Inlet_Condition (src/Inlet_Boundary_Conditions.cpp:41-74)
+        double u = in.W[1], v = in.W[2];
+        double vmag = (u * u + v * v) * 0.5;
+        Ug[0] = fs[0];
+        Ug[1] = fs[0] * u;
+        Ug[2] = fs[0] * v;
+        Ug[3] = (fs[3] / kGm1) + fs[0] * vmag;
+        break;
+    }
+    case BC_SUBSONIC_INLET_REF: {  // Subsonic_Inlet_Condition (:7-39), energy as written there
+        double u = in.W[1], v = in.W[2];
+        double Vm = 0.5 * (u * u + v * v);
+        Ug[0] = fs[0];
+        Ug[1] = fs[0] * u;
+        Ug[2] = fs[0] * v;
+        Ug[3] = (fs[3] / kGm1) + fs[0] * Vm * Vm;
+        break;
+    }
+    case BC_WALL:
+    case BC_SYMMETRY: {  // InViscid_Wall / Symmetry (src/Wall_Boundary_Conditions.cpp:8-124)
+        double u = in.W[1], v = in.W[2];
+        double vdotn = (u * n1 + v * n2);
+        double v1g = u - 2.0 * vdotn * n1;
+        double v2g = v - 2.0 * vdotn * n2;
+        double vmag = 0.5 * (v1g * v1g + v2g * v2g);
+        Ug[0] = in.W[0];
+        Ug[1] = in.W[0] * v1g;
+        Ug[2] = in.W[0] * v2g;
+        Ug[3] = ((in.W[3] / kGm1) + in.W[0] * vmag);
+        break;
+    }
+    case BC_FREESTREAM:
+        cons_from_prim(fs[0], fs[1], fs[2], fs[3], Ug);
+        break;
+    case BC_FARFIELD: {  // characteristic far field (Riemann invariants normal to the face)
+        double ri = in.W[0], ui = in.W[1], vi = in.W[2], pi = in.W[3], ci = in.W[4];
+        double rf = fs[0], uf = fs[1], vf = fs[2], pf = fs[3];
+        double cf = sqrt(kGamma * pf / rf);
+        double vni = ui * n1 + vi * n2, vnf = uf * n1 + vf * n2;
+        double rb, ub, vb, pb;
+        if (fabs(vni) >= ci) {
+            if (vni < 0.0) { rb = rf; ub = uf; vb = vf; pb = pf; }
+            else { rb = ri; ub = ui; vb = vi; pb = pi; }
+        } else {
+            double Rp = vni + 2.0 * ci / kGm1;
+            double Rm = vnf - 2.0 * cf / kGm1;
+            double vnb = 0.5 * (Rp + Rm);
+            double cb = 0.25 * kGm1 * (Rp - Rm);
+            double s, ut, vt;
+            if (vnb < 0.0) {
+                s = pf / pow(rf, kGamma);
+                ut = uf - vnf * n1;
+                vt = vf - vnf * n2;
+            } else {
+                s = pi / pow(ri, kGamma);
+                ut = ui - vni * n1;
+                vt = vi - vni * n2;
+            }
+            rb = pow(cb * cb / (kGamma * s), 1.0 / kGm1);
+            pb = rb * cb * cb / kGamma;
+            ub = ut + vnb * n1;
+            vb = vt + vnb * n2;
+        }
+        cons_from_prim(rb, ub, vb, pb, Ug);
+        break;
+    }
+    case BC_EXIT:  // Supersonic_Exit_Boundary_Condition (src/Exit_Boundary_Conditions.cpp:68-81)
+    default:
+        g = in;
+        return;
+    }
+    state_from_U(Ug[0], Ug[1], Ug[2], Ug[3], g);
+}
+
+// Calculate_Face_Average_Flux (src/Average_Interface_Flux.cpp:20-82).
+__device__ __forceinline__ void avg_flux(const FState &L, const FState &R, double nx, double ny,
+                                         double dl, double *A) {
+    double Rho_L = L.W[0], u_L = L.W[1], v_L = L.W[2], P_L = L.W[3];
+    double Rho_R = R.W[0], u_R = R.W[1], v_R = R.W[2], P_R = R.W[3];
+    double Vdotn_L = (u_L * nx + v_L * ny);
+    double Vdotn_R = (u_R * nx + v_R * ny);
+    double Vmag_L = 0.5 * (u_L * u_L + v_L * v_L);
+    double Vmag_R = 0.5 * (u_R * u_R + v_R * v_R);
+    double Vdotn_L_dl = Vdotn_L * dl;
+    double Vdotn_R_dl = Vdotn_R * dl;
+    A[0] = 0.5 * (Rho_L * Vdotn_L_dl + Rho_R * Vdotn_R_dl);
+    A[1] = 0.5 * ((Rho_L * u_L * Vdotn_L_dl + P_L * nx * dl) + (Rho_R * u_R * Vdotn_R_dl + P_R * nx * dl));
+    A[2] = 0.5 * ((Rho_L * v_L * Vdotn_L_dl + P_L * ny * dl) + (Rho_R * v_R * Vdotn_R_dl + P_R * ny * dl));
+    A[3] = 0.5 * (((kGamma1 * P_L + Rho_L * Vmag_L) * Vdotn_L_dl) + ((kGamma1 * P_R + Rho_R * Vmag_R) * Vdotn_R_dl));
+}
+
+// ROE dissipation (src/Roe_Scheme.cpp:6-256), Harten entropy fix delta = 0.1 a~.
+__device__ __forceinline__ void roe_diss(const FState &L, const FState &R, double nx, double ny,
+                                         double dl, double *D) {
+    D[0] = D[1] = D[2] = D[3] = 0.0;
+    double normal_magnitude = sqrt(nx * nx + ny * ny);
+    if (normal_magnitude < 1e-12) return;
+    double Rho_L = L.W[0], u_L = L.W[1], v_L = L.W[2], P_L = L.W[3], C_L = L.W[4];
+    if (Rho_L <= 0.0 || P_L <= 0.0 || C_L <= 0.0) return;
+    double Rho_R = R.W[0], u_R = R.W[1], v_R = R.W[2], P_R = R.W[3], C_R = R.W[4];
+    if (Rho_R <= 0.0 || P_R <= 0.0 || C_R <= 0.0) return;
+    double Vmag_L = 0.5 * (u_L * u_L + v_L * v_L);
+    double H_L = ((C_L * C_L) / (kGamma - 1.0)) + Vmag_L;
+    double Vmag_R = 0.5 * (u_R * u_R + v_R * v_R);
+    double H_R = ((C_R * C_R) / (kGamma - 1.0)) + Vmag_R;
+    double sqrt_Rho_L = sqrt(Rho_L);
+    double sqrt_Rho_R = sqrt(Rho_R);
+    double Term1 = 1.0 / (sqrt_Rho_L + sqrt_Rho_R);
+    double Roe_Rho = sqrt(Rho_L * Rho_R);
+    double Roe_u = Term1 * (u_R * sqrt_Rho_R + u_L * sqrt_Rho_L);
+    double Roe_v = Term1 * (v_R * sqrt_Rho_R + v_L * sqrt_Rho_L);
+    double Roe_H = Term1 * (H_R * sqrt_Rho_R + H_L * sqrt_Rho_L);
+    double Roe_Vmag = 0.5 * (Roe_u * Roe_u + Roe_v * Roe_v);
+    double Roe_a = sqrt((kGamma - 1.0) * (Roe_H - Roe_Vmag));
+    if (Roe_a <= 0.0) {
+        Roe_a = 0.5 * (C_L + C_R);
+        if (Roe_a <= 0.0) return;
+    }
+    double Un = nx * Roe_u + ny * Roe_v;
+    double Ut = -ny * Roe_u + nx * Roe_v;
+    double du = (u_R - u_L), dv = (v_R - v_L), dP = (P_R - P_L), dRho = (Rho_R - Rho_L);
+    double dUn = nx * du + ny * dv;
+    double dUt = -ny * du + nx * dv;
+    double R12 = Roe_u - nx * Roe_a, R13 = Roe_v - ny * Roe_a, R14 = Roe_H - Roe_a * Un;
+    double R22 = Roe_u, R23 = Roe_v, R24 = Roe_Vmag;
+    double R32 = -ny, R33 = nx, R34 = Ut;
+    double R42 = Roe_u + nx * Roe_a, R43 = Roe_v + ny * Roe_a, R44 = Roe_H + Roe_a * Un;
+    double Term2 = 1.0 / (Roe_a * Roe_a);
+    double a1 = 0.5 * Term2 * (dP - Roe_Rho * Roe_a * dUn);
+    double a2 = dRho - Term2 * dP;
+    double a3 = Roe_Rho * dUt;
+    double a4 = 0.5 * Term2 * (dP + Roe_Rho * Roe_a * dUn);
+    double L1 = fabs(Un - Roe_a), L2 = fabs(Un), L3 = fabs(Un), L4 = fabs(Un + Roe_a);
+    double ef = 0.1 * Roe_a;
+    if (L1 < ef) L1 = 0.5 * (L1 * L1 / ef + ef);
+    if (L4 < ef) L4 = 0.5 * (L4 * L4 / ef + ef);
+    double t1 = L1 * a1, t2 = L2 * a2, t3 = L3 * a3, t4 = L4 * a4;
+    D[0] = 0.5 * (t1 * 1.0 + t2 * 1.0 + t3 * 0.0 + t4 * 1.0) * dl;
+    D[1] = 0.5 * (t1 * R12 + t2 * R22 + t3 * R32 + t4 * R42) * dl;
+    D[2] = 0.5 * (t1 * R13 + t2 * R23 + t3 * R33 + t4 * R43) * dl;
+    D[3] = 0.5 * (t1 * R14 + t2 * R24 + t3 * R34 + t4 * R44) * dl;
+}
+
+__device__ __forceinline__ double max3_ref(double a, double b, double c) {
+    if ((a >= b) && (a >= c)) return a;  // Maximum (src/Utility_Files.cpp:5-13)
+    if ((b >= a) && (b >= c)) return b;
+    return c;
+}
+
+// LLF dissipation (src/LLF.cpp:8-83).
+__device__ __forceinline__ void llf_diss(const FState &L, const FState &R, double nx, double ny,
+                                         double dl, double *D) {
+    double Vdotn_L = (L.W[1] * nx + L.W[2] * ny);
+    double Vdotn_R = (R.W[1] * nx + R.W[2] * ny);
+    double C_L = L.W[4], C_R = R.W[4];
+    double Max1 = max3_ref(fabs(Vdotn_L - C_L) * dl, fabs(Vdotn_L + C_L) * dl, fabs(Vdotn_L) * dl);
+    double Max2 = max3_ref(fabs(Vdotn_R - C_R) * dl, fabs(Vdotn_R + C_R) * dl, fabs(Vdotn_R) * dl);
+    double mev = (Max1 >= Max2) ? Max1 : Max2;
+    for (int k = 0; k < 4; k++) D[k] = 0.5 * mev * (R.U[k] - L.U[k]);
+}
+
+// HLLC (CUDA_KERNELS/HLLC_LLF_Flux_Cuda_Kernels.cu:25-208), times the face length.
+__device__ __forceinline__ void hllc_flux(const FState &L, const FState &R, double nx, double ny,
+                                          double dl, double *F) {
+    double rho_L = L.W[0], u_L = L.W[1], v_L = L.W[2], P_L = L.W[3], a_L = L.W[4];
+    double rho_R = R.W[0], u_R = R.W[1], v_R = R.W[2], P_R = R.W[3], a_R = R.W[4];
+    double un_L = nx * u_L + ny * v_L;
+    double un_R = nx * u_R + ny * v_R;
+    double sqrt_rho_L = sqrt(rho_L);
+    double sqrt_rho_R = sqrt(rho_R);
+    double a_roe = (a_L * sqrt_rho_L + a_R * sqrt_rho_R) / (sqrt_rho_L + sqrt_rho_R);
+    double u_roe = (u_L * sqrt_rho_L + u_R * sqrt_rho_R) / (sqrt_rho_L + sqrt_rho_R);
+    double v_roe = (v_L * sqrt_rho_L + v_R * sqrt_rho_R) / (sqrt_rho_L + sqrt_rho_R);
+    double un_roe = nx * u_roe + ny * v_roe;
+    double S_L = fmin(un_L - a_L, un_roe - a_roe);
+    double S_R = fmax(un_R + a_R, un_roe + a_roe);
+    double numer = P_R - P_L + rho_L * un_L * (S_L - un_L) - rho_R * un_R * (S_R - un_R);
+    double denom = rho_L * (S_L - un_L) - rho_R * (S_R - un_R);
+    double S_star = (fabs(denom) > 1e-14) ? numer / denom : 0.5 * (un_L + un_R);
+    double f[4];
+    if (S_L >= 0.0) {
+        f[0] = rho_L * un_L;
+        f[1] = rho_L * u_L * un_L + P_L * nx;
+        f[2] = rho_L * v_L * un_L + P_L * ny;
+        f[3] = (L.U[3] + P_L) * un_L;
+    } else if (S_R <= 0.0) {
+        f[0] = rho_R * un_R;
+        f[1] = rho_R * u_R * un_R + P_R * nx;
+        f[2] = rho_R * v_R * un_R + P_R * ny;
+        f[3] = (R.U[3] + P_R) * un_R;
+    } else {
+        const bool left = (S_star >= 0.0);
+        const FState &K = left ? L : R;
+        double S_K = left ? S_L : S_R;
+        double rho = K.W[0], u = K.W[1], v = K.W[2], P = K.W[3], E = K.U[3];
+        double un = nx * u + ny * v;
+        double factor = rho * (S_K - un) / (S_K - S_star);
+        double Us0 = factor;
+        double Us1 = factor * (u + (S_star - un) * nx);
+        double Us2 = factor * (v + (S_star - un) * ny);
+        double Us3 = factor * (E / rho + (S_star - un) * (S_star + P / (rho * (S_K - un))));
+        f[0] = rho * un + S_K * (Us0 - K.U[0]);
+        f[1] = (rho * u * un + P * nx) + S_K * (Us1 - K.U[1]);
+        f[2] = (rho * v * un + P * ny) + S_K * (Us2 - K.U[2]);
+        f[3] = ((E + P) * un) + S_K * (Us3 - K.U[3]);
+    }
+    for (int i = 0; i < 4; i++) F[i] = f[i] * dl;
+}
+
+// AUSM (src/Ausm_Flux.cpp:36-160), from the conserved variables.
+__device__ __forceinline__ void ausm_flux(const double *UL, const double *UR, double nx, double ny,
+                                          double face_area, double *F) {
+    double rho_L = UL[0], u_L = UL[1] / rho_L, v_L = UL[2] / rho_L, rhoE_L = UL[3];
+    double p_L = (kGamma - 1.0) * (rhoE_L - 0.5 * rho_L * (u_L * u_L + v_L * v_L));
+    double H_L = (rhoE_L + p_L) / rho_L;
+    double rho_R = UR[0], u_R = UR[1] / rho_R, v_R = UR[2] / rho_R, rhoE_R = UR[3];
+    double p_R = (kGamma - 1.0) * (rhoE_R - 0.5 * rho_R * (u_R * u_R + v_R * v_R));
+    double H_R = (rhoE_R + p_R) / rho_R;
+    double Vn_L = u_L * nx + v_L * ny;
+    double Vn_R = u_R * nx + v_R * ny;
+    double sqrt_rho_L = sqrt(rho_L);
+    double sqrt_rho_R = sqrt(rho_R);
+    double inv_sqrt_sum = 1.0 / (sqrt_rho_L + sqrt_rho_R);
+    double u_roe = (sqrt_rho_L * u_L + sqrt_rho_R * u_R) * inv_sqrt_sum;
+    double v_roe = (sqrt_rho_L * v_L + sqrt_rho_R * v_R) * inv_sqrt_sum;
+    double H_roe = (sqrt_rho_L * H_L + sqrt_rho_R * H_R) * inv_sqrt_sum;
+    double a_roe = sqrt((kGamma - 1.0) * (H_roe - 0.5 * (u_roe * u_roe + v_roe * v_roe)));
+    double M_L = Vn_L / a_roe;
+    double M_R = Vn_R / a_roe;
+    double M_plus = (fabs(M_L) >= 1.0) ? ((M_L > 0.0) ? M_L : 0.0) : 0.25 * (M_L + 1.0) * (M_L + 1.0);
+    double M_minus = (fabs(M_R) >= 1.0) ? ((M_R < 0.0) ? M_R : 0.0) : -0.25 * (M_R - 1.0) * (M_R - 1.0);
+    double M_interface = M_plus + M_minus;
+    double P_plus = (fabs(M_L) >= 1.0) ? ((M_L > 0.0) ? 1.0 : 0.0)
+                                       : 0.25 * (M_L + 1.0) * (M_L + 1.0) * (2.0 - M_L);
+    double P_minus = (fabs(M_R) >= 1.0) ? ((M_R < 0.0) ? 1.0 : 0.0)
+                                        : 0.25 * (M_R - 1.0) * (M_R - 1.0) * (2.0 + M_R);
+    double p_interface = P_plus * p_L + P_minus * p_R;
+    const bool up = (M_interface >= 0.0);
+    double mass_flux = M_interface * a_roe * (up ? rho_L : rho_R);
+    double uu = up ? u_L : u_R, vv = up ? v_L : v_R, HH = up ? H_L : H_R;
+    F[0] = mass_flux * face_area;
+    F[1] = (mass_flux * uu + p_interface * nx) * face_area;
+    F[2] = (mass_flux * vv + p_interface * ny) * face_area;
+    F[3] = mass_flux * HH * face_area;
+}
+
+template <int FLUX>
+__device__ __forceinline__ void face_flux(bool boundary, int roe_bdiss, const FState &L,
+                                          const FState &R, double nx, double ny, double dl,
+                                          double *F) {
+    if constexpr (FLUX == FLUX_ROE || FLUX == FLUX_LLF) {
+        double A[4], D[4];
+        avg_flux(L, R, nx, ny, dl, A);
+        if constexpr (FLUX == FLUX_ROE) {
+            if (boundary && !roe_bdiss) {
+                D[0] = D[1] = D[2] = D[3] = 0.0;
+            } else {
+                roe_diss(L, R, nx, ny, dl, D);
+            }
+        } else {
+            llf_diss(L, R, nx, ny, dl, D);
+        }
+        for (int k = 0; k < 4; k++) F[k] = A[k] - D[k];
+    } else if constexpr (FLUX == FLUX_HLLC) {
+        hllc_flux(L, R, nx, ny, dl, F);
+    } else {
+        ausm_flux(L.U, R.U, nx, ny, dl, F);
+    }
+}
+
+// phi(r) * dplus for r = dminus / dplus, without the division (limiter functions of
+// CUDA_KERNELS/Limiter_Cuda_Kernels.cu:89-157).
+template <int LIM>
+__device__ __forceinline__ double limited_slope(double a, double b) {
+    if constexpr (LIM == LIM_NONE) {
+        return 0.5 * (a + b);
+    } else if constexpr (LIM == LIM_MINMOD) {
+        if (a * b <= 0.0) return 0.0;
+        return (b > 0.0) ? fmin(a, b) : fmax(a, b);
+    } else if constexpr (LIM == LIM_VANLEER) {
+        if (a * b <= 0.0) return 0.0;
+        return 2.0 * a * b / (a + b);
+    } else {
+        if (a * b <= 0.0) return 0.0;
+        double aa = fabs(a), bb = fabs(b);
+        double s = fmax(fmin(2.0 * aa, bb), fmin(aa, 2.0 * bb));
+        return (b > 0.0) ? s : -s;
+    }
+}
+
+// MUSCL face state on one side: W_f = W_c + sgn/2 * slope(dminus, dplus) with
+// dplus = sgn (W_other - W_c), dminus = 2 grad(W_c).d - dplus; falls back to the cell state when
+// the reconstructed density or pressure is not positive.
+template <int LIM>
+__device__ __forceinline__ void reconstruct_side(const FState &c, const FState &o, const double *g,
+                                                 double dx, double dy, double sgn, FState &out) {
+    double Wf[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        double dplus = sgn * (o.W[k] - c.W[k]);
+        double dminus = 2.0 * (g[2 * k] * dx + g[2 * k + 1] * dy) - dplus;
+        Wf[k] = c.W[k] + sgn * 0.5 * limited_slope<LIM>(dminus, dplus);
+    }
+    if (!(Wf[0] > 0.0) || !(Wf[3] > 0.0)) {
+        out = c;
+        return;
+    }
+    out.W[0] = Wf[0];
+    out.W[1] = Wf[1];
+    out.W[2] = Wf[2];
+    out.W[3] = Wf[3];
+    out.W[4] = sqrt(kGamma * Wf[3] * (1.0 / Wf[0]));
+    cons_from_prim(Wf[0], Wf[1], Wf[2], Wf[3], out.U);
+}
+
+}  // namespace cfd

@@ -1,0 +1,43 @@
+// Launchers of the native finite-volume kernels (fv_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "cfd_types.h"
+
+namespace cfd {
+
+enum { MODE_RESID = 0, MODE_STAGE = 1 };
+
+// Arguments of the flux pass. MODE_RESID uses R_out/dt_out/dtmin_out; MODE_STAGE uses the rest.
+struct StageArgs {
+    int rk;      // 3 or 4
+    int stage;   // 0-based stage index
+    int lts;     // per-cell time step (stored in hbuf at stage 0) instead of *dt_global
+    int last;    // last stage: emit error partials / next-step dt
+    const double4 *Un;   // state at the start of the step
+    double4 *Uout;       // stage output
+    double4 *acc;        // RK4 accumulator
+    double *hbuf;        // per-cell dt/area (LTS)
+    const double *dt_global;
+    double *R_out;
+    double *dt_out;
+    double *dtmin_out;   // atomic minimum of the local time steps
+    double *err_partial; // [4*ntiles] per-tile sums of the reference error terms
+    const double4 *Uexit;  // exit ghosts copy this state when set (reference RK stages)
+};
+
+size_t grad_smem_bytes(const DevMesh &m);
+size_t flux_smem_bytes(const DevMesh &m, int order);
+hipError_t launch_grad(const DevMesh &m, const DevCfg &c, int grad_type, const double4 *Us,
+                       double *grad, int block, hipStream_t s);
+hipError_t launch_flux(const DevMesh &m, const DevCfg &c, int flux, int order, int lim, int mode,
+                       const double4 *Us, const double *grad, const StageArgs &st, int block,
+                       hipStream_t s);
+hipError_t launch_fill(double *p, double v, int n, hipStream_t s);
+hipError_t launch_err_finalize(const double *partial, int n, double *err4, hipStream_t s);
+hipError_t launch_ref_update(int nc, int kind, int lts, const double *inv_area, const double4 *Un,
+                             const double4 *Us, const double4 *R, const double *dt_cell,
+                             const double *dt_global, double4 *Uout, double *err_partial,
+                             hipStream_t s);
+
+}  // namespace cfd

@@ -1,0 +1,596 @@
+// Native HIP kernels of the explicit finite-volume path (gfx950, wave64, FP64).
+//
+// One workgroup owns one tile: a contiguous range of renumbered cells plus the list of faces
+// touching them. Each pass
+//   A) stages the tile's cell states in LDS (coalesced loads of the contiguous range),
+//   B) evaluates every face of the tile once (neighbour cells outside the tile come from
+//      global memory, i.e. L2) and writes the face result into LDS,
+//   C) gathers, per cell and in the cell's own face order, the face results with their sign.
+// The gather order is fixed, so results are deterministic and need no atomics; faces shared by
+// two tiles are evaluated by both with identical inputs, so the scheme stays exactly
+// conservative. Phase C of the flux pass also applies the Runge-Kutta stage update, so a
+// stage is one gradient pass plus one flux pass.
+#include <hip/hip_runtime.h>
+
+#include "fv_device.h"
+#include "fv_kernels.h"
+
+namespace cfd {
+
+namespace {
+
+__device__ __forceinline__ int tile_of_block(int b, int ntiles, int remap) {
+    if (!remap) return b;
+    // Workgroups are dispatched round-robin over the 8 XCDs (block b -> XCD b % 8); give each
+    // XCD a contiguous run of tiles so that faces shared by neighbouring tiles hit its L2.
+    const int per = (ntiles + 7) / 8;
+    return (b % 8) * per + b / 8;
+}
+
+__device__ __forceinline__ void atomic_min_pos(double *addr, double v) {
+    // non-negative doubles order like their bit patterns; NaN patterns never win
+    atomicMin(reinterpret_cast<unsigned long long *>(addr),
+              static_cast<unsigned long long>(__double_as_longlong(v)));
+}
+
+__device__ __forceinline__ void load_state(const double4 *__restrict__ U, int c, FState &s) {
+    double4 u = U[c];
+    state_from_U(u.x, u.y, u.z, u.w, s);
+}
+
+// Block-wide sum of 4 values per thread into out[0..3] (fixed tree order). `red` holds
+// 4 * blockDim.x doubles of LDS.
+__device__ __forceinline__ void block_sum4(double *red, const double *v, double *out) {
+    const int t = threadIdx.x, n = blockDim.x;
+    for (int k = 0; k < 4; k++) red[k * n + t] = v[k];
+    __syncthreads();
+    for (int s = n / 2; s > 0; s >>= 1) {
+        if (t < s)
+            for (int k = 0; k < 4; k++) red[k * n + t] += red[k * n + t + s];
+        __syncthreads();
+    }
+    if (t == 0)
+        for (int k = 0; k < 4; k++) out[k] = red[k * n];
+}
+
+__device__ __forceinline__ double block_min(double *red, double v) {
+    const int t = threadIdx.x, n = blockDim.x;
+    red[t] = v;
+    __syncthreads();
+    for (int s = n / 2; s > 0; s >>= 1) {
+        if (t < s) red[t] = fmin(red[t], red[t + s]);
+        __syncthreads();
+    }
+    return red[0];
+}
+
+// reference error term (src/Error_Estimate_Update.cpp:10-28)
+__device__ __forceinline__ double err_term(double d, double u) {
+    double t = (d < 1e-9) ? fabs(d) : (fabs(d) / u);
+    return t * t;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------
+// Gradient pass: Green-Gauss (reference semantics, ghost faces use the cell value) or weighted
+// least squares (ghost neighbours at the reflected centre with the boundary ghost state).
+// grad[8c..8c+7] = (d/dx, d/dy) of (rho, u, v, p).
+// ------------------------------------------------------------------------------------------
+template <int GRAD>
+__global__ void __launch_bounds__(256) grad_tile_kernel(DevMesh m, DevCfg cfg,
+                                                        const double4 *__restrict__ Us,
+                                                        double *__restrict__ grad) {
+    extern __shared__ double smem[];
+    const int tile = tile_of_block(blockIdx.x, m.ntiles, cfg.xcd_remap);
+    if (tile >= m.ntiles) return;
+    const int T = m.tile;
+    const int c0 = tile * T;
+    const int nct = min(T, m.nc - c0);
+    const int fs = m.tile_fstart[tile], nft = m.tile_fstart[tile + 1] - fs;
+    double *sW = smem;              // [5][T]
+    double *sX = smem + 5 * T;      // [8][max_tile_faces]
+    const int FT = m.max_tile_faces;
+
+    for (int k = threadIdx.x; k < nct; k += blockDim.x) {
+        FState s;
+        load_state(Us, c0 + k, s);
+        for (int q = 0; q < 5; q++) sW[q * T + k] = s.W[q];
+    }
+    __syncthreads();
+
+    for (int j = threadIdx.x; j < nft; j += blockDim.x) {
+        const int e = fs + j;
+        const int L = m.fL[e], R = m.fR[e];
+        const int lL = L - c0;
+        double WL[5], WR[5];
+        if ((unsigned)lL < (unsigned)nct) {
+            for (int q = 0; q < 5; q++) WL[q] = sW[q * T + lL];
+        } else {
+            FState s;
+            load_state(Us, L, s);
+            for (int q = 0; q < 5; q++) WL[q] = s.W[q];
+        }
+        if (R < 0) {
+            if constexpr (GRAD == GRAD_GG) {
+                for (int q = 0; q < 4; q++) WR[q] = WL[q];
+            } else {
+                FState in, g;
+                for (int q = 0; q < 5; q++) in.W[q] = WL[q];
+                for (int q = 0; q < 4; q++) in.U[q] = 0.0;
+                ghost_state(cfg.bc[-R], in, m.fnx[e], m.fny[e], cfg.fs, g);
+                for (int q = 0; q < 4; q++) WR[q] = g.W[q];
+            }
+        } else {
+            const int lR = R - c0;
+            if ((unsigned)lR < (unsigned)nct) {
+                for (int q = 0; q < 4; q++) WR[q] = sW[q * T + lR];
+            } else {
+                FState s;
+                load_state(Us, R, s);
+                for (int q = 0; q < 4; q++) WR[q] = s.W[q];
+            }
+        }
+        if constexpr (GRAD == GRAD_GG) {
+            const double nx = m.fnx[e], ny = m.fny[e], dl = m.fdl[e];
+            for (int q = 0; q < 4; q++) {
+                double phi_face = 0.5 * (WL[q] + WR[q]);
+                sX[(2 * q) * FT + j] = phi_face * nx * dl;
+                sX[(2 * q + 1) * FT + j] = phi_face * ny * dl;
+            }
+        } else {
+            for (int q = 0; q < 4; q++) sX[q * FT + j] = WR[q] - WL[q];
+            double2 a = m.fwL[e], b = (R < 0) ? make_double2(0.0, 0.0) : m.fwR[e];
+            sX[4 * FT + j] = a.x;
+            sX[5 * FT + j] = a.y;
+            sX[6 * FT + j] = b.x;
+            sX[7 * FT + j] = b.y;
+        }
+    }
+    __syncthreads();
+
+    for (int k = threadIdx.x; k < nct; k += blockDim.x) {
+        const int c = c0 + k;
+        double g[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int f = 0; f < 4; f++) {
+            const uint16_t sl = m.slots[4 * c + f];
+            if (sl == 0xFFFF) break;
+            const int j = sl >> 1;
+            const bool isR = sl & 1;
+            if constexpr (GRAD == GRAD_GG) {
+                for (int q = 0; q < 8; q++) {
+                    double v = sX[q * FT + j];
+                    g[q] += isR ? -v : v;
+                }
+            } else {
+                const double cx = isR ? sX[6 * FT + j] : sX[4 * FT + j];
+                const double cy = isR ? sX[7 * FT + j] : sX[5 * FT + j];
+                for (int q = 0; q < 4; q++) {
+                    double d = sX[q * FT + j];
+                    double dW = isR ? -d : d;
+                    g[2 * q] += cx * dW;
+                    g[2 * q + 1] += cy * dW;
+                }
+            }
+        }
+        if constexpr (GRAD == GRAD_GG) {
+            const double ia = m.inv_area[c];
+            for (int q = 0; q < 8; q++) g[q] = g[q] * ia;
+        }
+        double2 *out = reinterpret_cast<double2 *>(grad + 8 * (size_t)c);
+        out[0] = make_double2(g[0], g[1]);
+        out[1] = make_double2(g[2], g[3]);
+        out[2] = make_double2(g[4], g[5]);
+        out[3] = make_double2(g[6], g[7]);
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Flux pass. MODE_RESID writes R and the local time step; MODE_STAGE applies one stage of
+// TVD-RK3 / RK4 in place of writing R.
+// ------------------------------------------------------------------------------------------
+template <int FLUX, int ORDER, int LIM, int MODE>
+__global__ void __launch_bounds__(256) flux_tile_kernel(DevMesh m, DevCfg cfg,
+                                                        const double4 *__restrict__ Us,
+                                                        const double *__restrict__ grad,
+                                                        StageArgs st) {
+    extern __shared__ double smem[];
+    const int tile = tile_of_block(blockIdx.x, m.ntiles, cfg.xcd_remap);
+    if (tile >= m.ntiles) return;
+    const int T = m.tile;
+    const int FT = m.max_tile_faces;
+    const int c0 = tile * T;
+    const int nct = min(T, m.nc - c0);
+    const int fs = m.tile_fstart[tile], nft = m.tile_fstart[tile + 1] - fs;
+    double *sW = smem;            // [5][T]
+    double *sU = sW + 5 * T;      // [4][T]
+    double *sF = sU + 4 * T;      // [6][FT]: flux (4), spectral terms of L and R
+    double *sG = sF + 6 * FT;     // [8][T] (second order)
+
+    // ---- A: stage the tile's states --------------------------------------------------
+    for (int k = threadIdx.x; k < nct; k += blockDim.x) {
+        FState s;
+        load_state(Us, c0 + k, s);
+        for (int q = 0; q < 5; q++) sW[q * T + k] = s.W[q];
+        for (int q = 0; q < 4; q++) sU[q * T + k] = s.U[q];
+        if constexpr (ORDER == 2) {
+            const double2 *gp = reinterpret_cast<const double2 *>(grad + 8 * (size_t)(c0 + k));
+            for (int q = 0; q < 4; q++) {
+                double2 v = gp[q];
+                sG[(2 * q) * T + k] = v.x;
+                sG[(2 * q + 1) * T + k] = v.y;
+            }
+        }
+    }
+    __syncthreads();
+
+    // ---- B: one flux per face ----------------------------------------------------------
+    for (int j = threadIdx.x; j < nft; j += blockDim.x) {
+        const int e = fs + j;
+        const int L = m.fL[e], R = m.fR[e];
+        const double nx = m.fnx[e], ny = m.fny[e], dl = m.fdl[e];
+        const int lL = L - c0;
+        const bool locL = (unsigned)lL < (unsigned)nct;
+        FState SL, SR;
+        if (locL) {
+            for (int q = 0; q < 5; q++) SL.W[q] = sW[q * T + lL];
+            for (int q = 0; q < 4; q++) SL.U[q] = sU[q * T + lL];
+        } else {
+            load_state(Us, L, SL);
+        }
+        const bool boundary = R < 0;
+        int bc = 0;
+        bool locR = false;
+        int lR = 0;
+        if (boundary) {
+            bc = cfg.bc[-R];
+            if (bc == BC_EXIT && st.Uexit) {
+                FState src;
+                load_state(st.Uexit, L, src);
+                ghost_state(bc, src, nx, ny, cfg.fs, SR);
+            } else {
+                ghost_state(bc, SL, nx, ny, cfg.fs, SR);
+            }
+        } else {
+            lR = R - c0;
+            locR = (unsigned)lR < (unsigned)nct;
+            if (locR) {
+                for (int q = 0; q < 5; q++) SR.W[q] = sW[q * T + lR];
+                for (int q = 0; q < 4; q++) SR.U[q] = sU[q * T + lR];
+            } else {
+                load_state(Us, R, SR);
+            }
+        }
+        // spectral terms (|u.n| + a) dl of the two cell-centre states for the local time step
+        const double lamL = (fabs(SL.W[1] * nx + SL.W[2] * ny) + SL.W[4]) * dl;
+        const double lamR = boundary ? 0.0 : (fabs(SR.W[1] * nx + SR.W[2] * ny) + SR.W[4]) * dl;
+        double F[4];
+        if constexpr (ORDER == 2) {
+            const double dx = m.fdx[e], dy = m.fdy[e];
+            double gL[8];
+            if (locL) {
+                for (int q = 0; q < 8; q++) gL[q] = sG[q * T + lL];
+            } else {
+                const double2 *gp = reinterpret_cast<const double2 *>(grad + 8 * (size_t)L);
+                for (int q = 0; q < 4; q++) {
+                    double2 v = gp[q];
+                    gL[2 * q] = v.x;
+                    gL[2 * q + 1] = v.y;
+                }
+            }
+            FState fL, fR;
+            reconstruct_side<LIM>(SL, SR, gL, dx, dy, 1.0, fL);
+            if (boundary) {
+                if (bc == BC_EXIT && st.Uexit) {
+                    FState src;
+                    load_state(st.Uexit, L, src);
+                    ghost_state(bc, src, nx, ny, cfg.fs, fR);
+                } else {
+                    ghost_state(bc, fL, nx, ny, cfg.fs, fR);
+                }
+            } else {
+                double gR[8];
+                if (locR) {
+                    for (int q = 0; q < 8; q++) gR[q] = sG[q * T + lR];
+                } else {
+                    const double2 *gp = reinterpret_cast<const double2 *>(grad + 8 * (size_t)R);
+                    for (int q = 0; q < 4; q++) {
+                        double2 v = gp[q];
+                        gR[2 * q] = v.x;
+                        gR[2 * q + 1] = v.y;
+                    }
+                }
+                reconstruct_side<LIM>(SR, SL, gR, dx, dy, -1.0, fR);
+            }
+            face_flux<FLUX>(boundary, cfg.roe_bdiss, fL, fR, nx, ny, dl, F);
+        } else {
+            face_flux<FLUX>(boundary, cfg.roe_bdiss, SL, SR, nx, ny, dl, F);
+        }
+        for (int q = 0; q < 4; q++) sF[q * FT + j] = F[q];
+        sF[4 * FT + j] = lamL;
+        sF[5 * FT + j] = lamR;
+    }
+    __syncthreads();
+
+    // ---- C: gather per cell, time step, stage update -----------------------------------
+    double errp[4] = {0.0, 0.0, 0.0, 0.0};
+    double dtmin = 1.7976931348623157e308;
+    for (int k = threadIdx.x; k < nct; k += blockDim.x) {
+        const int c = c0 + k;
+        double r[4] = {0.0, 0.0, 0.0, 0.0};
+        double denom = 0.0;
+        for (int f = 0; f < 4; f++) {
+            const uint16_t sl = m.slots[4 * c + f];
+            if (sl == 0xFFFF) break;
+            const int j = sl >> 1;
+            const bool isR = sl & 1;
+            for (int q = 0; q < 4; q++) {
+                double v = sF[q * FT + j];
+                r[q] += isR ? -v : v;
+            }
+            denom += isR ? sF[5 * FT + j] : sF[4 * FT + j];
+        }
+        // Inviscid_Time_Step (src/Time_Step.cpp:293-329)
+        const double dtc = (denom <= 0.0 || !isfinite(denom)) ? 0.0 : cfg.cfl * m.area[c] / denom;
+        if constexpr (MODE == MODE_RESID) {
+            reinterpret_cast<double4 *>(st.R_out)[c] = make_double4(r[0], r[1], r[2], r[3]);
+            if (st.dt_out) st.dt_out[c] = dtc;
+            dtmin = fmin(dtmin, dtc);
+        } else {
+            double h;
+            if (st.lts) {
+                if (st.stage == 0) {
+                    h = dtc * m.inv_area[c];
+                    st.hbuf[c] = h;
+                } else {
+                    h = st.hbuf[c];
+                }
+            } else {
+                h = (*st.dt_global) * m.inv_area[c];
+            }
+            const double4 un4 = st.Un[c];
+            const double Un[4] = {un4.x, un4.y, un4.z, un4.w};
+            double Uo[4];
+            if (st.rk == 4) {
+                double4 a4 = (st.stage == 0) ? make_double4(0, 0, 0, 0) : st.acc[c];
+                double acc[4] = {a4.x, a4.y, a4.z, a4.w};
+                for (int q = 0; q < 4; q++) {
+                    switch (st.stage) {
+                    case 0: acc[q] = r[q]; Uo[q] = Un[q] - 0.5 * h * r[q]; break;
+                    case 1: acc[q] += 2.0 * r[q]; Uo[q] = Un[q] - 0.5 * h * r[q]; break;
+                    case 2: acc[q] += 2.0 * r[q]; Uo[q] = Un[q] - h * r[q]; break;
+                    default: Uo[q] = Un[q] - (h / 6.0) * (acc[q] + r[q]); break;
+                    }
+                }
+                if (st.stage < 3) st.acc[c] = make_double4(acc[0], acc[1], acc[2], acc[3]);
+            } else {
+                for (int q = 0; q < 4; q++) {
+                    const double us = sU[q * T + k];
+                    switch (st.stage) {
+                    case 0: Uo[q] = Un[q] - h * r[q]; break;
+                    case 1: Uo[q] = 0.75 * Un[q] + 0.25 * (us - h * r[q]); break;
+                    default: Uo[q] = (1.0 / 3.0) * Un[q] + (2.0 / 3.0) * (us - h * r[q]); break;
+                    }
+                }
+            }
+            st.Uout[c] = make_double4(Uo[0], Uo[1], Uo[2], Uo[3]);
+            if (st.last) {
+                if (st.err_partial) {
+                    for (int q = 0; q < 4; q++) errp[q] += err_term(Uo[q] - Un[q], Un[q]);
+                }
+                if (st.dtmin_out) {
+                    // time step of the new state, for the next step's global dt
+                    FState s;
+                    state_from_U(Uo[0], Uo[1], Uo[2], Uo[3], s);
+                    double den = 0.0;
+                    for (int f = 0; f < 4; f++) {
+                        const uint16_t sl = m.slots[4 * c + f];
+                        if (sl == 0xFFFF) break;
+                        const int e = fs + (sl >> 1);
+                        den += (fabs(s.W[1] * m.fnx[e] + s.W[2] * m.fny[e]) + s.W[4]) * m.fdl[e];
+                    }
+                    const double dtn = (den <= 0.0 || !isfinite(den)) ? 0.0 : cfg.cfl * m.area[c] / den;
+                    dtmin = fmin(dtmin, dtn);
+                }
+            }
+        }
+    }
+    // block reductions reuse LDS from its base (the allocation holds >= 4 * blockDim doubles)
+    if constexpr (MODE == MODE_RESID) {
+        if (st.dtmin_out) {
+            __syncthreads();
+            double b = block_min(smem, dtmin);
+            if (threadIdx.x == 0) atomic_min_pos(st.dtmin_out, b);
+        }
+    } else {
+        if (st.last && st.err_partial) {
+            __syncthreads();
+            double out[4];
+            block_sum4(smem, errp, out);
+            if (threadIdx.x == 0)
+                for (int q = 0; q < 4; q++) st.err_partial[4 * tile + q] = out[q];
+            __syncthreads();
+        }
+        if (st.last && st.dtmin_out) {
+            __syncthreads();
+            double b = block_min(smem, dtmin);
+            if (threadIdx.x == 0) atomic_min_pos(st.dtmin_out, b);
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Small kernels
+// ------------------------------------------------------------------------------------------
+__global__ void fill_kernel(double *p, double v, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Sums the per-tile error partials in tile order and takes the square root (Estimate_Error).
+__global__ void err_finalize_kernel(const double *partial, int ntiles, double *err4) {
+    __shared__ double red[4 * 256];
+    const int t = threadIdx.x;
+    double v[4] = {0, 0, 0, 0};
+    for (int i = t; i < ntiles; i += blockDim.x)
+        for (int q = 0; q < 4; q++) v[q] += partial[4 * i + q];
+    for (int q = 0; q < 4; q++) red[q * 256 + t] = v[q];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (t < s)
+            for (int q = 0; q < 4; q++) red[q * 256 + t] += red[q * 256 + t + s];
+        __syncthreads();
+    }
+    if (t == 0)
+        for (int q = 0; q < 4; q++) err4[q] = sqrt(red[q * 256]);
+}
+
+// The reference's Runge_Kutta_Method / Explicit_Method updates (src/Numerical_Method.cpp:26-103,
+// 342-457) with the stage's global dt (device scalar) and, in stage 0 with LTS, the cell's dt.
+// kind 0: U1 = Un - dt/A R; 1: U2 = 1/4 (U1 + 3 Un - dt/A R); 2: dU = 2/3 (U2 - Un - dt/A R),
+// Uout = Un + dU; 3 (forward Euler): dU = -dt/A R, Uout = Un + dU. Kinds 2 and 3 also emit the
+// error partials of dU.
+__global__ void __launch_bounds__(256) ref_update_kernel(int nc, int kind, int lts,
+                                                         const double *__restrict__ inv_area,
+                                                         const double4 *__restrict__ Un,
+                                                         const double4 *__restrict__ Us,
+                                                         const double4 *__restrict__ R,
+                                                         const double *__restrict__ dt_cell,
+                                                         const double *__restrict__ dt_global,
+                                                         double4 *__restrict__ Uout,
+                                                         double *__restrict__ err_partial) {
+    __shared__ double red[4 * 256];
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    double e[4] = {0, 0, 0, 0};
+    if (c < nc) {
+        const double dtg = *dt_global;
+        const double ia = inv_area[c];
+        const double4 n4 = Un[c], s4 = Us[c], r4 = R[c];
+        const double un[4] = {n4.x, n4.y, n4.z, n4.w};
+        const double us[4] = {s4.x, s4.y, s4.z, s4.w};
+        const double r[4] = {r4.x, r4.y, r4.z, r4.w};
+        double o[4];
+        for (int q = 0; q < 4; q++) {
+            switch (kind) {
+            case 0: {
+                const double d = lts ? dt_cell[c] : dtg;
+                o[q] = un[q] - d * ia * (r[q] - 0.0);
+                break;
+            }
+            case 1:
+                o[q] = 0.25 * (us[q] + 3.0 * un[q] - dtg * ia * (r[q] - 0.0));
+                break;
+            case 2: {
+                const double dU = (2.0 / 3.0) * (us[q] - un[q] - dtg * ia * (r[q] - 0.0));
+                e[q] = err_term(dU, un[q]);
+                o[q] = un[q] + dU;
+                break;
+            }
+            default: {
+                const double dU = -dtg * ia * r[q];
+                e[q] = err_term(dU, un[q]);
+                o[q] = un[q] + dU;
+                break;
+            }
+            }
+        }
+        Uout[c] = make_double4(o[0], o[1], o[2], o[3]);
+    }
+    if (kind >= 2) {
+        double out[4];
+        block_sum4(red, e, out);
+        if (threadIdx.x == 0)
+            for (int q = 0; q < 4; q++) err_partial[4 * blockIdx.x + q] = out[q];
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Host-side launchers
+// ------------------------------------------------------------------------------------------
+size_t grad_smem_bytes(const DevMesh &m) {
+    return sizeof(double) * (5 * (size_t)m.tile + 8 * (size_t)m.max_tile_faces);
+}
+
+size_t flux_smem_bytes(const DevMesh &m, int order) {
+    size_t d = 5 * (size_t)m.tile + 4 * (size_t)m.tile + 6 * (size_t)m.max_tile_faces;
+    if (order == 2) d += 8 * (size_t)m.tile;
+    // the block reductions of phase C need 4 doubles per thread (blockDim <= 256)
+    if (d < 4 * 256) d = 4 * 256;
+    return sizeof(double) * d;
+}
+
+static int grid_of(const DevMesh &m, const DevCfg &c) {
+    return c.xcd_remap ? 8 * ((m.ntiles + 7) / 8) : m.ntiles;
+}
+
+hipError_t launch_grad(const DevMesh &m, const DevCfg &c, int grad_type, const double4 *Us,
+                       double *grad, int block, hipStream_t s) {
+    const size_t sm = grad_smem_bytes(m);
+    if (grad_type == GRAD_GG)
+        hipLaunchKernelGGL(grad_tile_kernel<GRAD_GG>, dim3(grid_of(m, c)), dim3(block), sm, s, m, c, Us, grad);
+    else
+        hipLaunchKernelGGL(grad_tile_kernel<GRAD_WLSQ>, dim3(grid_of(m, c)), dim3(block), sm, s, m, c, Us, grad);
+    return hipGetLastError();
+}
+
+template <int FLUX, int ORDER, int LIM>
+static void launch_flux_t(const DevMesh &m, const DevCfg &c, int mode, const double4 *Us,
+                          const double *grad, const StageArgs &st, int block, size_t sm,
+                          hipStream_t s) {
+    if (mode == MODE_RESID)
+        hipLaunchKernelGGL((flux_tile_kernel<FLUX, ORDER, LIM, MODE_RESID>), dim3(grid_of(m, c)), dim3(block), sm, s, m, c, Us, grad, st);
+    else
+        hipLaunchKernelGGL((flux_tile_kernel<FLUX, ORDER, LIM, MODE_STAGE>), dim3(grid_of(m, c)), dim3(block), sm, s, m, c, Us, grad, st);
+}
+
+template <int FLUX>
+static void launch_flux_f(const DevMesh &m, const DevCfg &c, int order, int lim, int mode,
+                          const double4 *Us, const double *grad, const StageArgs &st, int block,
+                          size_t sm, hipStream_t s) {
+    if (order == 1) {
+        launch_flux_t<FLUX, 1, LIM_NONE>(m, c, mode, Us, grad, st, block, sm, s);
+        return;
+    }
+    switch (lim) {
+    case LIM_NONE: launch_flux_t<FLUX, 2, LIM_NONE>(m, c, mode, Us, grad, st, block, sm, s); break;
+    case LIM_MINMOD: launch_flux_t<FLUX, 2, LIM_MINMOD>(m, c, mode, Us, grad, st, block, sm, s); break;
+    case LIM_VANLEER: launch_flux_t<FLUX, 2, LIM_VANLEER>(m, c, mode, Us, grad, st, block, sm, s); break;
+    default: launch_flux_t<FLUX, 2, LIM_SUPERBEE>(m, c, mode, Us, grad, st, block, sm, s); break;
+    }
+}
+
+hipError_t launch_flux(const DevMesh &m, const DevCfg &c, int flux, int order, int lim, int mode,
+                       const double4 *Us, const double *grad, const StageArgs &st, int block,
+                       hipStream_t s) {
+    const size_t sm = flux_smem_bytes(m, order);
+    switch (flux) {
+    case FLUX_ROE: launch_flux_f<FLUX_ROE>(m, c, order, lim, mode, Us, grad, st, block, sm, s); break;
+    case FLUX_HLLC: launch_flux_f<FLUX_HLLC>(m, c, order, lim, mode, Us, grad, st, block, sm, s); break;
+    case FLUX_LLF: launch_flux_f<FLUX_LLF>(m, c, order, lim, mode, Us, grad, st, block, sm, s); break;
+    default: launch_flux_f<FLUX_AUSM>(m, c, order, lim, mode, Us, grad, st, block, sm, s); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_fill(double *p, double v, int n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, v, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_err_finalize(const double *partial, int n, double *err4, hipStream_t s) {
+    hipLaunchKernelGGL(err_finalize_kernel, dim3(1), dim3(256), 0, s, partial, n, err4);
+    return hipGetLastError();
+}
+
+hipError_t launch_ref_update(int nc, int kind, int lts, const double *inv_area, const double4 *Un,
+                             const double4 *Us, const double4 *R, const double *dt_cell,
+                             const double *dt_global, double4 *Uout, double *err_partial,
+                             hipStream_t s) {
+    if (nc <= 0) return hipSuccess;
+    hipLaunchKernelGGL(ref_update_kernel, dim3((nc + 255) / 256), dim3(256), 0, s, nc, kind, lts,
+                       inv_area, Un, Us, R, dt_cell, dt_global, Uout, err_partial);
+    return hipGetLastError();
+}
+
+}  // namespace cfd

@@ -1,0 +1,370 @@
+// Host-side mesh processing (see mesh_host.h). Compiled without floating-point contraction so
+// that geometry is bit-identical to the reference's x86-64 build.
+#include "mesh_host.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <stdexcept>
+
+namespace cfd {
+
+namespace {
+
+constexpr double kBboxEps = 1e-6;  // EPSILON, include/definitions.h:70
+
+// Sort_Points_AntiClockWise(V, idx) (src/Grid_Computations.cpp:1092-1142): atan2 about the
+// vertex average; already-ascending input is kept; otherwise std::sort with the reference's
+// comparator, which for <= 16 elements is a stable insertion sort.
+void sort_ccw(int n, double *v, int *idx) {
+    double x = 0.0, y = 0.0;
+    for (int i = 0; i < n; i++) {
+        x += v[3 * i];
+        y += v[3 * i + 1];
+    }
+    const double cx = x / n, cy = y / n;
+    bool sorted = true;
+    for (int i = 1; i < n && sorted; i++) {
+        double ap = std::atan2(v[3 * (i - 1) + 1] - cy, v[3 * (i - 1)] - cx);
+        double ac = std::atan2(v[3 * i + 1] - cy, v[3 * i] - cx);
+        if (ap > ac) sorted = false;
+    }
+    if (sorted) return;
+    auto less = [&](const double *A, const double *B) {
+        double aA = std::atan2(A[1] - cy, A[0] - cx);
+        double aB = std::atan2(B[1] - cy, B[0] - cx);
+        if (std::fabs(aA - aB) < 1e-9) return A[2] < B[2];
+        return aA < aB;
+    };
+    for (int i = 1; i < n; i++) {
+        double tv[3] = {v[3 * i], v[3 * i + 1], v[3 * i + 2]};
+        int ti = idx[i];
+        int j = i;
+        while (j > 0 && less(tv, &v[3 * (j - 1)])) {
+            std::memcpy(&v[3 * j], &v[3 * (j - 1)], 3 * sizeof(double));
+            idx[j] = idx[j - 1];
+            j--;
+        }
+        std::memcpy(&v[3 * j], tv, 3 * sizeof(double));
+        idx[j] = ti;
+    }
+}
+
+struct Edge {
+    int a, b, cell, slot;
+};
+
+uint64_t hilbert_d(uint32_t x, uint32_t y, int order) {
+    uint64_t d = 0;
+    for (uint32_t s = 1u << (order - 1); s > 0; s >>= 1) {
+        uint32_t rx = (x & s) > 0, ry = (y & s) > 0;
+        d += (uint64_t)s * s * ((3 * rx) ^ ry);
+        if (ry == 0) {
+            if (rx == 1) {
+                x = s - 1 - x;
+                y = s - 1 - y;
+            }
+            std::swap(x, y);
+        }
+    }
+    return d;
+}
+
+}  // namespace
+
+void build_ref_mesh(int npts, const double *pts, int nc, const int *cells4, RefMesh &m) {
+    m = RefMesh();
+    m.np = npts;
+    m.nc = nc;
+    m.nf.assign(nc, 0);
+    m.nodes.assign(4 * (size_t)nc, -1);
+    m.nbr.assign(4 * (size_t)nc, -1);
+    m.kind.assign(4 * (size_t)nc, 0);
+    m.vtx.assign(12 * (size_t)nc, 0.0);
+    m.nrm.assign(8 * (size_t)nc, 0.0);
+    m.len.assign(4 * (size_t)nc, 0.0);
+    m.dist.assign(4 * (size_t)nc, 0.0);
+    std::vector<double> ctr(3 * (size_t)nc), area(nc), inv(nc);
+    for (int c = 0; c < nc; c++) {
+        const int n = cells4[4 * c + 3] < 0 ? 3 : 4;
+        m.nf[c] = n;
+        double *v = &m.vtx[12 * (size_t)c];
+        int *id = &m.nodes[4 * (size_t)c];
+        for (int i = 0; i < n; i++) {
+            id[i] = cells4[4 * c + i];
+            if (id[i] < 0 || id[i] >= npts) throw std::runtime_error("cell node index out of range");
+            v[3 * i] = pts[3 * id[i]];
+            v[3 * i + 1] = pts[3 * id[i] + 1];
+            v[3 * i + 2] = pts[3 * id[i] + 2];
+        }
+        sort_ccw(n, v, id);
+        double x = 0.0, y = 0.0, z = 0.0;  // Compute_Centroid (:957-980)
+        for (int i = 0; i < n; i++) {
+            x += v[3 * i];
+            y += v[3 * i + 1];
+            z += v[3 * i + 2];
+        }
+        ctr[3 * c] = x / n;
+        ctr[3 * c + 1] = y / n;
+        ctr[3 * c + 2] = z / n;
+        for (int f = 0; f < n; f++) {  // Construct_Face (:691-752)
+            const double *a = &v[3 * f], *b = &v[3 * ((f + 1) % n)];
+            double dy = b[1] - a[1];
+            double dx = b[0] - a[0];
+            double dL = std::sqrt(dx * dx + dy * dy);
+            if (dL == 0.0) throw std::runtime_error("zero-length face");
+            m.len[4 * c + f] = dL;
+            m.nrm[8 * c + 2 * f] = dy / dL;
+            m.nrm[8 * c + 2 * f + 1] = -dx / dL;
+        }
+        double twice = 0.0;  // shoelace (:541-557)
+        for (int i = 0; i < n; i++) {
+            int j = (i + 1) % n;
+            twice += (v[3 * i] * v[3 * j + 1] - v[3 * j] * v[3 * i + 1]);
+        }
+        area[c] = 0.5 * std::fabs(twice);
+        if (!(area[c] > 0.0) || !std::isfinite(area[c])) throw std::runtime_error("invalid cell area");
+        inv[c] = 1.0 / area[c];
+    }
+    // mapFacesToCells + fillFaceOrderedNeighbours (src/Read_Gmsh_File.cpp:325-402)
+    std::vector<Edge> E;
+    E.reserve(4 * (size_t)nc);
+    for (int c = 0; c < nc; c++) {
+        const int n = m.nf[c];
+        for (int f = 0; f < n; f++) {
+            int a = m.nodes[4 * c + f], b = m.nodes[4 * c + (f + 1) % n];
+            if (a > b) std::swap(a, b);
+            E.push_back({a, b, c, f});
+        }
+    }
+    std::sort(E.begin(), E.end(), [](const Edge &p, const Edge &q) {
+        if (p.a != q.a) return p.a < q.a;
+        if (p.b != q.b) return p.b < q.b;
+        if (p.cell != q.cell) return p.cell < q.cell;
+        return p.slot < q.slot;
+    });
+    for (size_t i = 0; i < E.size();) {
+        size_t j = i;
+        while (j < E.size() && E[j].a == E[i].a && E[j].b == E[i].b) j++;
+        int distinct = 0, c0 = -1, c1 = -1;
+        for (size_t q = i; q < j; q++)
+            if (E[q].cell != c0 && E[q].cell != c1) {
+                if (distinct == 0) c0 = E[q].cell;
+                else if (distinct == 1) c1 = E[q].cell;
+                distinct++;
+            }
+        if (distinct == 2)
+            for (size_t q = i; q < j; q++) m.nbr[4 * (size_t)E[q].cell + E[q].slot] = (E[q].cell == c0) ? c1 : c0;
+        i = j;
+    }
+    int ghost = nc;
+    std::vector<char> hasb(nc, 0);
+    for (int c = 0; c < nc; c++)
+        for (int f = 0; f < m.nf[c]; f++)
+            if (m.nbr[4 * c + f] < 0) {
+                m.nbr[4 * c + f] = ghost++;
+                hasb[c] = 1;
+            }
+    const int ng = ghost - nc;
+    m.ng = ng;
+    // BoundingBox (:441-472)
+    double xmin = 0, xmax = 0, ymin = 0, ymax = 0;
+    for (int i = 0; i < npts; i++) {
+        double x = pts[3 * i], y = pts[3 * i + 1];
+        if (i == 0) {
+            xmin = xmax = x;
+            ymin = ymax = y;
+        } else {
+            xmin = x < xmin ? x : xmin;
+            xmax = x > xmax ? x : xmax;
+            ymin = y < ymin ? y : ymin;
+            ymax = y > ymax ? y : ymax;
+        }
+    }
+    // Classify_Domain_Boundaries (:490-577)
+    for (int c = 0; c < nc; c++) {
+        const int n = m.nf[c];
+        for (int e = 0; e < n; e++) {
+            const double *p1 = &m.vtx[12 * c + 3 * e], *p2 = &m.vtx[12 * c + 3 * ((e + 1) % n)];
+            int kd = 0;
+            if (hasb[c]) {
+                if (std::fabs(p1[0] - xmin) < kBboxEps && std::fabs(p2[0] - xmin) < kBboxEps) kd = 1;
+                else if (std::fabs(p1[0] - xmax) < kBboxEps && std::fabs(p2[0] - xmax) < kBboxEps) kd = 2;
+                else if (std::fabs(p1[1] - ymin) < kBboxEps && std::fabs(p2[1] - ymin) < kBboxEps) kd = 4;
+                else if (std::fabs(p1[1] - ymax) < kBboxEps && std::fabs(p2[1] - ymax) < kBboxEps) kd = 3;
+                else kd = 5;
+            }
+            m.kind[4 * c + e] = kd;
+        }
+    }
+    // Create_Boundary_Cells_Lists (:579-621)
+    m.gcell.assign(ng, 0);
+    m.gface.assign(ng, 0);
+    for (int c = 0; c < nc; c++)
+        for (int f = 0; f < m.nf[c]; f++) {
+            int g = m.nbr[4 * c + f];
+            if (g < nc) continue;
+            m.gcell[g - nc] = c;
+            m.gface[g - nc] = f;
+            int kd = m.kind[4 * c + f];
+            std::vector<int> *L = kd == 1 ? &m.inl : kd == 2 ? &m.exl : (kd >= 3 ? &m.wal : nullptr);
+            if (L) {
+                L->push_back(c);
+                L->push_back(f);
+                L->push_back(g);
+            }
+        }
+    // Construct_Ghost_Cells (:394-442, :589-629)
+    const size_t nt = (size_t)nc + ng;
+    m.ctr.assign(3 * nt, 0.0);
+    m.area.assign(nt, 0.0);
+    m.inv_area.assign(nt, 0.0);
+    std::copy(ctr.begin(), ctr.end(), m.ctr.begin());
+    std::copy(area.begin(), area.end(), m.area.begin());
+    std::copy(inv.begin(), inv.end(), m.inv_area.begin());
+    for (int gi = 0; gi < ng; gi++) {
+        const int c = m.gcell[gi], f = m.gface[gi], g = nc + gi, nv = m.nf[c];
+        const double *a = &m.vtx[12 * c + 3 * (f % nv)], *b = &m.vtx[12 * c + 3 * ((f + 1) % nv)];
+        for (int q = 0; q < 3; q++) {
+            double mid = 0.5 * (a[q] + b[q]);
+            m.ctr[3 * (size_t)g + q] = m.ctr[3 * c + q] + 2.0 * (mid - m.ctr[3 * c + q]);
+        }
+        m.area[g] = m.area[c];
+        m.inv_area[g] = m.inv_area[c];
+    }
+    // Calculate_Cell_Center_Distances (:802-827)
+    for (int c = 0; c < nc; c++)
+        for (int f = 0; f < m.nf[c]; f++) {
+            const int j = m.nbr[4 * c + f];
+            double D = 0.0;
+            for (int q = 0; q < 3; q++) {
+                double t = m.ctr[3 * (size_t)j + q] - m.ctr[3 * c + q];
+                D += t * t;
+            }
+            m.dist[4 * c + f] = std::sqrt(D);
+        }
+}
+
+void wlsq_coefficients(const RefMesh &m, std::vector<double> &CX, std::vector<double> &CY) {
+    CX.assign(4 * (size_t)m.nc, 0.0);
+    CY.assign(4 * (size_t)m.nc, 0.0);
+    for (int c = 0; c < m.nc; c++) {
+        double a = 0.0, b = 0.0, d = 0.0;
+        const int n = m.nf[c];
+        double dx[4], dy[4], w[4];
+        for (int f = 0; f < n; f++) {
+            const int j = m.nbr[4 * c + f];
+            dx[f] = m.ctr[3 * (size_t)j] - m.ctr[3 * c];
+            dy[f] = m.ctr[3 * (size_t)j + 1] - m.ctr[3 * c + 1];
+            w[f] = 1.0 / (dx[f] * dx[f] + dy[f] * dy[f]);
+            a += w[f] * dx[f] * dx[f];
+            b += w[f] * dx[f] * dy[f];
+            d += w[f] * dy[f] * dy[f];
+        }
+        const double det = a * d - b * b;
+        const double i11 = d / det, i12 = -b / det, i22 = a / det;
+        for (int f = 0; f < n; f++) {
+            CX[4 * c + f] = (i11 * dx[f] + i12 * dy[f]) * w[f];
+            CY[4 * c + f] = (i12 * dx[f] + i22 * dy[f]) * w[f];
+        }
+    }
+}
+
+void build_layout(const RefMesh &m, int tile, int renumber, Layout &L) {
+    if (tile < 16 || tile > 4096) throw std::runtime_error("tile size out of range [16, 4096]");
+    L = Layout();
+    const int nc = m.nc;
+    L.tile = tile;
+    L.perm.resize(nc);
+    std::iota(L.perm.begin(), L.perm.end(), 0);
+    if (renumber == 1 && nc > 0) {
+        double xmin = m.ctr[0], xmax = m.ctr[0], ymin = m.ctr[1], ymax = m.ctr[1];
+        for (int c = 0; c < nc; c++) {
+            xmin = std::min(xmin, m.ctr[3 * c]);
+            xmax = std::max(xmax, m.ctr[3 * c]);
+            ymin = std::min(ymin, m.ctr[3 * c + 1]);
+            ymax = std::max(ymax, m.ctr[3 * c + 1]);
+        }
+        const double span = std::max(xmax - xmin, ymax - ymin);
+        const double s = span > 0 ? 65535.0 / span : 0.0;
+        std::vector<uint64_t> key(nc);
+        for (int c = 0; c < nc; c++) {
+            uint32_t x = (uint32_t)std::lround((m.ctr[3 * c] - xmin) * s);
+            uint32_t y = (uint32_t)std::lround((m.ctr[3 * c + 1] - ymin) * s);
+            key[c] = hilbert_d(x, y, 16);
+        }
+        std::stable_sort(L.perm.begin(), L.perm.end(), [&](int a, int b) { return key[a] < key[b]; });
+    }
+    L.iperm.assign(nc, 0);
+    for (int k = 0; k < nc; k++) L.iperm[L.perm[k]] = k;
+    L.area.resize(nc);
+    L.inv_area.resize(nc);
+    for (int k = 0; k < nc; k++) {
+        L.area[k] = m.area[L.perm[k]];
+        L.inv_area[k] = m.inv_area[L.perm[k]];
+    }
+    // unique faces: canonical side = lower original index (or the owner of a boundary face)
+    std::vector<int> face_of(4 * (size_t)nc, -1), canon_cell, canon_slot;
+    for (int c = 0; c < nc; c++)
+        for (int f = 0; f < m.nf[c]; f++) {
+            const int j = m.nbr[4 * c + f];
+            if (j < nc && j < c) continue;
+            const int id = (int)canon_cell.size();
+            canon_cell.push_back(c);
+            canon_slot.push_back(f);
+            face_of[4 * c + f] = id;
+            if (j < nc)
+                for (int s = 0; s < m.nf[j]; s++)
+                    if (m.nbr[4 * j + s] == c) face_of[4 * (size_t)j + s] = id;
+        }
+    L.nfaces = (int)canon_cell.size();
+    std::vector<double> CX, CY;
+    wlsq_coefficients(m, CX, CY);
+    L.ntiles = (nc + tile - 1) / tile;
+    L.tile_fstart.assign(L.ntiles + 1, 0);
+    L.slots.assign(4 * (size_t)nc, 0xFFFF);
+    std::vector<int> stamp(L.nfaces, -1), local(L.nfaces, 0);
+    for (int t = 0; t < L.ntiles; t++) {
+        L.tile_fstart[t] = (int)L.fL.size();
+        const int k0 = t * tile, k1 = std::min(nc, k0 + tile);
+        int nloc = 0;
+        for (int k = k0; k < k1; k++) {
+            const int c = L.perm[k];
+            for (int f = 0; f < m.nf[c]; f++) {
+                const int id = face_of[4 * (size_t)c + f];
+                if (stamp[id] != t) {
+                    stamp[id] = t;
+                    local[id] = nloc++;
+                    const int a = canon_cell[id], s = canon_slot[id];
+                    const int j = m.nbr[4 * (size_t)a + s];
+                    L.fL.push_back(L.iperm[a]);
+                    L.fR.push_back(j < nc ? L.iperm[j] : -m.kind[4 * (size_t)a + s]);
+                    L.fnx.push_back(m.nrm[8 * (size_t)a + 2 * s]);
+                    L.fny.push_back(m.nrm[8 * (size_t)a + 2 * s + 1]);
+                    L.fdl.push_back(m.len[4 * (size_t)a + s]);
+                    L.fdx.push_back(m.ctr[3 * (size_t)j] - m.ctr[3 * (size_t)a]);
+                    L.fdy.push_back(m.ctr[3 * (size_t)j + 1] - m.ctr[3 * (size_t)a + 1]);
+                    L.fwL.push_back(CX[4 * (size_t)a + s]);
+                    L.fwL.push_back(CY[4 * (size_t)a + s]);
+                    double rx = 0.0, ry = 0.0;
+                    if (j < nc)
+                        for (int q = 0; q < m.nf[j]; q++)
+                            if (m.nbr[4 * (size_t)j + q] == a) {
+                                rx = CX[4 * (size_t)j + q];
+                                ry = CY[4 * (size_t)j + q];
+                            }
+                    L.fwR.push_back(rx);
+                    L.fwR.push_back(ry);
+                }
+                const bool isR = canon_cell[id] != c;
+                if (local[id] >= 32767) throw std::runtime_error("too many faces in one tile");
+                L.slots[4 * (size_t)k + f] = (uint16_t)((local[id] << 1) | (isR ? 1 : 0));
+            }
+        }
+        L.max_tile_faces = std::max(L.max_tile_faces, nloc);
+    }
+    L.tile_fstart[L.ntiles] = (int)L.fL.size();
+}
+
+}  // namespace cfd

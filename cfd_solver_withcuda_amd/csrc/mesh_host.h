@@ -1,0 +1,46 @@
+// Host-side mesh processing: the reference's connectivity and geometry, and the tiled,
+// renumbered layout the kernels use.
+#pragma once
+#include <cstdint>
+#include <vector>
+
+namespace cfd {
+
+// Connectivity and geometry in the reference's cell order and layout (4 face slots per cell),
+// as Read_VTK_Grid + Form_Cells produce them (src/Read_Gmsh_File.cpp:11-621,
+// src/Grid_Computations.cpp:319-827,957-1142).
+struct RefMesh {
+    int np = 0, nc = 0, ng = 0;
+    std::vector<int> nf, nodes, nbr, kind;        // [nc], [4nc] x3
+    std::vector<double> vtx;                      // [12nc] CCW-sorted vertices
+    std::vector<double> nrm, len, dist;           // [8nc], [4nc], [4nc]
+    std::vector<double> ctr, area, inv_area;      // [3(nc+ng)], [nc+ng], [nc+ng]
+    std::vector<int> inl, exl, wal;               // (cell, face, ghost) triplets
+    std::vector<int> gcell, gface;                // per ghost: owner cell and face
+};
+
+void build_ref_mesh(int npts, const double *pts3, int ncells, const int *cells4, RefMesh &m);
+
+// Least-squares coefficient vectors c_{cell,slot} = M^-1 w d (weights 1/|d|^2, ghost
+// neighbours at the reflected centre): grad W_c = sum_slot c (W_n - W_c).
+void wlsq_coefficients(const RefMesh &m, std::vector<double> &cx, std::vector<double> &cy);
+
+// Tiled layout on the renumbered cells.
+struct Layout {
+    int tile = 256;
+    int ntiles = 0;
+    int max_tile_faces = 0;
+    int nfaces = 0;                     // unique faces
+    std::vector<int> perm, iperm;       // new -> old, old -> new
+    std::vector<int> tile_fstart;       // [ntiles+1]
+    std::vector<int> fL, fR;            // per face entry (renumbered; fR = -kind at boundary)
+    std::vector<double> fnx, fny, fdl, fdx, fdy;
+    std::vector<double> fwL, fwR;       // [2*nfe]
+    std::vector<uint16_t> slots;        // [4*nc]
+    std::vector<double> area, inv_area; // renumbered
+};
+
+// renumber: 0 = keep the input order, 1 = Hilbert order of the cell centres.
+void build_layout(const RefMesh &m, int tile, int renumber, Layout &L);
+
+}  // namespace cfd
