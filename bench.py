@@ -37,6 +37,9 @@ def parse():
     p.add_argument("--block", type=int, default=256)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--kernel-reps", type=int, default=20)
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="only run a 512 MiB device copy (read 512 MiB + write 512 MiB) five times, "
+                        "to scale the FETCH_SIZE / WRITE_SIZE counters")
     return p.parse_args()
 
 
@@ -105,6 +108,16 @@ def main():
               "(the multi-GPU partitioning is a design note, not built).", file=sys.stderr)
         return 2
     import torch
+
+    if args.copy_calibration:
+        a = torch.empty(64 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(1.0)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read": a.numel() * 8, "copies": 5}))
+        return 0
 
     from cfd_solver_withcuda_amd.solver import Mesh, Solver, cons_from_prim, make_config
 

@@ -106,7 +106,12 @@ def test_boundary_types_vs_oracle(orc, bump, bc):
                make_config(flux="roe", order=2, limiter="minmod", gradient="wlsq", roe_bdiss=1, bc=bcs, fs=FS))
     s.set_state(U)
     R, _, _ = s.residual()
-    assert np.array_equal(R, Ro)
+    if bc == "farfield":
+        # the far-field ghost state calls pow(); device pow and glibc pow are not both
+        # correctly rounded, so this case is compared to 1e-13 relative, not bitwise
+        assert rel(R, Ro) <= 1e-13
+    else:
+        assert np.array_equal(R, Ro)
 
 
 @pytest.mark.parametrize("flux,order,rk,lts", [("roe", 1, 3, 0), ("hllc", 2, 3, 1), ("ausm", 2, 4, 0),
