@@ -28,7 +28,7 @@ class Config(ctypes.Structure):
     """cfd_config_t (include/cfd_hip.h)."""
     _fields_ = [("flux", _i), ("order", _i), ("limiter", _i), ("gradient", _i), ("roe_bdiss", _i),
                 ("cfl", _d), ("bc", _i * 6), ("fs", _d * 4), ("rk", _i), ("lts", _i),
-                ("ref_mode", _i), ("block", _i), ("xcd_remap", _i)]
+                ("ref_mode", _i), ("block", _i), ("xcd_remap", _i), ("grad_lds", _i)]
 
 
 def _sig(lib, name, res, args):
@@ -49,12 +49,12 @@ def lib() -> ctypes.CDLL:
     _sig(L, "cfd_last_error", ctypes.c_char_p, [])
     _sig(L, "cfd_version", _i, [])
     for nm in ("cfd_mesh_create", "cfd_mesh_create_host"):
-        _sig(L, nm, _i, [_i, _dp, _i, _ip, _i, _i, ctypes.POINTER(_vp)])
+        _sig(L, nm, _i, [_i, _dp, _i, _ip, _i, _i, _i, ctypes.POINTER(_vp)])
     _sig(L, "cfd_mesh_destroy", None, [_vp])
     _sig(L, "cfd_mesh_counts", _i, [_vp, _ip])
     _sig(L, "cfd_mesh_export", _i, [_vp, _ip, _ip, _ip, _dp, _dp, _dp, _dp, _dp, _dp, _ip, _ip, _ip, _ip])
     _sig(L, "cfd_mesh_permutation", _i, [_vp, _ip])
-    _sig(L, "cfd_mesh_layout_export", _i, [_vp, _ip, _ip, _ip, _up])
+    _sig(L, "cfd_mesh_layout_export", _i, [_vp, _ip, _ip, _ip, _ip, _up])
     _sig(L, "cfd_solver_create", _i, [_vp, cfgp, ctypes.POINTER(_vp)])
     _sig(L, "cfd_solver_destroy", None, [_vp])
     _sig(L, "cfd_solver_configure", _i, [_vp, cfgp])

@@ -49,14 +49,14 @@ struct cfd_mesh_s {
     RefMesh ref;
     Layout lay;
     bool on_device = false;
-    int *d_tile_fstart = nullptr, *d_fL = nullptr, *d_fR = nullptr;
+    int *d_tile_cstart = nullptr, *d_tile_fstart = nullptr, *d_fL = nullptr, *d_fR = nullptr;
     double *d_fnx = nullptr, *d_fny = nullptr, *d_fdl = nullptr, *d_fdx = nullptr, *d_fdy = nullptr;
     double *d_fwL = nullptr, *d_fwR = nullptr;
     uint16_t *d_slots = nullptr;
     double *d_area = nullptr, *d_inv_area = nullptr;
     DevMesh dev{};
     ~cfd_mesh_s() {
-        void *ps[] = {d_tile_fstart, d_fL, d_fR, d_fnx, d_fny, d_fdl, d_fdx, d_fdy, d_fwL, d_fwR, d_slots, d_area, d_inv_area};
+        void *ps[] = {d_tile_cstart, d_tile_fstart, d_fL, d_fR, d_fnx, d_fny, d_fdl, d_fdx, d_fdy, d_fwL, d_fwR, d_slots, d_area, d_inv_area};
         for (void *p : ps)
             if (p) (void)hipFree(p);
     }
@@ -85,13 +85,13 @@ const char *cfd_last_error(void) { return g_err.c_str(); }
 int cfd_version(void) { return 1; }
 
 int cfd_mesh_create_host(int npts, const double *pts3, int ncells, const int *cells4, int tile,
-                         int renumber, cfd_mesh_t **out) {
+                         int face_budget, int renumber, cfd_mesh_t **out) {
     if (!out || !pts3 || !cells4 || npts <= 0 || ncells <= 0) return fail(CFD_E_ARG, "cfd_mesh_create: bad arguments");
     *out = nullptr;
     cfd_mesh_s *m = new cfd_mesh_s();
     try {
         build_ref_mesh(npts, pts3, ncells, cells4, m->ref);
-        build_layout(m->ref, tile, renumber, m->lay);
+        build_layout(m->ref, tile, renumber, m->lay, face_budget);
     } catch (const std::exception &e) {
         delete m;
         return fail(CFD_E_MESH, std::string("cfd_mesh_create: ") + e.what());
@@ -101,14 +101,15 @@ int cfd_mesh_create_host(int npts, const double *pts3, int ncells, const int *ce
 }
 
 int cfd_mesh_create(int npts, const double *pts3, int ncells, const int *cells4, int tile,
-                    int renumber, cfd_mesh_t **out) {
-    int rc = cfd_mesh_create_host(npts, pts3, ncells, cells4, tile, renumber, out);
+                    int face_budget, int renumber, cfd_mesh_t **out) {
+    int rc = cfd_mesh_create_host(npts, pts3, ncells, cells4, tile, face_budget, renumber, out);
     if (rc) return rc;
     cfd_mesh_s *m = *out;
     *out = nullptr;
     const Layout &L = m->lay;
     hipError_t e = hipSuccess;
-    if ((e = upload(&m->d_tile_fstart, L.tile_fstart)) != hipSuccess ||
+    if ((e = upload(&m->d_tile_cstart, L.tile_cstart)) != hipSuccess ||
+        (e = upload(&m->d_tile_fstart, L.tile_fstart)) != hipSuccess ||
         (e = upload(&m->d_fL, L.fL)) != hipSuccess || (e = upload(&m->d_fR, L.fR)) != hipSuccess ||
         (e = upload(&m->d_fnx, L.fnx)) != hipSuccess || (e = upload(&m->d_fny, L.fny)) != hipSuccess ||
         (e = upload(&m->d_fdl, L.fdl)) != hipSuccess || (e = upload(&m->d_fdx, L.fdx)) != hipSuccess ||
@@ -123,6 +124,7 @@ int cfd_mesh_create(int npts, const double *pts3, int ncells, const int *cells4,
     d.ntiles = L.ntiles;
     d.tile = L.tile;
     d.max_tile_faces = L.max_tile_faces;
+    d.tile_cstart = m->d_tile_cstart;
     d.tile_fstart = m->d_tile_fstart;
     d.fL = m->d_fL;
     d.fR = m->d_fR;
@@ -194,10 +196,11 @@ int cfd_mesh_permutation(const cfd_mesh_t *m, int *new_to_old) {
     return 0;
 }
 
-int cfd_mesh_layout_export(const cfd_mesh_t *m, int *tile_fstart, int *fL, int *fR,
-                           unsigned short *slots4) {
+int cfd_mesh_layout_export(const cfd_mesh_t *m, int *tile_cstart, int *tile_fstart, int *fL,
+                           int *fR, unsigned short *slots4) {
     if (!m) return fail(CFD_E_ARG, "cfd_mesh_layout_export: null mesh");
     const Layout &L = m->lay;
+    if (tile_cstart) std::memcpy(tile_cstart, L.tile_cstart.data(), L.tile_cstart.size() * sizeof(int));
     if (tile_fstart) std::memcpy(tile_fstart, L.tile_fstart.data(), L.tile_fstart.size() * sizeof(int));
     if (fL) std::memcpy(fL, L.fL.data(), L.fL.size() * sizeof(int));
     if (fR) std::memcpy(fR, L.fR.data(), L.fR.size() * sizeof(int));
@@ -217,7 +220,7 @@ static int check_config(const cfd_mesh_s *m, const cfd_config_t *c) {
     if (c->block != 64 && c->block != 128 && c->block != 256) return fail(CFD_E_ARG, "config: block must be 64, 128 or 256");
     for (int k = 1; k <= 5; k++)
         if (c->bc[k] < 1 || c->bc[k] > 7) return fail(CFD_E_ARG, "config: bc types must be 1..7");
-    const size_t sm = flux_smem_bytes(m->dev, c->order);
+    const size_t sm = flux_smem_bytes(m->dev, c->order, c->grad_lds);
     if (sm > 65536 || grad_smem_bytes(m->dev) > 65536)
         return fail(CFD_E_ARG, "config: tile too large for 64 KiB of LDS per workgroup; use a smaller tile");
     return 0;
@@ -233,6 +236,7 @@ int cfd_solver_configure(cfd_solver_t *s, const cfd_config_t *cfg) {
     s->dcfg.cfl = cfg->cfl;
     s->dcfg.roe_bdiss = cfg->roe_bdiss;
     s->dcfg.xcd_remap = cfg->xcd_remap;
+    s->dcfg.grad_lds = cfg->grad_lds;
     s->dt_valid = false;
     return 0;
 }

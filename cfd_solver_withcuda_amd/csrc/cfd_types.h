@@ -27,8 +27,9 @@ enum GradType { GRAD_GG = 0, GRAD_WLSQ = 1 };
 struct DevMesh {
     int nc;        // physical cells
     int ntiles;
-    int tile;      // cells per tile
+    int tile;      // maximum cells per tile
     int max_tile_faces;
+    const int *tile_cstart;   // [ntiles+1] first cell of each tile
     const int *tile_fstart;   // [ntiles+1] first face entry of each tile
     const int *fL;            // [nfe] left cell (renumbered)
     const int *fR;            // [nfe] right cell, or -(face kind) for a boundary face
@@ -46,6 +47,7 @@ struct DevCfg {
     double cfl;
     int roe_bdiss;
     int xcd_remap;   // map consecutive tiles onto one XCD
+    int grad_lds;    // stage the tile's gradients in LDS (1) or read them through L2 (0)
 };
 
 }  // namespace cfd

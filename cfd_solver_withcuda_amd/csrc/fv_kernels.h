@@ -27,7 +27,7 @@ struct StageArgs {
 };
 
 size_t grad_smem_bytes(const DevMesh &m);
-size_t flux_smem_bytes(const DevMesh &m, int order);
+size_t flux_smem_bytes(const DevMesh &m, int order, int grad_lds);
 hipError_t launch_grad(const DevMesh &m, const DevCfg &c, int grad_type, const double4 *Us,
                        double *grad, int block, hipStream_t s);
 hipError_t launch_flux(const DevMesh &m, const DevCfg &c, int flux, int order, int lim, int mode,

@@ -85,8 +85,8 @@ __global__ void __launch_bounds__(256) grad_tile_kernel(DevMesh m, DevCfg cfg,
     const int tile = tile_of_block(blockIdx.x, m.ntiles, cfg.xcd_remap);
     if (tile >= m.ntiles) return;
     const int T = m.tile;
-    const int c0 = tile * T;
-    const int nct = min(T, m.nc - c0);
+    const int c0 = m.tile_cstart[tile];
+    const int nct = m.tile_cstart[tile + 1] - c0;
     const int fs = m.tile_fstart[tile], nft = m.tile_fstart[tile + 1] - fs;
     double *sW = smem;              // [5][T]
     double *sX = smem + 5 * T;      // [8][max_tile_faces]
@@ -199,8 +199,9 @@ __global__ void __launch_bounds__(256) flux_tile_kernel(DevMesh m, DevCfg cfg,
     if (tile >= m.ntiles) return;
     const int T = m.tile;
     const int FT = m.max_tile_faces;
-    const int c0 = tile * T;
-    const int nct = min(T, m.nc - c0);
+    const int c0 = m.tile_cstart[tile];
+    const int nct = m.tile_cstart[tile + 1] - c0;
+    const bool glds = (ORDER == 2) && cfg.grad_lds;
     const int fs = m.tile_fstart[tile], nft = m.tile_fstart[tile + 1] - fs;
     double *sW = smem;            // [5][T]
     double *sU = sW + 5 * T;      // [4][T]
@@ -213,7 +214,7 @@ __global__ void __launch_bounds__(256) flux_tile_kernel(DevMesh m, DevCfg cfg,
         load_state(Us, c0 + k, s);
         for (int q = 0; q < 5; q++) sW[q * T + k] = s.W[q];
         for (int q = 0; q < 4; q++) sU[q * T + k] = s.U[q];
-        if constexpr (ORDER == 2) {
+        if (glds) {
             const double2 *gp = reinterpret_cast<const double2 *>(grad + 8 * (size_t)(c0 + k));
             for (int q = 0; q < 4; q++) {
                 double2 v = gp[q];
@@ -268,7 +269,7 @@ __global__ void __launch_bounds__(256) flux_tile_kernel(DevMesh m, DevCfg cfg,
         if constexpr (ORDER == 2) {
             const double dx = m.fdx[e], dy = m.fdy[e];
             double gL[8];
-            if (locL) {
+            if (glds && locL) {
                 for (int q = 0; q < 8; q++) gL[q] = sG[q * T + lL];
             } else {
                 const double2 *gp = reinterpret_cast<const double2 *>(grad + 8 * (size_t)L);
@@ -290,7 +291,7 @@ __global__ void __launch_bounds__(256) flux_tile_kernel(DevMesh m, DevCfg cfg,
                 }
             } else {
                 double gR[8];
-                if (locR) {
+                if (glds && locR) {
                     for (int q = 0; q < 8; q++) gR[q] = sG[q * T + lR];
                 } else {
                     const double2 *gp = reinterpret_cast<const double2 *>(grad + 8 * (size_t)R);
@@ -511,9 +512,9 @@ size_t grad_smem_bytes(const DevMesh &m) {
     return sizeof(double) * (5 * (size_t)m.tile + 8 * (size_t)m.max_tile_faces);
 }
 
-size_t flux_smem_bytes(const DevMesh &m, int order) {
+size_t flux_smem_bytes(const DevMesh &m, int order, int grad_lds) {
     size_t d = 5 * (size_t)m.tile + 4 * (size_t)m.tile + 6 * (size_t)m.max_tile_faces;
-    if (order == 2) d += 8 * (size_t)m.tile;
+    if (order == 2 && grad_lds) d += 8 * (size_t)m.tile;
     // the block reductions of phase C need 4 doubles per thread (blockDim <= 256)
     if (d < 4 * 256) d = 4 * 256;
     return sizeof(double) * d;
@@ -562,7 +563,7 @@ static void launch_flux_f(const DevMesh &m, const DevCfg &c, int order, int lim,
 hipError_t launch_flux(const DevMesh &m, const DevCfg &c, int flux, int order, int lim, int mode,
                        const double4 *Us, const double *grad, const StageArgs &st, int block,
                        hipStream_t s) {
-    const size_t sm = flux_smem_bytes(m, order);
+    const size_t sm = flux_smem_bytes(m, order, c.grad_lds);
     switch (flux) {
     case FLUX_ROE: launch_flux_f<FLUX_ROE>(m, c, order, lim, mode, Us, grad, st, block, sm, s); break;
     case FLUX_HLLC: launch_flux_f<FLUX_HLLC>(m, c, order, lim, mode, Us, grad, st, block, sm, s); break;

@@ -271,11 +271,14 @@ void wlsq_coefficients(const RefMesh &m, std::vector<double> &CX, std::vector<do
     }
 }
 
-void build_layout(const RefMesh &m, int tile, int renumber, Layout &L) {
+void build_layout(const RefMesh &m, int tile, int renumber, Layout &L, int face_budget) {
     if (tile < 16 || tile > 4096) throw std::runtime_error("tile size out of range [16, 4096]");
+    if (face_budget != 0 && face_budget < 4)
+        throw std::runtime_error("face budget must be 0 (none) or at least 4 faces");
     L = Layout();
     const int nc = m.nc;
     L.tile = tile;
+    L.face_budget = face_budget;
     L.perm.resize(nc);
     std::iota(L.perm.begin(), L.perm.end(), 0);
     if (renumber == 1 && nc > 0) {
@@ -321,13 +324,36 @@ void build_layout(const RefMesh &m, int tile, int renumber, Layout &L) {
     L.nfaces = (int)canon_cell.size();
     std::vector<double> CX, CY;
     wlsq_coefficients(m, CX, CY);
-    L.ntiles = (nc + tile - 1) / tile;
+    // tile boundaries: contiguous cells, at most `tile` cells and `face_budget` faces each
+    std::vector<int> stamp(L.nfaces, -1), local(L.nfaces, 0);
+    L.tile_cstart.clear();
+    {
+        int k = 0, t = 0;
+        while (k < nc) {
+            L.tile_cstart.push_back(k);
+            int ncell = 0, nface = 0;
+            while (k < nc && ncell < tile) {
+                const int c = L.perm[k];
+                int fresh = 0;
+                for (int f = 0; f < m.nf[c]; f++)
+                    if (stamp[face_of[4 * (size_t)c + f]] != t) fresh++;
+                if (face_budget && ncell > 0 && nface + fresh > face_budget) break;
+                for (int f = 0; f < m.nf[c]; f++) stamp[face_of[4 * (size_t)c + f]] = t;
+                nface += fresh;
+                ncell++;
+                k++;
+            }
+            t++;
+        }
+        L.tile_cstart.push_back(nc);
+        std::fill(stamp.begin(), stamp.end(), -1);
+    }
+    L.ntiles = (int)L.tile_cstart.size() - 1;
     L.tile_fstart.assign(L.ntiles + 1, 0);
     L.slots.assign(4 * (size_t)nc, 0xFFFF);
-    std::vector<int> stamp(L.nfaces, -1), local(L.nfaces, 0);
     for (int t = 0; t < L.ntiles; t++) {
         L.tile_fstart[t] = (int)L.fL.size();
-        const int k0 = t * tile, k1 = std::min(nc, k0 + tile);
+        const int k0 = L.tile_cstart[t], k1 = L.tile_cstart[t + 1];
         int nloc = 0;
         for (int k = k0; k < k1; k++) {
             const int c = L.perm[k];

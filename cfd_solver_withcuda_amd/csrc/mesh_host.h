@@ -25,13 +25,16 @@ void build_ref_mesh(int npts, const double *pts3, int ncells, const int *cells4,
 // neighbours at the reflected centre): grad W_c = sum_slot c (W_n - W_c).
 void wlsq_coefficients(const RefMesh &m, std::vector<double> &cx, std::vector<double> &cy);
 
-// Tiled layout on the renumbered cells.
+// Tiled layout on the renumbered cells. A tile is a contiguous range of at most `tile` cells
+// whose faces number at most `face_budget` (0 = no face limit).
 struct Layout {
     int tile = 256;
+    int face_budget = 0;
     int ntiles = 0;
     int max_tile_faces = 0;
     int nfaces = 0;                     // unique faces
     std::vector<int> perm, iperm;       // new -> old, old -> new
+    std::vector<int> tile_cstart;       // [ntiles+1]
     std::vector<int> tile_fstart;       // [ntiles+1]
     std::vector<int> fL, fR;            // per face entry (renumbered; fR = -kind at boundary)
     std::vector<double> fnx, fny, fdl, fdx, fdy;
@@ -41,6 +44,6 @@ struct Layout {
 };
 
 // renumber: 0 = keep the input order, 1 = Hilbert order of the cell centres.
-void build_layout(const RefMesh &m, int tile, int renumber, Layout &L);
+void build_layout(const RefMesh &m, int tile, int renumber, Layout &L, int face_budget = 0);
 
 }  // namespace cfd

@@ -29,7 +29,7 @@ REF_BCS = ("inlet_ref", "exit", "wall", "wall", "wall")
 
 def make_config(flux="roe", order=1, limiter="vanleer", gradient="gg", roe_bdiss=0, cfl=0.5,
                 bc=REF_BCS, fs=(1.0, 0.0, 0.0, 1.0), rk=3, lts=0, ref_mode=0, block=256,
-                xcd_remap=1) -> Config:
+                xcd_remap=1, grad_lds=1) -> Config:
     c = Config()
     c.flux = FLUX[flux]
     c.order = int(order)
@@ -49,6 +49,7 @@ def make_config(flux="roe", order=1, limiter="vanleer", gradient="gg", roe_bdiss
     c.ref_mode = int(ref_mode)
     c.block = int(block)
     c.xcd_remap = int(xcd_remap)
+    c.grad_lds = int(grad_lds)
     return c
 
 
@@ -93,7 +94,7 @@ class Mesh:
     """A mixed triangle/quad mesh with the reference's connectivity, tiled for the GPU."""
 
     def __init__(self, points: np.ndarray, cells: np.ndarray, tile: int = 256, renumber: int = 1,
-                 device: bool = True):
+                 device: bool = True, face_budget: int = 0):
         L = lib()
         self.points = np.ascontiguousarray(points, np.float64)
         self.cells = np.ascontiguousarray(cells, np.int32)
@@ -104,7 +105,7 @@ class Mesh:
         h = ctypes.c_void_p()
         fn = L.cfd_mesh_create if device else L.cfd_mesh_create_host
         check(fn(len(self.points), dptr(self.points), len(self.cells), iptr(self.cells), tile,
-                 renumber, ctypes.byref(h)), "cfd_mesh_create")
+                 face_budget, renumber, ctypes.byref(h)), "cfd_mesh_create")
         self.h = h
         self.device = device
         c = np.zeros(9, np.int32)
@@ -144,13 +145,14 @@ class Mesh:
         return p
 
     def layout(self):
+        tc = np.zeros(self.ntiles + 1, np.int32)
         tf = np.zeros(self.ntiles + 1, np.int32)
         fL = np.zeros(self.nface_entries, np.int32)
         fR = np.zeros(self.nface_entries, np.int32)
         sl = np.zeros(4 * self.nphys, np.uint16)
-        check(lib().cfd_mesh_layout_export(self.h, iptr(tf), iptr(fL), iptr(fR), uptr(sl)),
+        check(lib().cfd_mesh_layout_export(self.h, iptr(tc), iptr(tf), iptr(fL), iptr(fR), uptr(sl)),
               "cfd_mesh_layout_export")
-        return tf, fL, fR, sl.reshape(-1, 4)
+        return tc, tf, fL, fR, sl.reshape(-1, 4)
 
 
 class Solver:

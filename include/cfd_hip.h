@@ -50,6 +50,7 @@ typedef struct {
                       stage 1 only, exit ghosts from U^n); 0: native fused stages */
     int block;     /* workgroup size (64..1024, power of two; 256 default) */
     int xcd_remap; /* map consecutive tiles onto one XCD */
+    int grad_lds;  /* stage the tile's gradients in LDS (1) or read them through L2 (0) */
 } cfd_config_t;
 
 const char *cfd_last_error(void);
@@ -58,13 +59,13 @@ int cfd_version(void);
 /* ---- mesh -------------------------------------------------------------------------------
  * Builds the reference connectivity (Read_VTK_Grid + Form_Cells, src/Read_Gmsh_File.cpp:11-621,
  * src/Grid_Computations.cpp:319-827) from points [3*npts] and cells [4*ncells] (triangles padded
- * with -1), renumbers cells (renumber = 1: Hilbert order) into tiles of `tile` cells and uploads
- * the face lists to the device. */
+ * with -1), renumbers cells (renumber = 1: Hilbert order) into tiles of at most `tile` cells and
+ * `face_budget` faces (0 = no face limit) and uploads the face lists to the device. */
 int cfd_mesh_create(int npts, const double *pts3, int ncells, const int *cells4, int tile,
-                    int renumber, cfd_mesh_t **out);
+                    int face_budget, int renumber, cfd_mesh_t **out);
 /* Same, host only (no device upload; such a mesh cannot back a solver). */
 int cfd_mesh_create_host(int npts, const double *pts3, int ncells, const int *cells4, int tile,
-                         int renumber, cfd_mesh_t **out);
+                         int face_budget, int renumber, cfd_mesh_t **out);
 void cfd_mesh_destroy(cfd_mesh_t *m);
 /* out[9] = nphys, nghost, n_inlet, n_exit, n_wall, nfaces, ntiles, nface_entries, max_tile_faces */
 int cfd_mesh_counts(const cfd_mesh_t *m, int *out9);
@@ -73,10 +74,11 @@ int cfd_mesh_export(const cfd_mesh_t *m, int *nf, int *nodes4, int *nbr4, double
                     double *len4, double *ctr3_all, double *area_all, double *inv_area_all,
                     double *dist4, int *kind4, int *inlet3, int *exit3, int *wall3);
 int cfd_mesh_permutation(const cfd_mesh_t *m, int *new_to_old);
-/* Tiled face lists: tile_fstart[ntiles+1], fL/fR[nface_entries] (renumbered cells; fR = -kind
- * on a boundary face), slots4[4*nphys] ((local face << 1) | cell-is-right, 0xFFFF unused). */
-int cfd_mesh_layout_export(const cfd_mesh_t *m, int *tile_fstart, int *fL, int *fR,
-                           unsigned short *slots4);
+/* Tiled face lists: tile_cstart/tile_fstart[ntiles+1], fL/fR[nface_entries] (renumbered cells;
+ * fR = -kind on a boundary face), slots4[4*nphys] ((local face << 1) | cell-is-right, 0xFFFF
+ * unused). */
+int cfd_mesh_layout_export(const cfd_mesh_t *m, int *tile_cstart, int *tile_fstart, int *fL,
+                           int *fR, unsigned short *slots4);
 
 /* ---- solver -------------------------------------------------------------------------------- */
 int cfd_solver_create(cfd_mesh_t *m, const cfd_config_t *cfg, cfd_solver_t **out);

@@ -59,23 +59,27 @@ def test_connectivity_matches_oracle_bit_exact(name):
     assert np.array_equal(a.nrm[np.repeat(msk, 2, axis=1)], b.nrm[np.repeat(msk, 2, axis=1)])
 
 
-@pytest.mark.parametrize("tile,renumber", [(32, 0), (64, 1), (256, 1)])
-def test_tiled_layout_is_consistent(tile, renumber):
+@pytest.mark.parametrize("tile,renumber,budget", [(32, 0, 0), (64, 1, 0), (256, 1, 0), (256, 1, 256), (128, 1, 192)])
+def test_tiled_layout_is_consistent(tile, renumber, budget):
     """Every cell slot points at a face of its own tile whose L/R is that cell and whose other
     side is the reference neighbour; interior faces shared by two tiles appear in both."""
     m = meshgen.bump_mesh(40, 20, 0.1, seed=7)
-    pm = Mesh(m.points, m.cells, tile=tile, renumber=renumber, device=False)
+    pm = Mesh(m.points, m.cells, tile=tile, renumber=renumber, device=False, face_budget=budget)
     a = pm.arrays()
     perm = pm.permutation()
     iperm = np.empty_like(perm)
     iperm[perm] = np.arange(len(perm))
     assert np.array_equal(np.sort(perm), np.arange(pm.nphys))
-    tf, fL, fR, slots = pm.layout()
+    tc, tf, fL, fR, slots = pm.layout()
     assert tf[0] == 0 and tf[-1] == pm.nface_entries and np.all(np.diff(tf) > 0)
+    assert tc[0] == 0 and tc[-1] == pm.nphys and np.all(np.diff(tc) > 0) and np.diff(tc).max() <= tile
     assert np.diff(tf).max() == pm.max_tile_faces
+    if budget:
+        assert pm.max_tile_faces <= budget
+    tile_of = np.repeat(np.arange(pm.ntiles), np.diff(tc))
     for k in range(pm.nphys):
         c = perm[k]
-        t = k // tile
+        t = tile_of[k]
         for f in range(a.nf[c]):
             s = int(slots[k, f])
             assert s != 0xFFFF
