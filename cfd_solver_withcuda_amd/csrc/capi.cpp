@@ -220,6 +220,7 @@ static int check_config(const cfd_mesh_s *m, const cfd_config_t *c) {
     if (c->block != 64 && c->block != 128 && c->block != 256) return fail(CFD_E_ARG, "config: block must be 64, 128 or 256");
     for (int k = 1; k <= 5; k++)
         if (c->bc[k] < 1 || c->bc[k] > 7) return fail(CFD_E_ARG, "config: bc types must be 1..7");
+    if (m->dev.tile > c->block) return fail(CFD_E_ARG, "config: the mesh tile (cells per tile) must not exceed the block size");
     const size_t sm = flux_smem_bytes(m->dev, c->order, c->grad_lds);
     if (sm > 65536 || grad_smem_bytes(m->dev) > 65536)
         return fail(CFD_E_ARG, "config: tile too large for 64 KiB of LDS per workgroup; use a smaller tile");

@@ -73,10 +73,84 @@ __device__ __forceinline__ double err_term(double d, double u) {
 }  // namespace
 
 // ------------------------------------------------------------------------------------------
+// Face records. Every load that does not depend on LDS is issued at kernel entry: the face
+// records of a thread's first two faces and its cell's phase-C inputs are held in registers
+// across phase A, so a workgroup pays one memory round trip for them instead of one per phase.
+// ------------------------------------------------------------------------------------------
+struct FaceRec {
+    int L, R;
+    double nx, ny, dl, dx, dy;
+};
+
+template <bool GEOM, bool CENTRE>
+__device__ __forceinline__ void load_face(const DevMesh &m, int e, FaceRec &f) {
+    f.L = m.fL[e];
+    f.R = m.fR[e];
+    if constexpr (GEOM) {
+        f.nx = m.fnx[e];
+        f.ny = m.fny[e];
+        f.dl = m.fdl[e];
+    }
+    if constexpr (CENTRE) {
+        f.dx = m.fdx[e];
+        f.dy = m.fdy[e];
+    }
+}
+
+// ------------------------------------------------------------------------------------------
 // Gradient pass: Green-Gauss (reference semantics, ghost faces use the cell value) or weighted
 // least squares (ghost neighbours at the reflected centre with the boundary ghost state).
-// grad[8c..8c+7] = (d/dx, d/dy) of (rho, u, v, p).
+// grad[8c..8c+7] = (d/dx, d/dy) of (rho, u, v, p). One cell per thread (tile <= blockDim).
 // ------------------------------------------------------------------------------------------
+template <int GRAD>
+__device__ __forceinline__ void grad_face(const DevMesh &m, const DevCfg &cfg,
+                                          const double4 *__restrict__ Us, const double *sW,
+                                          double *sX, int T, int FT, int c0, int nct, int e,
+                                          int j, const FaceRec &f, double2 wl, double2 wr) {
+    const int lL = f.L - c0;
+    double WL[5], WR[4];
+    if ((unsigned)lL < (unsigned)nct) {
+        for (int q = 0; q < 5; q++) WL[q] = sW[q * T + lL];
+    } else {
+        FState s;
+        load_state(Us, f.L, s);
+        for (int q = 0; q < 5; q++) WL[q] = s.W[q];
+    }
+    if (f.R < 0) {
+        if constexpr (GRAD == GRAD_GG) {
+            for (int q = 0; q < 4; q++) WR[q] = WL[q];
+        } else {
+            FState in, g;
+            for (int q = 0; q < 5; q++) in.W[q] = WL[q];
+            for (int q = 0; q < 4; q++) in.U[q] = 0.0;
+            ghost_state(cfg.bc[-f.R], in, m.fnx[e], m.fny[e], cfg.fs, g);
+            for (int q = 0; q < 4; q++) WR[q] = g.W[q];
+        }
+    } else {
+        const int lR = f.R - c0;
+        if ((unsigned)lR < (unsigned)nct) {
+            for (int q = 0; q < 4; q++) WR[q] = sW[q * T + lR];
+        } else {
+            FState s;
+            load_state(Us, f.R, s);
+            for (int q = 0; q < 4; q++) WR[q] = s.W[q];
+        }
+    }
+    if constexpr (GRAD == GRAD_GG) {
+        for (int q = 0; q < 4; q++) {
+            double phi_face = 0.5 * (WL[q] + WR[q]);
+            sX[(2 * q) * FT + j] = phi_face * f.nx * f.dl;
+            sX[(2 * q + 1) * FT + j] = phi_face * f.ny * f.dl;
+        }
+    } else {
+        for (int q = 0; q < 4; q++) sX[q * FT + j] = WR[q] - WL[q];
+        sX[4 * FT + j] = wl.x;
+        sX[5 * FT + j] = wl.y;
+        sX[6 * FT + j] = wr.x;
+        sX[7 * FT + j] = wr.y;
+    }
+}
+
 template <int GRAD>
 __global__ void __launch_bounds__(256) grad_tile_kernel(DevMesh m, DevCfg cfg,
                                                         const double4 *__restrict__ Us,
@@ -85,79 +159,75 @@ __global__ void __launch_bounds__(256) grad_tile_kernel(DevMesh m, DevCfg cfg,
     const int tile = tile_of_block(blockIdx.x, m.ntiles, cfg.xcd_remap);
     if (tile >= m.ntiles) return;
     const int T = m.tile;
+    const int FT = m.max_tile_faces;
+    const int B = blockDim.x;
     const int c0 = m.tile_cstart[tile];
     const int nct = m.tile_cstart[tile + 1] - c0;
     const int fs = m.tile_fstart[tile], nft = m.tile_fstart[tile + 1] - fs;
     double *sW = smem;              // [5][T]
-    double *sX = smem + 5 * T;      // [8][max_tile_faces]
-    const int FT = m.max_tile_faces;
+    double *sX = smem + 5 * T;      // [8][FT]
+    constexpr bool GG = (GRAD == GRAD_GG);
+    const int k = threadIdx.x;
+    const bool hasCell = k < nct;
+    const int c = c0 + k;
 
-    for (int k = threadIdx.x; k < nct; k += blockDim.x) {
+    // ---- prefetch -----------------------------------------------------------------------
+    double4 u4 = make_double4(1.0, 0.0, 0.0, 1.0);
+    ushort4 sl4 = make_ushort4(0xFFFF, 0xFFFF, 0xFFFF, 0xFFFF);
+    double ia = 0.0;
+    if (hasCell) {
+        u4 = Us[c];
+        sl4 = reinterpret_cast<const ushort4 *>(m.slots)[c];
+        if constexpr (GG) ia = m.inv_area[c];
+    }
+    FaceRec f0{}, f1{};
+    double2 w0l{}, w0r{}, w1l{}, w1r{};
+    const bool h0 = k < nft, h1 = k + B < nft;
+    if (h0) {
+        load_face<GG, false>(m, fs + k, f0);
+        if constexpr (!GG) {
+            w0l = m.fwL[fs + k];
+            if (f0.R >= 0) w0r = m.fwR[fs + k];
+        }
+    }
+    if (h1) {
+        load_face<GG, false>(m, fs + k + B, f1);
+        if constexpr (!GG) {
+            w1l = m.fwL[fs + k + B];
+            if (f1.R >= 0) w1r = m.fwR[fs + k + B];
+        }
+    }
+    // ---- A ------------------------------------------------------------------------------
+    if (hasCell) {
         FState s;
-        load_state(Us, c0 + k, s);
+        state_from_U(u4.x, u4.y, u4.z, u4.w, s);
         for (int q = 0; q < 5; q++) sW[q * T + k] = s.W[q];
     }
     __syncthreads();
-
-    for (int j = threadIdx.x; j < nft; j += blockDim.x) {
-        const int e = fs + j;
-        const int L = m.fL[e], R = m.fR[e];
-        const int lL = L - c0;
-        double WL[5], WR[5];
-        if ((unsigned)lL < (unsigned)nct) {
-            for (int q = 0; q < 5; q++) WL[q] = sW[q * T + lL];
-        } else {
-            FState s;
-            load_state(Us, L, s);
-            for (int q = 0; q < 5; q++) WL[q] = s.W[q];
+    // ---- B ------------------------------------------------------------------------------
+    if (h0) grad_face<GRAD>(m, cfg, Us, sW, sX, T, FT, c0, nct, fs + k, k, f0, w0l, w0r);
+    if (h1) grad_face<GRAD>(m, cfg, Us, sW, sX, T, FT, c0, nct, fs + k + B, k + B, f1, w1l, w1r);
+    for (int j = k + 2 * B; j < nft; j += B) {
+        FaceRec f;
+        load_face<GG, false>(m, fs + j, f);
+        double2 wl{}, wr{};
+        if constexpr (!GG) {
+            wl = m.fwL[fs + j];
+            if (f.R >= 0) wr = m.fwR[fs + j];
         }
-        if (R < 0) {
-            if constexpr (GRAD == GRAD_GG) {
-                for (int q = 0; q < 4; q++) WR[q] = WL[q];
-            } else {
-                FState in, g;
-                for (int q = 0; q < 5; q++) in.W[q] = WL[q];
-                for (int q = 0; q < 4; q++) in.U[q] = 0.0;
-                ghost_state(cfg.bc[-R], in, m.fnx[e], m.fny[e], cfg.fs, g);
-                for (int q = 0; q < 4; q++) WR[q] = g.W[q];
-            }
-        } else {
-            const int lR = R - c0;
-            if ((unsigned)lR < (unsigned)nct) {
-                for (int q = 0; q < 4; q++) WR[q] = sW[q * T + lR];
-            } else {
-                FState s;
-                load_state(Us, R, s);
-                for (int q = 0; q < 4; q++) WR[q] = s.W[q];
-            }
-        }
-        if constexpr (GRAD == GRAD_GG) {
-            const double nx = m.fnx[e], ny = m.fny[e], dl = m.fdl[e];
-            for (int q = 0; q < 4; q++) {
-                double phi_face = 0.5 * (WL[q] + WR[q]);
-                sX[(2 * q) * FT + j] = phi_face * nx * dl;
-                sX[(2 * q + 1) * FT + j] = phi_face * ny * dl;
-            }
-        } else {
-            for (int q = 0; q < 4; q++) sX[q * FT + j] = WR[q] - WL[q];
-            double2 a = m.fwL[e], b = (R < 0) ? make_double2(0.0, 0.0) : m.fwR[e];
-            sX[4 * FT + j] = a.x;
-            sX[5 * FT + j] = a.y;
-            sX[6 * FT + j] = b.x;
-            sX[7 * FT + j] = b.y;
-        }
+        grad_face<GRAD>(m, cfg, Us, sW, sX, T, FT, c0, nct, fs + j, j, f, wl, wr);
     }
     __syncthreads();
-
-    for (int k = threadIdx.x; k < nct; k += blockDim.x) {
-        const int c = c0 + k;
+    // ---- C ------------------------------------------------------------------------------
+    if (hasCell) {
         double g[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        const uint16_t sls[4] = {sl4.x, sl4.y, sl4.z, sl4.w};
         for (int f = 0; f < 4; f++) {
-            const uint16_t sl = m.slots[4 * c + f];
+            const uint16_t sl = sls[f];
             if (sl == 0xFFFF) break;
             const int j = sl >> 1;
             const bool isR = sl & 1;
-            if constexpr (GRAD == GRAD_GG) {
+            if constexpr (GG) {
                 for (int q = 0; q < 8; q++) {
                     double v = sX[q * FT + j];
                     g[q] += isR ? -v : v;
@@ -173,10 +243,8 @@ __global__ void __launch_bounds__(256) grad_tile_kernel(DevMesh m, DevCfg cfg,
                 }
             }
         }
-        if constexpr (GRAD == GRAD_GG) {
-            const double ia = m.inv_area[c];
+        if constexpr (GG)
             for (int q = 0; q < 8; q++) g[q] = g[q] * ia;
-        }
         double2 *out = reinterpret_cast<double2 *>(grad + 8 * (size_t)c);
         out[0] = make_double2(g[0], g[1]);
         out[1] = make_double2(g[2], g[3]);
@@ -187,8 +255,96 @@ __global__ void __launch_bounds__(256) grad_tile_kernel(DevMesh m, DevCfg cfg,
 
 // ------------------------------------------------------------------------------------------
 // Flux pass. MODE_RESID writes R and the local time step; MODE_STAGE applies one stage of
-// TVD-RK3 / RK4 in place of writing R.
+// TVD-RK3 / RK4 in place of writing R. One cell per thread (tile <= blockDim).
 // ------------------------------------------------------------------------------------------
+__device__ __forceinline__ void load_grad(const double *__restrict__ grad, int c, double *g) {
+    const double2 *gp = reinterpret_cast<const double2 *>(grad + 8 * (size_t)c);
+    for (int q = 0; q < 4; q++) {
+        double2 v = gp[q];
+        g[2 * q] = v.x;
+        g[2 * q + 1] = v.y;
+    }
+}
+
+template <int FLUX, int ORDER, int LIM>
+__device__ __forceinline__ void flux_face(const DevMesh &m, const DevCfg &cfg,
+                                          const double4 *__restrict__ Us,
+                                          const double *__restrict__ grad, const double4 *Uexit,
+                                          const double *sW, const double *sU, const double *sG,
+                                          double *sF, bool glds, int T, int FT, int c0, int nct,
+                                          int j, const FaceRec &f) {
+    const double nx = f.nx, ny = f.ny, dl = f.dl;
+    const int lL = f.L - c0;
+    const bool locL = (unsigned)lL < (unsigned)nct;
+    FState SL, SR;
+    if (locL) {
+        for (int q = 0; q < 5; q++) SL.W[q] = sW[q * T + lL];
+        for (int q = 0; q < 4; q++) SL.U[q] = sU[q * T + lL];
+    } else {
+        load_state(Us, f.L, SL);
+    }
+    const bool boundary = f.R < 0;
+    int bc = 0;
+    bool locR = false;
+    int lR = 0;
+    if (boundary) {
+        bc = cfg.bc[-f.R];
+        if (bc == BC_EXIT && Uexit) {
+            FState src;
+            load_state(Uexit, f.L, src);
+            ghost_state(bc, src, nx, ny, cfg.fs, SR);
+        } else {
+            ghost_state(bc, SL, nx, ny, cfg.fs, SR);
+        }
+    } else {
+        lR = f.R - c0;
+        locR = (unsigned)lR < (unsigned)nct;
+        if (locR) {
+            for (int q = 0; q < 5; q++) SR.W[q] = sW[q * T + lR];
+            for (int q = 0; q < 4; q++) SR.U[q] = sU[q * T + lR];
+        } else {
+            load_state(Us, f.R, SR);
+        }
+    }
+    // spectral terms (|u.n| + a) dl of the two cell-centre states for the local time step
+    const double lamL = (fabs(SL.W[1] * nx + SL.W[2] * ny) + SL.W[4]) * dl;
+    const double lamR = boundary ? 0.0 : (fabs(SR.W[1] * nx + SR.W[2] * ny) + SR.W[4]) * dl;
+    double F[4];
+    if constexpr (ORDER == 2) {
+        double gL[8];
+        if (glds && locL) {
+            for (int q = 0; q < 8; q++) gL[q] = sG[q * T + lL];
+        } else {
+            load_grad(grad, f.L, gL);
+        }
+        FState fL, fR;
+        reconstruct_side<LIM>(SL, SR, gL, f.dx, f.dy, 1.0, fL);
+        if (boundary) {
+            if (bc == BC_EXIT && Uexit) {
+                FState src;
+                load_state(Uexit, f.L, src);
+                ghost_state(bc, src, nx, ny, cfg.fs, fR);
+            } else {
+                ghost_state(bc, fL, nx, ny, cfg.fs, fR);
+            }
+        } else {
+            double gR[8];
+            if (glds && locR) {
+                for (int q = 0; q < 8; q++) gR[q] = sG[q * T + lR];
+            } else {
+                load_grad(grad, f.R, gR);
+            }
+            reconstruct_side<LIM>(SR, SL, gR, f.dx, f.dy, -1.0, fR);
+        }
+        face_flux<FLUX>(boundary, cfg.roe_bdiss, fL, fR, nx, ny, dl, F);
+    } else {
+        face_flux<FLUX>(boundary, cfg.roe_bdiss, SL, SR, nx, ny, dl, F);
+    }
+    for (int q = 0; q < 4; q++) sF[q * FT + j] = F[q];
+    sF[4 * FT + j] = lamL;
+    sF[5 * FT + j] = lamR;
+}
+
 template <int FLUX, int ORDER, int LIM, int MODE>
 __global__ void __launch_bounds__(256) flux_tile_kernel(DevMesh m, DevCfg cfg,
                                                         const double4 *__restrict__ Us,
@@ -199,129 +355,71 @@ __global__ void __launch_bounds__(256) flux_tile_kernel(DevMesh m, DevCfg cfg,
     if (tile >= m.ntiles) return;
     const int T = m.tile;
     const int FT = m.max_tile_faces;
+    const int B = blockDim.x;
     const int c0 = m.tile_cstart[tile];
     const int nct = m.tile_cstart[tile + 1] - c0;
-    const bool glds = (ORDER == 2) && cfg.grad_lds;
     const int fs = m.tile_fstart[tile], nft = m.tile_fstart[tile + 1] - fs;
+    const bool glds = (ORDER == 2) && cfg.grad_lds;
     double *sW = smem;            // [5][T]
     double *sU = sW + 5 * T;      // [4][T]
     double *sF = sU + 4 * T;      // [6][FT]: flux (4), spectral terms of L and R
-    double *sG = sF + 6 * FT;     // [8][T] (second order)
+    double *sG = sF + 6 * FT;     // [8][T] (second order, LDS-staged gradients)
+    const int k = threadIdx.x;
+    const bool hasCell = k < nct;
+    const int c = c0 + k;
 
-    // ---- A: stage the tile's states --------------------------------------------------
-    for (int k = threadIdx.x; k < nct; k += blockDim.x) {
+    // ---- prefetch -----------------------------------------------------------------------
+    double4 u4 = make_double4(1.0, 0.0, 0.0, 1.0), un4 = make_double4(0, 0, 0, 0);
+    ushort4 sl4 = make_ushort4(0xFFFF, 0xFFFF, 0xFFFF, 0xFFFF);
+    double ar = 0.0, ia = 0.0, hprev = 0.0;
+    if (hasCell) {
+        u4 = Us[c];
+        sl4 = reinterpret_cast<const ushort4 *>(m.slots)[c];
+        ar = m.area[c];
+        if constexpr (MODE == MODE_STAGE) {
+            un4 = st.Un[c];
+            ia = m.inv_area[c];
+            if (st.lts && st.stage > 0) hprev = st.hbuf[c];
+        }
+    }
+    FaceRec f0{}, f1{};
+    const bool h0 = k < nft, h1 = k + B < nft;
+    if (h0) load_face<true, ORDER == 2>(m, fs + k, f0);
+    if (h1) load_face<true, ORDER == 2>(m, fs + k + B, f1);
+
+    // ---- A: stage the tile's states ----------------------------------------------------
+    if (hasCell) {
         FState s;
-        load_state(Us, c0 + k, s);
+        state_from_U(u4.x, u4.y, u4.z, u4.w, s);
         for (int q = 0; q < 5; q++) sW[q * T + k] = s.W[q];
         for (int q = 0; q < 4; q++) sU[q * T + k] = s.U[q];
         if (glds) {
-            const double2 *gp = reinterpret_cast<const double2 *>(grad + 8 * (size_t)(c0 + k));
-            for (int q = 0; q < 4; q++) {
-                double2 v = gp[q];
-                sG[(2 * q) * T + k] = v.x;
-                sG[(2 * q + 1) * T + k] = v.y;
-            }
+            double g[8];
+            load_grad(grad, c, g);
+            for (int q = 0; q < 8; q++) sG[q * T + k] = g[q];
         }
     }
     __syncthreads();
 
     // ---- B: one flux per face ----------------------------------------------------------
-    for (int j = threadIdx.x; j < nft; j += blockDim.x) {
-        const int e = fs + j;
-        const int L = m.fL[e], R = m.fR[e];
-        const double nx = m.fnx[e], ny = m.fny[e], dl = m.fdl[e];
-        const int lL = L - c0;
-        const bool locL = (unsigned)lL < (unsigned)nct;
-        FState SL, SR;
-        if (locL) {
-            for (int q = 0; q < 5; q++) SL.W[q] = sW[q * T + lL];
-            for (int q = 0; q < 4; q++) SL.U[q] = sU[q * T + lL];
-        } else {
-            load_state(Us, L, SL);
-        }
-        const bool boundary = R < 0;
-        int bc = 0;
-        bool locR = false;
-        int lR = 0;
-        if (boundary) {
-            bc = cfg.bc[-R];
-            if (bc == BC_EXIT && st.Uexit) {
-                FState src;
-                load_state(st.Uexit, L, src);
-                ghost_state(bc, src, nx, ny, cfg.fs, SR);
-            } else {
-                ghost_state(bc, SL, nx, ny, cfg.fs, SR);
-            }
-        } else {
-            lR = R - c0;
-            locR = (unsigned)lR < (unsigned)nct;
-            if (locR) {
-                for (int q = 0; q < 5; q++) SR.W[q] = sW[q * T + lR];
-                for (int q = 0; q < 4; q++) SR.U[q] = sU[q * T + lR];
-            } else {
-                load_state(Us, R, SR);
-            }
-        }
-        // spectral terms (|u.n| + a) dl of the two cell-centre states for the local time step
-        const double lamL = (fabs(SL.W[1] * nx + SL.W[2] * ny) + SL.W[4]) * dl;
-        const double lamR = boundary ? 0.0 : (fabs(SR.W[1] * nx + SR.W[2] * ny) + SR.W[4]) * dl;
-        double F[4];
-        if constexpr (ORDER == 2) {
-            const double dx = m.fdx[e], dy = m.fdy[e];
-            double gL[8];
-            if (glds && locL) {
-                for (int q = 0; q < 8; q++) gL[q] = sG[q * T + lL];
-            } else {
-                const double2 *gp = reinterpret_cast<const double2 *>(grad + 8 * (size_t)L);
-                for (int q = 0; q < 4; q++) {
-                    double2 v = gp[q];
-                    gL[2 * q] = v.x;
-                    gL[2 * q + 1] = v.y;
-                }
-            }
-            FState fL, fR;
-            reconstruct_side<LIM>(SL, SR, gL, dx, dy, 1.0, fL);
-            if (boundary) {
-                if (bc == BC_EXIT && st.Uexit) {
-                    FState src;
-                    load_state(st.Uexit, L, src);
-                    ghost_state(bc, src, nx, ny, cfg.fs, fR);
-                } else {
-                    ghost_state(bc, fL, nx, ny, cfg.fs, fR);
-                }
-            } else {
-                double gR[8];
-                if (glds && locR) {
-                    for (int q = 0; q < 8; q++) gR[q] = sG[q * T + lR];
-                } else {
-                    const double2 *gp = reinterpret_cast<const double2 *>(grad + 8 * (size_t)R);
-                    for (int q = 0; q < 4; q++) {
-                        double2 v = gp[q];
-                        gR[2 * q] = v.x;
-                        gR[2 * q + 1] = v.y;
-                    }
-                }
-                reconstruct_side<LIM>(SR, SL, gR, dx, dy, -1.0, fR);
-            }
-            face_flux<FLUX>(boundary, cfg.roe_bdiss, fL, fR, nx, ny, dl, F);
-        } else {
-            face_flux<FLUX>(boundary, cfg.roe_bdiss, SL, SR, nx, ny, dl, F);
-        }
-        for (int q = 0; q < 4; q++) sF[q * FT + j] = F[q];
-        sF[4 * FT + j] = lamL;
-        sF[5 * FT + j] = lamR;
+    if (h0) flux_face<FLUX, ORDER, LIM>(m, cfg, Us, grad, st.Uexit, sW, sU, sG, sF, glds, T, FT, c0, nct, k, f0);
+    if (h1) flux_face<FLUX, ORDER, LIM>(m, cfg, Us, grad, st.Uexit, sW, sU, sG, sF, glds, T, FT, c0, nct, k + B, f1);
+    for (int j = k + 2 * B; j < nft; j += B) {
+        FaceRec f;
+        load_face<true, ORDER == 2>(m, fs + j, f);
+        flux_face<FLUX, ORDER, LIM>(m, cfg, Us, grad, st.Uexit, sW, sU, sG, sF, glds, T, FT, c0, nct, j, f);
     }
     __syncthreads();
 
     // ---- C: gather per cell, time step, stage update -----------------------------------
     double errp[4] = {0.0, 0.0, 0.0, 0.0};
     double dtmin = 1.7976931348623157e308;
-    for (int k = threadIdx.x; k < nct; k += blockDim.x) {
-        const int c = c0 + k;
+    if (hasCell) {
         double r[4] = {0.0, 0.0, 0.0, 0.0};
         double denom = 0.0;
+        const uint16_t sls[4] = {sl4.x, sl4.y, sl4.z, sl4.w};
         for (int f = 0; f < 4; f++) {
-            const uint16_t sl = m.slots[4 * c + f];
+            const uint16_t sl = sls[f];
             if (sl == 0xFFFF) break;
             const int j = sl >> 1;
             const bool isR = sl & 1;
@@ -332,7 +430,7 @@ __global__ void __launch_bounds__(256) flux_tile_kernel(DevMesh m, DevCfg cfg,
             denom += isR ? sF[5 * FT + j] : sF[4 * FT + j];
         }
         // Inviscid_Time_Step (src/Time_Step.cpp:293-329)
-        const double dtc = (denom <= 0.0 || !isfinite(denom)) ? 0.0 : cfg.cfl * m.area[c] / denom;
+        const double dtc = (denom <= 0.0 || !isfinite(denom)) ? 0.0 : cfg.cfl * ar / denom;
         if constexpr (MODE == MODE_RESID) {
             reinterpret_cast<double4 *>(st.R_out)[c] = make_double4(r[0], r[1], r[2], r[3]);
             if (st.dt_out) st.dt_out[c] = dtc;
@@ -341,15 +439,14 @@ __global__ void __launch_bounds__(256) flux_tile_kernel(DevMesh m, DevCfg cfg,
             double h;
             if (st.lts) {
                 if (st.stage == 0) {
-                    h = dtc * m.inv_area[c];
+                    h = dtc * ia;
                     st.hbuf[c] = h;
                 } else {
-                    h = st.hbuf[c];
+                    h = hprev;
                 }
             } else {
-                h = (*st.dt_global) * m.inv_area[c];
+                h = (*st.dt_global) * ia;
             }
-            const double4 un4 = st.Un[c];
             const double Un[4] = {un4.x, un4.y, un4.z, un4.w};
             double Uo[4];
             if (st.rk == 4) {
@@ -365,12 +462,12 @@ __global__ void __launch_bounds__(256) flux_tile_kernel(DevMesh m, DevCfg cfg,
                 }
                 if (st.stage < 3) st.acc[c] = make_double4(acc[0], acc[1], acc[2], acc[3]);
             } else {
+                const double us[4] = {u4.x, u4.y, u4.z, u4.w};
                 for (int q = 0; q < 4; q++) {
-                    const double us = sU[q * T + k];
                     switch (st.stage) {
                     case 0: Uo[q] = Un[q] - h * r[q]; break;
-                    case 1: Uo[q] = 0.75 * Un[q] + 0.25 * (us - h * r[q]); break;
-                    default: Uo[q] = (1.0 / 3.0) * Un[q] + (2.0 / 3.0) * (us - h * r[q]); break;
+                    case 1: Uo[q] = 0.75 * Un[q] + 0.25 * (us[q] - h * r[q]); break;
+                    default: Uo[q] = (1.0 / 3.0) * Un[q] + (2.0 / 3.0) * (us[q] - h * r[q]); break;
                     }
                 }
             }
@@ -385,12 +482,12 @@ __global__ void __launch_bounds__(256) flux_tile_kernel(DevMesh m, DevCfg cfg,
                     state_from_U(Uo[0], Uo[1], Uo[2], Uo[3], s);
                     double den = 0.0;
                     for (int f = 0; f < 4; f++) {
-                        const uint16_t sl = m.slots[4 * c + f];
+                        const uint16_t sl = sls[f];
                         if (sl == 0xFFFF) break;
                         const int e = fs + (sl >> 1);
                         den += (fabs(s.W[1] * m.fnx[e] + s.W[2] * m.fny[e]) + s.W[4]) * m.fdl[e];
                     }
-                    const double dtn = (den <= 0.0 || !isfinite(den)) ? 0.0 : cfg.cfl * m.area[c] / den;
+                    const double dtn = (den <= 0.0 || !isfinite(den)) ? 0.0 : cfg.cfl * ar / den;
                     dtmin = fmin(dtmin, dtn);
                 }
             }
