@@ -65,6 +65,11 @@ def lib() -> ctypes.CDLL:
     _sig(L, "cfd_step", _i, [_vp, _i, _dp, _dp, _vp])
     _sig(L, "cfd_run_steady", _i, [_vp, _d, _i, _i, _ip, _dp, _vp])
     _sig(L, "cfd_time_pass", _i, [_vp, _i, _i, ctypes.POINTER(ctypes.c_float), _vp])
+    _sig(L, "cfd_device_alloc", _i, [ctypes.c_size_t, ctypes.POINTER(_vp)])
+    _sig(L, "cfd_device_free", _i, [_vp])
+    _sig(L, "cfd_memcpy_h2d", _i, [_vp, _vp, ctypes.c_size_t])
+    _sig(L, "cfd_memcpy_d2h", _i, [_vp, _vp, ctypes.c_size_t])
+    _sig(L, "cfd_synchronize", _i, [])
     _lib = L
     return L
 
@@ -86,3 +91,34 @@ def iptr(a):
 
 def uptr(a):
     return a.ctypes.data_as(_up)
+
+
+class DeviceArray:
+    """A device copy of a numpy array (allocated and copied through the library)."""
+
+    def __init__(self, host):
+        import numpy as np
+        self.host = np.ascontiguousarray(host)
+        self.nbytes = self.host.nbytes
+        p = _vp()
+        check(lib().cfd_device_alloc(self.nbytes, ctypes.byref(p)), "cfd_device_alloc")
+        self.ptr = p
+        check(lib().cfd_memcpy_h2d(self.ptr, self.host.ctypes.data_as(_vp), self.nbytes), "cfd_memcpy_h2d")
+
+    def get(self):
+        import numpy as np
+        out = np.empty_like(self.host)
+        check(lib().cfd_synchronize(), "cfd_synchronize")
+        check(lib().cfd_memcpy_d2h(out.ctypes.data_as(_vp), self.ptr, self.nbytes), "cfd_memcpy_d2h")
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().cfd_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

@@ -84,6 +84,29 @@ extern "C" {
 const char *cfd_last_error(void) { return g_err.c_str(); }
 int cfd_version(void) { return 1; }
 
+int cfd_device_alloc(size_t bytes, void **ptr) {
+    if (!ptr) return fail(CFD_E_ARG, "cfd_device_alloc: null argument");
+    *ptr = nullptr;
+    HIP_TRY(hipMalloc(ptr, bytes ? bytes : 1));
+    return 0;
+}
+int cfd_device_free(void *ptr) {
+    if (ptr) HIP_TRY(hipFree(ptr));
+    return 0;
+}
+int cfd_memcpy_h2d(void *dst, const void *src, size_t bytes) {
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return 0;
+}
+int cfd_memcpy_d2h(void *dst, const void *src, size_t bytes) {
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return 0;
+}
+int cfd_synchronize(void) {
+    HIP_TRY(hipDeviceSynchronize());
+    return 0;
+}
+
 int cfd_mesh_create_host(int npts, const double *pts3, int ncells, const int *cells4, int tile,
                          int face_budget, int renumber, cfd_mesh_t **out) {
     if (!out || !pts3 || !cells4 || npts <= 0 || ncells <= 0) return fail(CFD_E_ARG, "cfd_mesh_create: bad arguments");

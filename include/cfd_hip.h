@@ -18,6 +18,8 @@
 #ifndef CFD_HIP_H
 #define CFD_HIP_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -55,6 +57,13 @@ typedef struct {
 
 const char *cfd_last_error(void);
 int cfd_version(void);
+
+/* Device-memory helpers for callers without their own HIP runtime bindings. */
+int cfd_device_alloc(size_t bytes, void **ptr);
+int cfd_device_free(void *ptr);
+int cfd_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int cfd_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int cfd_synchronize(void);
 
 /* ---- mesh -------------------------------------------------------------------------------
  * Builds the reference connectivity (Read_VTK_Grid + Form_Cells, src/Read_Gmsh_File.cpp:11-621,

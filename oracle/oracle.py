@@ -60,13 +60,17 @@ def make_cfg(flux="roe", order=1, limiter="vanleer", gradient="gg", roe_bdiss=0,
     return c
 
 
+ORACLE_SOURCES = ("cfd_oracle.c", "ref_layout.c")
+
+
 def build_oracle(force: bool = False) -> str:
-    """Compiles oracle/cfd_oracle.c with gcc (contraction off, like the reference build)."""
-    src = os.path.join(HERE, "cfd_oracle.c")
+    """Compiles the oracle's C sources with gcc (contraction off, like the reference build)."""
+    srcs = [os.path.join(HERE, f) for f in ORACLE_SOURCES]
     os.makedirs(os.path.dirname(ORACLE_SO), exist_ok=True)
-    if force or not os.path.exists(ORACLE_SO) or os.path.getmtime(ORACLE_SO) < os.path.getmtime(src):
+    newest = max(os.path.getmtime(f) for f in srcs + [os.path.join(HERE, "cfd_oracle.h")])
+    if force or not os.path.exists(ORACLE_SO) or os.path.getmtime(ORACLE_SO) < newest:
         subprocess.check_call(["gcc", "-std=c11", "-O2", "-ffp-contract=off", "-fPIC", "-shared",
-                               "-Wall", "-o", ORACLE_SO, src, "-lm"])
+                               "-Wall", "-o", ORACLE_SO] + srcs + ["-lm"])
     return ORACLE_SO
 
 
@@ -135,6 +139,18 @@ class Oracle:
         L.or_ausm_cell.argtypes = [ctypes.c_void_p, _dp, _dp]
         L.or_roe2o_residual.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp]
         L.or_apply_bc_ref.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp]
+        # reference-layout restatements (oracle/ref_layout.c)
+        L.rl_flux.argtypes = [ctypes.c_int, _dp, _dp, _ip, _dp, _dp, _dp, ctypes.c_int, ctypes.c_double]
+        L.rl_roe2.argtypes = [_dp, _ip, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_double]
+        L.rl_phi.restype = ctypes.c_double
+        L.rl_phi.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double]
+        L.rl_limiter.argtypes = [_dp, _ip, _dp, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int]
+        L.rl_venkat.argtypes = [_dp, _ip, _dp, _dp, ctypes.c_int, ctypes.c_double]
+        L.rl_bj.argtypes = [_dp, _ip, _dp, _dp, _dp, ctypes.c_int]
+        L.rl_muscl.argtypes = [_dp, _ip, _dp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_double]
+        L.rl_bc.argtypes = [ctypes.c_int, _dp, _ip, _dp, _dp, ctypes.c_int]
+        L.rl_rk3_stage.argtypes = [_dp, _dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_int, ctypes.c_int]
+        L.rl_rk4_stage.argtypes = [_dp, _dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_int, ctypes.c_int]
 
     # ---- mesh ----
     def mesh(self, points: np.ndarray, cells: np.ndarray) -> "OracleMesh":
