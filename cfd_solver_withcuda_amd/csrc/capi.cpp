@@ -42,7 +42,6 @@ hipError_t upload(T **d, const std::vector<T> &h) {
     return hipMemcpy(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice);
 }
 
-const double kDblMax = 1.7976931348623157e308;
 }  // namespace
 
 struct cfd_mesh_s {
@@ -67,13 +66,13 @@ struct cfd_solver_s {
     cfd_config_t cfg{};
     DevCfg dcfg{};
     double4 *Un = nullptr, *Ua = nullptr, *Ub = nullptr, *acc = nullptr, *R = nullptr;
-    double *dt = nullptr, *grad = nullptr, *hbuf = nullptr, *err_partial = nullptr;
+    double *dt = nullptr, *grad = nullptr, *hbuf = nullptr, *err_partial = nullptr, *dt_partial = nullptr;
     double *scal = nullptr;  // [0],[1] global dt slots, [2] stage minimum, [3..6] error norm
     int cur = 0;
     bool dt_valid = false;
     bool has_state = false;
     ~cfd_solver_s() {
-        void *ps[] = {Un, Ua, Ub, acc, R, dt, grad, hbuf, err_partial, scal};
+        void *ps[] = {Un, Ua, Ub, acc, R, dt, grad, hbuf, err_partial, dt_partial, scal};
         for (void *p : ps)
             if (p) (void)hipFree(p);
     }
@@ -288,6 +287,7 @@ int cfd_solver_create(cfd_mesh_t *m, const cfd_config_t *cfg, cfd_solver_t **out
         (e = hipMalloc((void **)&s->grad, 8 * nc * sizeof(double))) != hipSuccess ||
         (e = hipMalloc((void **)&s->hbuf, nc * sizeof(double))) != hipSuccess ||
         (e = hipMalloc((void **)&s->err_partial, 4 * nparts * sizeof(double))) != hipSuccess ||
+        (e = hipMalloc((void **)&s->dt_partial, nparts * sizeof(double))) != hipSuccess ||
         (e = hipMalloc((void **)&s->scal, 8 * sizeof(double))) != hipSuccess) {
         delete s;
         return hip_fail(e, "cfd_solver_create hipMalloc");
@@ -325,11 +325,11 @@ int cfd_state_device(cfd_solver_t *s, void **d_U) {
     return 0;
 }
 
-static StageArgs resid_args(cfd_solver_s *s, double *dtmin, const double4 *uexit) {
+static StageArgs resid_args(cfd_solver_s *s, bool want_min, const double4 *uexit) {
     StageArgs st{};
     st.R_out = reinterpret_cast<double *>(s->R);
     st.dt_out = s->dt;
-    st.dtmin_out = dtmin;
+    st.dt_partial = want_min ? s->dt_partial : nullptr;
     st.Uexit = uexit;
     return st;
 }
@@ -340,9 +340,9 @@ static int residual_pass(cfd_solver_s *s, const double4 *in, double *dtmin, cons
     const cfd_config_t &c = s->cfg;
     const int order = order_override ? order_override : c.order;
     if (order == 2) HIP_TRY(launch_grad(s->m->dev, s->dcfg, c.gradient, in, s->grad, c.block, st));
-    if (dtmin) HIP_TRY(launch_fill(dtmin, kDblMax, 1, st));
-    StageArgs a = resid_args(s, dtmin, uexit);
+    StageArgs a = resid_args(s, dtmin != nullptr, uexit);
     HIP_TRY(launch_flux(s->m->dev, s->dcfg, c.flux, order, c.limiter, MODE_RESID, in, s->grad, a, c.block, st));
+    if (dtmin) HIP_TRY(launch_min_reduce(s->dt_partial, s->m->dev.ntiles, dtmin, st));
     return 0;
 }
 
@@ -382,7 +382,6 @@ static int native_step(cfd_solver_s *s, bool want_err, hipStream_t st) {
         if (rc) return rc;
         s->dt_valid = true;
     }
-    if (!c.lts) HIP_TRY(launch_fill(s->scal + (1 - s->cur), kDblMax, 1, st));
     const int nst = (c.rk == 4) ? 4 : 3;
     const double4 *ins3[3] = {s->Un, s->Ua, s->Ub};
     double4 *outs3[3] = {s->Ua, s->Ub, s->Un};
@@ -403,9 +402,10 @@ static int native_step(cfd_solver_s *s, bool want_err, hipStream_t st) {
         a.hbuf = s->hbuf;
         a.dt_global = s->scal + s->cur;
         a.err_partial = (a.last && want_err) ? s->err_partial : nullptr;
-        a.dtmin_out = (a.last && !c.lts) ? s->scal + (1 - s->cur) : nullptr;
+        a.dt_partial = (a.last && !c.lts) ? s->dt_partial : nullptr;
         HIP_TRY(launch_flux(dm, s->dcfg, c.flux, c.order, c.limiter, MODE_STAGE, in, s->grad, a, c.block, st));
     }
+    if (!c.lts) HIP_TRY(launch_min_reduce(s->dt_partial, dm.ntiles, s->scal + (1 - s->cur), st));
     if (want_err) HIP_TRY(launch_err_finalize(s->err_partial, dm.ntiles, s->scal + 3, st));
     return 0;
 }

@@ -21,7 +21,7 @@ struct StageArgs {
     const double *dt_global;
     double *R_out;
     double *dt_out;
-    double *dtmin_out;   // atomic minimum of the local time steps
+    double *dt_partial;  // [ntiles] per-tile minimum of the local time steps
     double *err_partial; // [4*ntiles] per-tile sums of the reference error terms
     const double4 *Uexit;  // exit ghosts copy this state when set (reference RK stages)
 };
@@ -35,6 +35,7 @@ hipError_t launch_flux(const DevMesh &m, const DevCfg &c, int flux, int order, i
                        hipStream_t s);
 hipError_t launch_fill(double *p, double v, int n, hipStream_t s);
 hipError_t launch_err_finalize(const double *partial, int n, double *err4, hipStream_t s);
+hipError_t launch_min_reduce(const double *partial, int n, double *out, hipStream_t s);
 hipError_t launch_ref_update(int nc, int kind, int lts, const double *inv_area, const double4 *Un,
                              const double4 *Us, const double4 *R, const double *dt_cell,
                              const double *dt_global, double4 *Uout, double *err_partial,

@@ -27,12 +27,6 @@ __device__ __forceinline__ int tile_of_block(int b, int ntiles, int remap) {
     return (b % 8) * per + b / 8;
 }
 
-__device__ __forceinline__ void atomic_min_pos(double *addr, double v) {
-    // non-negative doubles order like their bit patterns; NaN patterns never win
-    atomicMin(reinterpret_cast<unsigned long long *>(addr),
-              static_cast<unsigned long long>(__double_as_longlong(v)));
-}
-
 __device__ __forceinline__ void load_state(const double4 *__restrict__ U, int c, FState &s) {
     double4 u = U[c];
     state_from_U(u.x, u.y, u.z, u.w, s);
@@ -476,7 +470,7 @@ __global__ void __launch_bounds__(256) flux_tile_kernel(DevMesh m, DevCfg cfg,
                 if (st.err_partial) {
                     for (int q = 0; q < 4; q++) errp[q] += err_term(Uo[q] - Un[q], Un[q]);
                 }
-                if (st.dtmin_out) {
+                if (st.dt_partial) {
                     // time step of the new state, for the next step's global dt
                     FState s;
                     state_from_U(Uo[0], Uo[1], Uo[2], Uo[3], s);
@@ -495,10 +489,10 @@ __global__ void __launch_bounds__(256) flux_tile_kernel(DevMesh m, DevCfg cfg,
     }
     // block reductions reuse LDS from its base (the allocation holds >= 4 * blockDim doubles)
     if constexpr (MODE == MODE_RESID) {
-        if (st.dtmin_out) {
+        if (st.dt_partial) {
             __syncthreads();
             double b = block_min(smem, dtmin);
-            if (threadIdx.x == 0) atomic_min_pos(st.dtmin_out, b);
+            if (threadIdx.x == 0) st.dt_partial[tile] = b;
         }
     } else {
         if (st.last && st.err_partial) {
@@ -509,10 +503,10 @@ __global__ void __launch_bounds__(256) flux_tile_kernel(DevMesh m, DevCfg cfg,
                 for (int q = 0; q < 4; q++) st.err_partial[4 * tile + q] = out[q];
             __syncthreads();
         }
-        if (st.last && st.dtmin_out) {
+        if (st.last && st.dt_partial) {
             __syncthreads();
             double b = block_min(smem, dtmin);
-            if (threadIdx.x == 0) atomic_min_pos(st.dtmin_out, b);
+            if (threadIdx.x == 0) st.dt_partial[tile] = b;
         }
     }
 }
@@ -525,22 +519,53 @@ __global__ void fill_kernel(double *p, double v, int n) {
     if (i < n) p[i] = v;
 }
 
-// Sums the per-tile error partials in tile order and takes the square root (Estimate_Error).
-__global__ void err_finalize_kernel(const double *partial, int ntiles, double *err4) {
-    __shared__ double red[4 * 256];
-    const int t = threadIdx.x;
+// Sums the per-tile error partials and takes the square root (Estimate_Error). One workgroup of
+// 1024 threads; thread t sums tiles t, t + 1024, ... (four independent loads in flight), then a
+// fixed-order tree, so the result is deterministic.
+__global__ void __launch_bounds__(1024) err_finalize_kernel(const double *partial, int ntiles,
+                                                            double *err4) {
+    __shared__ double red[4 * 1024];
+    const int t = threadIdx.x, B = blockDim.x;
     double v[4] = {0, 0, 0, 0};
-    for (int i = t; i < ntiles; i += blockDim.x)
+    int i = t;
+    for (; i + 3 * B < ntiles; i += 4 * B) {
+        double a[4][4];
+        for (int u = 0; u < 4; u++)
+            for (int q = 0; q < 4; q++) a[u][q] = partial[4 * (i + u * B) + q];
+        for (int u = 0; u < 4; u++)
+            for (int q = 0; q < 4; q++) v[q] += a[u][q];
+    }
+    for (; i < ntiles; i += B)
         for (int q = 0; q < 4; q++) v[q] += partial[4 * i + q];
-    for (int q = 0; q < 4; q++) red[q * 256 + t] = v[q];
+    for (int q = 0; q < 4; q++) red[q * B + t] = v[q];
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
+    for (int s = B / 2; s > 0; s >>= 1) {
         if (t < s)
-            for (int q = 0; q < 4; q++) red[q * 256 + t] += red[q * 256 + t + s];
+            for (int q = 0; q < 4; q++) red[q * B + t] += red[q * B + t + s];
         __syncthreads();
     }
     if (t == 0)
-        for (int q = 0; q < 4; q++) err4[q] = sqrt(red[q * 256]);
+        for (int q = 0; q < 4; q++) err4[q] = sqrt(red[q * B]);
+}
+
+// Minimum of n per-tile partial minima (fmin ignores NaN, as the reference's `<` scan does).
+__global__ void __launch_bounds__(1024) min_reduce_kernel(const double *partial, int n, double *out) {
+    __shared__ double red[1024];
+    const int t = threadIdx.x, B = blockDim.x;
+    double v = 1.7976931348623157e308;
+    int i = t;
+    for (; i + 3 * B < n; i += 4 * B) {
+        const double a0 = partial[i], a1 = partial[i + B], a2 = partial[i + 2 * B], a3 = partial[i + 3 * B];
+        v = fmin(v, fmin(fmin(a0, a1), fmin(a2, a3)));
+    }
+    for (; i < n; i += B) v = fmin(v, partial[i]);
+    red[t] = v;
+    __syncthreads();
+    for (int s = B / 2; s > 0; s >>= 1) {
+        if (t < s) red[t] = fmin(red[t], red[t + s]);
+        __syncthreads();
+    }
+    if (t == 0) *out = red[0];
 }
 
 // The reference's Runge_Kutta_Method / Explicit_Method updates (src/Numerical_Method.cpp:26-103,
@@ -677,7 +702,12 @@ hipError_t launch_fill(double *p, double v, int n, hipStream_t s) {
 }
 
 hipError_t launch_err_finalize(const double *partial, int n, double *err4, hipStream_t s) {
-    hipLaunchKernelGGL(err_finalize_kernel, dim3(1), dim3(256), 0, s, partial, n, err4);
+    hipLaunchKernelGGL(err_finalize_kernel, dim3(1), dim3(1024), 0, s, partial, n, err4);
+    return hipGetLastError();
+}
+
+hipError_t launch_min_reduce(const double *partial, int n, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(min_reduce_kernel, dim3(1), dim3(1024), 0, s, partial, n, out);
     return hipGetLastError();
 }
 
