@@ -25,6 +25,25 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# rocprofv3 summaries of the flux kernel per workload, with calibrated FETCH_SIZE + WRITE_SIZE
+PROFILES = {"bump": "profiles/flux_tile_kernel_bump1400x700.md",
+            "ffs": "profiles/flux_tile_kernel_ffs4680x1560.md"}
+
+
+def measured_traffic(workload, config):
+    """HBM bytes per launch of the flux kernel recorded in the committed profile summary for
+    this workload, when that summary was taken with the same launch configuration."""
+    path = os.path.join(ROOT, PROFILES[workload])
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        if line.startswith("traffic_"):
+            k, _, v = line.partition(":")
+            vals[k.strip()] = v.strip()
+    if vals.get("traffic_config") != config or "traffic_bytes_per_launch" not in vals:
+        return None
+    return float(vals["traffic_bytes_per_launch"])
 
 
 def parse():
@@ -149,6 +168,8 @@ def main():
     ok = bool(np.all(np.isfinite(err)))
 
     t_step = ms_steps / 1e3 / args.steps
+    launch_cfg = (f"workload={args.workload} tile={args.tile} face_budget={args.face_budget} "
+                  f"block={args.block} grad_lds={args.grad_lds}")
     value = F * nres / t_step
     bflux = flux_pass_bytes(C, F)
     achieved = bflux / (ms_flux / 1e3) / 1e9
@@ -167,14 +188,16 @@ def main():
         "data": "synthetic (seeded generated mesh, freestream initial state)",
         "config": {"workload": desc, "cells": C, "faces": F, "tile": args.tile,
                    "face_budget": args.face_budget, "block": args.block, "grad_lds": args.grad_lds,
-                   "residual_evals_per_step": nres},
+                   "residual_evals_per_step": nres, "launch": launch_cfg},
         "cell_updates_per_s": C / t_step,
         "wall_s_timed": wall,
         "finite": ok,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(args.workload, launch_cfg),
                      "kernel": "flux_tile_kernel (fused face flux + gather + RK stage)",
-                     "algorithmic_bytes_per_launch": bflux, "avg_launch_ms": ms_flux},
+                     "algorithmic_bytes_per_launch": bflux, "avg_launch_ms": ms_flux,
+                     "traffic_source": PROFILES[args.workload]},
         "grad_pass": {"avg_launch_ms": ms_grad,
                       "algorithmic_bytes_per_launch": grad_pass_bytes(C, F, cfg["gradient"]),
                       "achieved_GBs": grad_pass_bytes(C, F, cfg["gradient"]) / (ms_grad / 1e3) / 1e9},

@@ -3,10 +3,11 @@
  * TEST INFRASTRUCTURE ONLY: the checker of the HIP implementation. Loaded by tests/,
  * __graft_entry__.smoke() and bench.py's cpu_baseline leg; never by the product path.
  *
- * Build: gcc -O2 -ffp-contract=off -fPIC -shared (oracle/build_oracle.py). Contraction is off
+ * Build: gcc -O2 -ffp-contract=off -fPIC -shared (build_oracle() in oracle/oracle.py). Contraction is off
  * so that expressions round exactly as the reference's g++ x86-64 build does.
  *
- * Parity pins (tests/test_oracle_*.py):
+ * Parity pins (tests/test_oracle_golden.py against the reference, tests/test_oracle_math.py
+ * against closed forms and invariants):
  *   om_build/om_export ........ bit-exact vs the reference build (oracle/_ref) and its golden
  *                               fixtures in tests/golden (Sod, jittered mixed meshes, the
  *                               reference's own Ramp_15o_52_18 grid); closed forms (shoelace,
@@ -20,13 +21,20 @@
  *   or_ausm_cell .............. bit-exact vs Ausm_Flux.
  *   or_roe2o_residual ......... bit-exact vs Evaluate_Cell_Net_Flux_2O (ROE_2O).
  *   HLLC (no CPU reference) ... consistency F(W,W) = F(W).n, supersonic upwinding, stationary
- *                               contact, Toro's exact Riemann solver on tests 1-5 (1-D tube).
- *   limiters .................. closed forms at r in {-1,0,0.5,1,2,3}, TVD region, symmetry.
- *   WLSQ ...................... exactness for linear fields on jittered mixed meshes; brute
- *                               force normal equations on a tiny mesh.
- *   RK3/RK4 ................... order of accuracy on du/dt = lambda u via a single-cell
- *                               periodic problem; equal to the reference delta form for
- *                               global dt.
+ *                               contact, rotational invariance, antisymmetry; Sod, Lax and
+ *                               Toro test 3 tubes against Toro's exact Riemann solution.
+ *   Roe / LLF / AUSM .......... the same flux properties (where the scheme has them).
+ *   limiters .................. closed forms of phi(r) at r in {-1,0,0.25,0.5,1,2,3,10}, TVD
+ *                               region, symmetry phi(r)/r = phi(1/r).
+ *   MUSCL (on_residual order 2) both sides reconstruct the same state for linear fields, so the
+ *                               dissipation of Roe, LLF and HLLC vanishes; MUSCL beats first
+ *                               order on the shock tubes; freestream preservation.
+ *   WLSQ / GG ................. exactness for linear fields (WLSQ on jittered mixed meshes, GG on
+ *                               uniform quads); brute-force normal equations on a tiny mesh.
+ *   or_rk_stage ............... convergence order 3 (TVD-RK3) and 4 (RK4) on a linear
+ *                               oscillator; RK4 and RK3 agree on Sod.
+ *   far field / freestream .... Riemann invariants, entropy and tangential velocity of the
+ *                               upwind side; supersonic limits; freestream is a fixed point.
  */
 #include "cfd_oracle.h"
 

@@ -3,7 +3,7 @@
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
 cpu_baseline leg. The product package never imports this module.
 
-`Oracle` wraps oracle/_build/liboracle.so (built by oracle/build_oracle.py).
+`Oracle` wraps oracle/_build/liboracle.so (built by build_oracle() below).
 `Reference` wraps oracle/_ref/libcfdref.so, the reference solver compiled from its own sources
 by oracle/Makefile.ref; it exists only where that build ran.
 """
