@@ -83,11 +83,13 @@ def workload(name):
     return m, cfg, desc
 
 
-def flux_pass_bytes(C, F):
-    # fused flux/stage pass, first stage: per cell U (32) + gradients (64) + area, 1/area (16)
-    # + face slots (8) + dt/area out (8) + new U out (32); per face L,R (8) + normal, length (24)
-    # + centre offset (16)
-    return 160 * C + 48 * F
+def flux_pass_bytes(C, F, cfg):
+    """Algorithmic bytes of the first-stage launch of the fused flux/stage pass. Per cell: U (32)
+    + gradients (64) + area, 1/area (16) + face slots (8) read; new U (32) written, plus dt/area
+    (8) with local time stepping and the accumulator (32) with RK4. Per face: L,R (8) + normal,
+    length (24) + centre offset (16)."""
+    per_cell = 32 + 64 + 16 + 8 + 32 + (8 if cfg["lts"] else 0) + (32 if cfg["rk"] == 4 else 0)
+    return per_cell * C + 48 * F
 
 
 def grad_pass_bytes(C, F, gradient):
@@ -171,7 +173,7 @@ def main():
     launch_cfg = (f"workload={args.workload} tile={args.tile} face_budget={args.face_budget} "
                   f"block={args.block} grad_lds={args.grad_lds}")
     value = F * nres / t_step
-    bflux = flux_pass_bytes(C, F)
+    bflux = flux_pass_bytes(C, F, cfg)
     achieved = bflux / (ms_flux / 1e3) / 1e9
     out = {
         "metric": "FP64 face-flux evals/s (time-to-1e-8 NN-warm vs plain: not measured in this line)",

@@ -27,13 +27,19 @@ def read_stats(path):
     return rows
 
 
-def counter_per_dispatch(path, kernel, counter):
-    """Average counter value (KiB) over the dispatches whose kernel name contains `kernel`."""
-    vals = []
+def counter_per_dispatch(path, kernel, counter, ordinals=None):
+    """Average counter value (KiB) over the dispatches whose kernel name contains `kernel`;
+    `ordinals` (start, stop) keeps only those dispatches of that kernel, counted from 0 in
+    dispatch order."""
+    rows = []
     with open(path) as f:
         for r in csv.DictReader(f):
             if r["Counter_Name"] == counter and kernel in r["Kernel_Name"]:
-                vals.append(float(r["Counter_Value"]))
+                rows.append((int(r["Dispatch_Id"]), float(r["Counter_Value"])))
+    rows.sort()
+    vals = [v for _, v in rows]
+    if ordinals:
+        vals = vals[ordinals[0]:ordinals[1]]
     if not vals:
         raise SystemExit(f"no {counter} rows for {kernel} in {path}")
     return sum(vals) / len(vals), len(vals)
@@ -56,6 +62,8 @@ def main(argv=None):
     p.add_argument("--copy-kernel", default="copy")
     p.add_argument("--kernel", default="flux_tile_kernel")
     p.add_argument("--algorithmic-bytes", type=float)
+    p.add_argument("--ordinals", default="",
+                   help="start:stop of the kernel's dispatches to average (default: all)")
     p.add_argument("--config", default="")
     p.add_argument("--note", action="append", default=[])
     a = p.parse_args(argv)
@@ -68,8 +76,9 @@ def main(argv=None):
         out.append(f"| `{short(r['name'])}` | {r['calls']} | {r['avg_ns'] / 1e3:.1f} | "
                    f"{r['min_ns'] / 1e3:.1f} | {r['max_ns'] / 1e3:.1f} | {r['pct']:.2f} |")
     if a.fetch and a.write and a.copy_fetch and a.copy_write:
-        f_kib, nf = counter_per_dispatch(a.fetch, a.kernel, "FETCH_SIZE")
-        w_kib, nw = counter_per_dispatch(a.write, a.kernel, "WRITE_SIZE")
+        ords = tuple(int(x) for x in a.ordinals.split(":")) if a.ordinals else None
+        f_kib, nf = counter_per_dispatch(a.fetch, a.kernel, "FETCH_SIZE", ords)
+        w_kib, nw = counter_per_dispatch(a.write, a.kernel, "WRITE_SIZE", ords)
         cf_kib, _ = counter_per_dispatch(a.copy_fetch, a.copy_kernel, "FETCH_SIZE")
         cw_kib, _ = counter_per_dispatch(a.copy_write, a.copy_kernel, "WRITE_SIZE")
         kf = a.copy_bytes / (cf_kib * 1024)
