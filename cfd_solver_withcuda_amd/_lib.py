@@ -63,6 +63,7 @@ def lib() -> ctypes.CDLL:
     _sig(L, "cfd_state_device", _i, [_vp, ctypes.POINTER(_vp)])
     _sig(L, "cfd_residual", _i, [_vp, _dp, _dp, _dp, _vp])
     _sig(L, "cfd_step", _i, [_vp, _i, _dp, _dp, _vp])
+    _sig(L, "cfd_apply_bc", _i, [_vp, _dp, _vp])
     _sig(L, "cfd_run_steady", _i, [_vp, _d, _i, _i, _ip, _dp, _vp])
     _sig(L, "cfd_time_pass", _i, [_vp, _i, _i, ctypes.POINTER(ctypes.c_float), _vp])
     _sig(L, "cfd_device_alloc", _i, [ctypes.c_size_t, ctypes.POINTER(_vp)])

@@ -53,9 +53,11 @@ struct cfd_mesh_s {
     double *d_fwL = nullptr, *d_fwR = nullptr;
     uint16_t *d_slots = nullptr;
     double *d_area = nullptr, *d_inv_area = nullptr;
+    int *d_gcell = nullptr, *d_gkind = nullptr;  // ghost k: renumbered owner cell, face kind
+    double *d_gnrm = nullptr;                    // ghost k: unit normal of the owner's face
     DevMesh dev{};
     ~cfd_mesh_s() {
-        void *ps[] = {d_tile_cstart, d_tile_fstart, d_fL, d_fR, d_fnx, d_fny, d_fdl, d_fdx, d_fdy, d_fwL, d_fwR, d_slots, d_area, d_inv_area};
+        void *ps[] = {d_tile_cstart, d_tile_fstart, d_fL, d_fR, d_fnx, d_fny, d_fdl, d_fdx, d_fdy, d_fwL, d_fwR, d_slots, d_area, d_inv_area, d_gcell, d_gkind, d_gnrm};
         for (void *p : ps)
             if (p) (void)hipFree(p);
     }
@@ -68,11 +70,12 @@ struct cfd_solver_s {
     double4 *Un = nullptr, *Ua = nullptr, *Ub = nullptr, *acc = nullptr, *R = nullptr;
     double *dt = nullptr, *grad = nullptr, *hbuf = nullptr, *err_partial = nullptr, *dt_partial = nullptr;
     double *scal = nullptr;  // [0],[1] global dt slots, [2] stage minimum, [3..6] error norm
+    double4 *Ug = nullptr;   // ghost states (reference ghost order), allocated on first use
     int cur = 0;
     bool dt_valid = false;
     bool has_state = false;
     ~cfd_solver_s() {
-        void *ps[] = {Un, Ua, Ub, acc, R, dt, grad, hbuf, err_partial, dt_partial, scal};
+        void *ps[] = {Un, Ua, Ub, acc, R, dt, grad, hbuf, err_partial, dt_partial, scal, Ug};
         for (void *p : ps)
             if (p) (void)hipFree(p);
     }
@@ -140,6 +143,23 @@ int cfd_mesh_create(int npts, const double *pts3, int ncells, const int *cells4,
         (e = upload(&m->d_area, L.area)) != hipSuccess || (e = upload(&m->d_inv_area, L.inv_area)) != hipSuccess) {
         delete m;
         return hip_fail(e, "cfd_mesh_create upload");
+    }
+    {
+        const RefMesh &R = m->ref;
+        std::vector<int> gc(R.ng), gk(R.ng);
+        std::vector<double> gn(2 * (size_t)R.ng);
+        for (int k = 0; k < R.ng; k++) {
+            const int c = R.gcell[k], f = R.gface[k];
+            gc[k] = L.iperm[c];
+            gk[k] = R.kind[4 * c + f];
+            gn[2 * k] = R.nrm[8 * c + 2 * f];
+            gn[2 * k + 1] = R.nrm[8 * c + 2 * f + 1];
+        }
+        if ((e = upload(&m->d_gcell, gc)) != hipSuccess || (e = upload(&m->d_gkind, gk)) != hipSuccess ||
+            (e = upload(&m->d_gnrm, gn)) != hipSuccess) {
+            delete m;
+            return hip_fail(e, "cfd_mesh_create upload");
+        }
     }
     DevMesh &d = m->dev;
     d.nc = m->ref.nc;
@@ -369,6 +389,23 @@ int cfd_residual(cfd_solver_t *s, double *R, double *dt, double *grad, void *str
         std::vector<double> h(8 * (size_t)nc);
         HIP_TRY(hipMemcpy(h.data(), s->grad, h.size() * sizeof(double), hipMemcpyDeviceToHost));
         for (int k = 0; k < nc; k++) std::memcpy(&grad[8 * (size_t)perm[k]], &h[8 * (size_t)k], 8 * sizeof(double));
+    }
+    return 0;
+}
+
+int cfd_apply_bc(cfd_solver_t *s, double *U_ghost, void *stream) {
+    if (!s) return fail(CFD_E_ARG, "cfd_apply_bc: null solver");
+    if (!s->has_state) return fail(CFD_E_STATE, "cfd_apply_bc: state not set");
+    const int ng = s->m->ref.ng;
+    if (ng == 0) return 0;
+    hipStream_t st = (hipStream_t)stream;
+    if (!s->Ug) HIP_TRY(hipMalloc((void **)&s->Ug, (size_t)ng * sizeof(double4)));
+    const cfd_mesh_s *m = s->m;
+    HIP_TRY(launch_ghosts(ng, m->d_gcell, reinterpret_cast<const double2 *>(m->d_gnrm), m->d_gkind,
+                          s->dcfg, s->Un, nullptr, s->Ug, st));
+    if (U_ghost) {
+        HIP_TRY(hipMemcpyAsync(U_ghost, s->Ug, (size_t)ng * sizeof(double4), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
     }
     return 0;
 }

@@ -35,6 +35,9 @@ hipError_t launch_flux(const DevMesh &m, const DevCfg &c, int flux, int order, i
                        hipStream_t s);
 hipError_t launch_fill(double *p, double v, int n, hipStream_t s);
 hipError_t launch_err_finalize(const double *partial, int n, double *err4, hipStream_t s);
+hipError_t launch_ghosts(int n, const int *gcell, const double2 *gnrm, const int *gkind,
+                         const DevCfg &c, const double4 *Us, const double4 *Uexit, double4 *Ug,
+                         hipStream_t s);
 hipError_t launch_min_reduce(const double *partial, int n, double *out, hipStream_t s);
 hipError_t launch_ref_update(int nc, int kind, int lts, const double *inv_area, const double4 *Un,
                              const double4 *Us, const double4 *R, const double *dt_cell,

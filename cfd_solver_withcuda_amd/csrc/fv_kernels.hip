@@ -548,6 +548,25 @@ __global__ void __launch_bounds__(1024) err_finalize_kernel(const double *partia
         for (int q = 0; q < 4; q++) err4[q] = sqrt(red[q * B]);
 }
 
+// Ghost states of the boundary faces (Apply_Boundary_Conditions, src/Boundary_Conditions.cpp:12-61):
+// ghost k of owner cell gcell[k] across a face of kind gkind[k] with unit normal gnrm[k].
+__global__ void __launch_bounds__(256) ghost_kernel(int n, const int *__restrict__ gcell,
+                                                    const double2 *__restrict__ gnrm,
+                                                    const int *__restrict__ gkind, DevCfg cfg,
+                                                    const double4 *__restrict__ Us,
+                                                    const double4 *__restrict__ Uexit,
+                                                    double4 *__restrict__ Ug) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int c = gcell[k];
+    const double2 nv = gnrm[k];
+    const int bc = cfg.bc[gkind[k]];
+    FState in, g;
+    load_state((bc == BC_EXIT && Uexit) ? Uexit : Us, c, in);
+    ghost_state(bc, in, nv.x, nv.y, cfg.fs, g);
+    Ug[k] = make_double4(g.U[0], g.U[1], g.U[2], g.U[3]);
+}
+
 // Minimum of n per-tile partial minima (fmin ignores NaN, as the reference's `<` scan does).
 __global__ void __launch_bounds__(1024) min_reduce_kernel(const double *partial, int n, double *out) {
     __shared__ double red[1024];
@@ -703,6 +722,14 @@ hipError_t launch_fill(double *p, double v, int n, hipStream_t s) {
 
 hipError_t launch_err_finalize(const double *partial, int n, double *err4, hipStream_t s) {
     hipLaunchKernelGGL(err_finalize_kernel, dim3(1), dim3(1024), 0, s, partial, n, err4);
+    return hipGetLastError();
+}
+
+hipError_t launch_ghosts(int n, const int *gcell, const double2 *gnrm, const int *gkind,
+                         const DevCfg &c, const double4 *Us, const double4 *Uexit, double4 *Ug,
+                         hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(ghost_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, gcell, gnrm, gkind, c, Us, Uexit, Ug);
     return hipGetLastError();
 }
 

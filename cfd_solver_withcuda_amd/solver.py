@@ -203,6 +203,12 @@ class Solver:
               "cfd_residual")
         return R, dt, grad
 
+    def ghost_states(self, stream: int = 0) -> np.ndarray:
+        """Conserved ghost states [nghost, 4] of the current state, ghost k = cell nphys + k."""
+        Ug = np.zeros((self.mesh.nghost, 4))
+        check(lib().cfd_apply_bc(self.h, dptr(Ug), ctypes.c_void_p(stream)), "cfd_apply_bc")
+        return Ug
+
     def step(self, nsteps: int = 1, history: bool = False, stream: int = 0):
         """Advances nsteps; with history returns (err[n,4], dt[n])."""
         if history:

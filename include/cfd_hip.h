@@ -104,6 +104,12 @@ int cfd_state_device(cfd_solver_t *s, void **d_U);
  * gradients, order 2; may be NULL). Host outputs in the caller's order. */
 int cfd_residual(cfd_solver_t *s, double *R, double *dt, double *grad, void *stream);
 
+/* Ghost states of the current state (Apply_Boundary_Conditions, src/Boundary_Conditions.cpp:12-61):
+ * U_ghost[4*nghost] (host, may be NULL to keep them on the device) in the reference's ghost
+ * order, ghost k = cell id nphys + k. The residual and step entries form these states on the fly;
+ * this entry exposes them. */
+int cfd_apply_bc(cfd_solver_t *s, double *U_ghost, void *stream);
+
 /* nsteps solver iterations. err_hist[4*nsteps] (the reference's Error[0..3] per step) and
  * dt_hist[nsteps] (the step's global dt, or min local dt with LTS) may be NULL; without them the
  * steps run without host synchronisation. */

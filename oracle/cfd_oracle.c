@@ -557,6 +557,13 @@ void on_ghost_state(int bc, const double *Win5, double nx, double ny, const doub
     for (int k = 0; k < 5; k++) Wg5[k] = g.W[k];
 }
 
+void on_ghost_cons(int bc, const double *U4, double nx, double ny, const double *fs4, double *Ug4) {
+    FState in, g;
+    state_from_U(U4, &in);
+    ghost_fstate(bc, &in, nx, ny, fs4, &g);
+    for (int k = 0; k < 4; k++) Ug4[k] = g.U[k];
+}
+
 void or_apply_bc_ref(const OMesh *m, int inlet_bc, double rho_in, double p_in, double *U_all,
                      double *prim_all, const double *U_exit_src) {
     double fs[4] = {rho_in, 0.0, 0.0, p_in};

@@ -69,6 +69,8 @@ typedef struct {
 void or_prim(const double *U4, double *prim11);
 void on_ghost_state(int bc, const double *Win5, double nx, double ny, const double *fs4,
                     double *Wg5);
+/* conserved ghost state of a cell with conserved state U4 across a face with unit normal n */
+void on_ghost_cons(int bc, const double *U4, double nx, double ny, const double *fs4, double *Ug4);
 double or_limiter_slope(int limiter, double dminus, double dplus);
 
 /* ---- face-loop residual (the spec of the GPU kernels) ------------------------------------

@@ -125,6 +125,7 @@ class Oracle:
         L.om_export.argtypes = [ctypes.c_void_p] + [_ip, _ip, _ip, _dp, _dp, _dp, _dp, _dp, _dp, _ip, _ip, _ip, _ip]
         L.or_prim.argtypes = [_dp, _dp]
         L.on_ghost_state.argtypes = [ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double, _dp, _dp]
+        L.on_ghost_cons.argtypes = [ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double, _dp, _dp]
         L.or_limiter_slope.restype = ctypes.c_double
         L.or_limiter_slope.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double]
         cfgp = ctypes.POINTER(ONatCfg)
@@ -184,6 +185,13 @@ class Oracle:
         f = np.ascontiguousarray(fs, np.float64)
         out = np.zeros(5)
         self.lib.on_ghost_state(OBC[bc], _d(W), nx, ny, _d(f), _d(out))
+        return out
+
+    def ghost_cons(self, bc: str, U4, nx, ny, fs) -> np.ndarray:
+        U = np.ascontiguousarray(U4, np.float64)
+        f = np.ascontiguousarray(fs, np.float64)
+        out = np.zeros(4)
+        self.lib.on_ghost_cons(OBC[bc], _d(U), nx, ny, _d(f), _d(out))
         return out
 
     def rk_stage(self, rk, stage, Un, Us, acc, R, h, out):

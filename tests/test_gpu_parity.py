@@ -114,6 +114,29 @@ def test_boundary_types_vs_oracle(orc, bump, bc):
         assert np.array_equal(R, Ro)
 
 
+@pytest.mark.parametrize("bcs", [BCS_NATIVE, ("farfield", "exit", "symmetry", "wall", "wall"),
+                                 ("inlet_ref", "subsonic_inlet_ref", "freestream", "symmetry", "wall")])
+def test_ghost_states_vs_oracle(orc, bump, bcs):
+    """cfd_apply_bc: every ghost (reference ghost order) equals the oracle's ghost of its owner
+    cell; bit-exact except the far field (pow, 1e-13 relative)."""
+    print(f"case bcs={bcs}", flush=True)
+    s = Solver(Mesh(bump.points, bump.cells, tile=64), make_config(bc=bcs, fs=FS))
+    a = s.mesh.arrays()
+    U = smooth_state(a.ctr, s.mesh.nphys)
+    s.set_state(U)
+    Ug = s.ghost_states()
+    trip = np.concatenate([a.inlet.reshape(-1, 3), a.exit.reshape(-1, 3), a.wall.reshape(-1, 3)])
+    assert len(trip) == s.mesh.nghost == len(Ug)
+    for c, f, g in trip:
+        kind = a.kind[c, f]
+        want = orc.ghost_cons(bcs[kind - 1], U[c], a.nrm[c, 2 * f], a.nrm[c, 2 * f + 1], FS)
+        got = Ug[g - s.mesh.nphys]
+        if bcs[kind - 1] == "farfield":
+            assert np.allclose(got, want, rtol=1e-13, atol=0), (c, f, got, want)
+        else:
+            assert np.array_equal(got, want), (c, f, bcs[kind - 1], got, want)
+
+
 @pytest.mark.parametrize("flux,order,rk,lts", [("roe", 1, 3, 0), ("hllc", 2, 3, 1), ("ausm", 2, 4, 0),
                                               ("llf", 2, 4, 1), ("hllc", 2, 3, 0)])
 def test_native_steps_vs_oracle(orc, bump, flux, order, rk, lts):
