@@ -28,7 +28,8 @@ class Config(ctypes.Structure):
     """cfd_config_t (include/cfd_hip.h)."""
     _fields_ = [("flux", _i), ("order", _i), ("limiter", _i), ("gradient", _i), ("roe_bdiss", _i),
                 ("cfl", _d), ("bc", _i * 6), ("fs", _d * 4), ("rk", _i), ("lts", _i),
-                ("ref_mode", _i), ("block", _i), ("xcd_remap", _i), ("grad_lds", _i)]
+                ("ref_mode", _i), ("block", _i), ("xcd_remap", _i), ("grad_lds", _i),
+                ("freeze_tol", _d), ("freeze_stall", _i)]
 
 
 def _sig(lib, name, res, args):
@@ -64,6 +65,26 @@ def lib() -> ctypes.CDLL:
     _sig(L, "cfd_residual", _i, [_vp, _dp, _dp, _dp, _vp])
     _sig(L, "cfd_step", _i, [_vp, _i, _dp, _dp, _vp])
     _sig(L, "cfd_apply_bc", _i, [_vp, _dp, _vp])
+    _sig(L, "cfd_set_freeze", _i, [_vp, _i])
+    _sig(L, "cfd_batch_create", _i, [_i, _ip, _dp, _ip, _ip, _dp, cfgp, _i, ctypes.POINTER(_vp)])
+    _sig(L, "cfd_batch_destroy", None, [_vp])
+    _sig(L, "cfd_batch_info", _i, [_vp, _ip, _dp])
+    _sig(L, "cfd_batch_set_state", _i, [_vp, _dp])
+    _sig(L, "cfd_batch_get_state", _i, [_vp, _dp])
+    _sig(L, "cfd_batch_solve", _i, [_vp, _d, _i, _i, _ip, _dp, _ip, _dp, _dp, _vp])
+    _sig(L, "cfd_batch_time_steps", _i, [_vp, _i, ctypes.POINTER(ctypes.c_float)])
+    _fp = ctypes.POINTER(ctypes.c_float)
+    _sig(L, "cfd_mlp_create", _i, [_i, _i, _i, _i, _i, ctypes.c_ulonglong, ctypes.POINTER(_vp)])
+    _sig(L, "cfd_mlp_destroy", None, [_vp])
+    _sig(L, "cfd_mlp_num_params", _i, [_vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)])
+    _sig(L, "cfd_mlp_set_params", _i, [_vp, _fp, _fp])
+    _sig(L, "cfd_mlp_get_params", _i, [_vp, _fp, _fp])
+    _sig(L, "cfd_mlp_set_normalization", _i, [_vp, _fp, _fp, _fp, _fp])
+    _sig(L, "cfd_mlp_gradients", _i, [_vp, _fp, _fp, _i, _fp, _fp, _dp])
+    _sig(L, "cfd_mlp_train", _i, [_vp, _fp, _fp, _i, _i, _i, ctypes.c_float, ctypes.c_float,
+                                  ctypes.c_ulonglong, _fp, _i])
+    _sig(L, "cfd_mlp_predict", _i, [_vp, _fp, _i, _fp])
+    _sig(L, "cfd_mlp_time_step", _i, [_vp, _i, _i, _fp])
     _sig(L, "cfd_run_steady", _i, [_vp, _d, _i, _i, _ip, _dp, _vp])
     _sig(L, "cfd_time_pass", _i, [_vp, _i, _i, ctypes.POINTER(ctypes.c_float), _vp])
     _sig(L, "cfd_device_alloc", _i, [ctypes.c_size_t, ctypes.POINTER(_vp)])
@@ -84,6 +105,10 @@ def check(rc: int, what: str = "") -> int:
 
 def dptr(a):
     return a.ctypes.data_as(_dp)
+
+
+def fptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
 
 
 def iptr(a):

@@ -44,6 +44,15 @@ hipError_t upload(T **d, const std::vector<T> &h) {
 
 }  // namespace
 
+struct cfd_solver_s;
+static int freeze_init(cfd_solver_s *s, int mode);
+
+namespace cfd {
+// Error reporting shared with the other C-ABI translation units (one message per thread).
+int capi_fail(int code, const std::string &msg) { return fail(code, msg); }
+int capi_hip_fail(hipError_t e, const char *where) { return hip_fail(e, where); }
+}  // namespace cfd
+
 struct cfd_mesh_s {
     RefMesh ref;
     Layout lay;
@@ -71,11 +80,14 @@ struct cfd_solver_s {
     double *dt = nullptr, *grad = nullptr, *hbuf = nullptr, *err_partial = nullptr, *dt_partial = nullptr;
     double *scal = nullptr;  // [0],[1] global dt slots, [2] stage minimum, [3..6] error norm
     double4 *Ug = nullptr;   // ghost states (reference ghost order), allocated on first use
+    double *psi = nullptr;   // [8 * face entries] frozen limiter factors, allocated on first use
+    int *fz = nullptr;       // [0] freezing state, [1] steps since the best Error[0]
+    double *fz_best = nullptr;
     int cur = 0;
     bool dt_valid = false;
     bool has_state = false;
     ~cfd_solver_s() {
-        void *ps[] = {Un, Ua, Ub, acc, R, dt, grad, hbuf, err_partial, dt_partial, scal, Ug};
+        void *ps[] = {Un, Ua, Ub, acc, R, dt, grad, hbuf, err_partial, dt_partial, scal, Ug, psi, fz, fz_best};
         for (void *p : ps)
             if (p) (void)hipFree(p);
     }
@@ -166,6 +178,7 @@ int cfd_mesh_create(int npts, const double *pts3, int ncells, const int *cells4,
     d.ntiles = L.ntiles;
     d.tile = L.tile;
     d.max_tile_faces = L.max_tile_faces;
+    d.nfe = (int)L.fL.size();
     d.tile_cstart = m->d_tile_cstart;
     d.tile_fstart = m->d_tile_fstart;
     d.fL = m->d_fL;
@@ -256,6 +269,7 @@ static int check_config(const cfd_mesh_s *m, const cfd_config_t *c) {
     if (c->limiter < 0 || c->limiter > 3) return fail(CFD_E_ARG, "config: limiter must be 0..3");
     if (c->gradient < 0 || c->gradient > 1) return fail(CFD_E_ARG, "config: gradient must be 0 or 1");
     if (!(c->cfl > 0.0)) return fail(CFD_E_ARG, "config: cfl must be positive");
+    if (!(c->freeze_tol >= 0.0) || c->freeze_stall < 0) return fail(CFD_E_ARG, "config: freeze_tol and freeze_stall must be >= 0");
     if (c->rk != 1 && c->rk != 3 && c->rk != 4) return fail(CFD_E_ARG, "config: rk must be 1, 3 or 4");
     if (c->rk == 1 && !c->ref_mode) return fail(CFD_E_ARG, "config: forward Euler (rk 1) exists in reference mode only");
     if (c->ref_mode && c->rk == 4) return fail(CFD_E_ARG, "config: the reference iteration has no RK4");
@@ -326,6 +340,7 @@ int cfd_set_state(cfd_solver_t *s, const double *U) {
     HIP_TRY(hipMemcpy(s->Un, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
     s->has_state = true;
     s->dt_valid = false;
+    if (s->fz) return freeze_init(s, 0);  // a new state starts with the limiter evaluated
     return 0;
 }
 
@@ -410,9 +425,40 @@ int cfd_apply_bc(cfd_solver_t *s, double *U_ghost, void *stream) {
     return 0;
 }
 
+static bool auto_freeze(const cfd_solver_s *s) {
+    return s->cfg.order == 2 && !s->cfg.ref_mode && (s->cfg.freeze_tol > 0.0 || s->cfg.freeze_stall > 0);
+}
+
+// Allocates the limiter-freezing buffers and sets the state to `mode` (best Error[0] reset).
+static int freeze_init(cfd_solver_s *s, int mode) {
+    if (!s->psi) {
+        HIP_TRY(hipMalloc((void **)&s->psi, 8 * (size_t)s->m->dev.nfe * sizeof(double)));
+        HIP_TRY(hipMalloc((void **)&s->fz, 2 * sizeof(int)));
+        HIP_TRY(hipMalloc((void **)&s->fz_best, sizeof(double)));
+    }
+    const int h[2] = {mode, 0};
+    const double b = 1.7976931348623157e308;
+    HIP_TRY(hipMemcpy(s->fz, h, sizeof(h), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(s->fz_best, &b, sizeof(b), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int cfd_set_freeze(cfd_solver_t *s, int mode) {
+    if (!s || mode < 0 || mode > 2) return fail(CFD_E_ARG, "cfd_set_freeze: mode must be 0, 1 or 2");
+    if (s->cfg.order != 2 || s->cfg.ref_mode) return fail(CFD_E_ARG, "cfd_set_freeze: limiter freezing needs order 2 in native mode");
+    return freeze_init(s, mode);
+}
+
 // One native step (fused stages). want_err: emit the error norm into scal[3..6].
 static int native_step(cfd_solver_s *s, bool want_err, hipStream_t st) {
     const cfd_config_t &c = s->cfg;
+    if (auto_freeze(s)) {
+        want_err = true;
+        if (!s->fz) {
+            int rc = freeze_init(s, 0);
+            if (rc) return rc;
+        }
+    }
     const DevMesh &dm = s->m->dev;
     if (!c.lts && !s->dt_valid) {
         int rc = residual_pass(s, s->Un, s->scal + s->cur, nullptr, st, 1);
@@ -440,10 +486,14 @@ static int native_step(cfd_solver_s *s, bool want_err, hipStream_t st) {
         a.dt_global = s->scal + s->cur;
         a.err_partial = (a.last && want_err) ? s->err_partial : nullptr;
         a.dt_partial = (a.last && !c.lts) ? s->dt_partial : nullptr;
+        a.freeze = s->fz;
+        a.psi = s->psi;
         HIP_TRY(launch_flux(dm, s->dcfg, c.flux, c.order, c.limiter, MODE_STAGE, in, s->grad, a, c.block, st));
     }
     if (!c.lts) HIP_TRY(launch_min_reduce(s->dt_partial, dm.ntiles, s->scal + (1 - s->cur), st));
     if (want_err) HIP_TRY(launch_err_finalize(s->err_partial, dm.ntiles, s->scal + 3, st));
+    if (want_err && auto_freeze(s))
+        HIP_TRY(launch_freeze_update(s->scal + 3, s->fz, s->fz_best, s->fz + 1, c.freeze_tol, c.freeze_stall, st));
     return 0;
 }
 

@@ -29,6 +29,7 @@ struct DevMesh {
     int ntiles;
     int tile;      // maximum cells per tile
     int max_tile_faces;
+    int nfe;       // face entries (all tiles)
     const int *tile_cstart;   // [ntiles+1] first cell of each tile
     const int *tile_fstart;   // [ntiles+1] first face entry of each tile
     const int *fL;            // [nfe] left cell (renumbered)
@@ -38,6 +39,11 @@ struct DevMesh {
     const double2 *fwL, *fwR; // least-squares coefficient vectors of L and R for this face
     const uint16_t *slots;    // [4*nc] (local face << 1) | (cell is R); 0xFFFF = none
     const double *area, *inv_area;  // [nc]
+    // Batched meshes (several independent instances concatenated; null for one mesh):
+    const int *tile_inst;     // [ntiles] instance of each tile
+    const double4 *inst_fs;   // [ninst] prescribed state (rho, u, v, p) of each instance
+    const int *tile_list;     // [ntiles] tiles of the instances still marching
+    const int *tile_count;    // [1] entries of tile_list in use
 };
 
 // Per-launch scheme settings.

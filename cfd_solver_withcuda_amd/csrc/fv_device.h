@@ -349,15 +349,27 @@ __device__ __forceinline__ double limited_slope(double a, double b) {
 // MUSCL face state on one side: W_f = W_c + sgn/2 * slope(dminus, dplus) with
 // dplus = sgn (W_other - W_c), dminus = 2 grad(W_c).d - dplus; falls back to the cell state when
 // the reconstructed density or pressure is not positive.
+// Limiter freezing: the limited slope equals psi * c with c = grad W . d the unlimited central
+// slope; freeze = 1 records psi = slope / c (0 where c = 0) at psi[k * pstride], freeze = 2 uses
+// slope = psi * c (smooth in the state, so steady marches do not stall on limiter switching).
 template <int LIM>
 __device__ __forceinline__ void reconstruct_side(const FState &c, const FState &o, const double *g,
-                                                 double dx, double dy, double sgn, FState &out) {
+                                                 double dx, double dy, double sgn, int freeze,
+                                                 double *psi, int pstride, FState &out) {
     double Wf[4];
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-        double dplus = sgn * (o.W[k] - c.W[k]);
-        double dminus = 2.0 * (g[2 * k] * dx + g[2 * k + 1] * dy) - dplus;
-        Wf[k] = c.W[k] + sgn * 0.5 * limited_slope<LIM>(dminus, dplus);
+        const double dplus = sgn * (o.W[k] - c.W[k]);
+        const double cc = g[2 * k] * dx + g[2 * k + 1] * dy;
+        double slope;
+        if (freeze == 2) {
+            slope = psi[k * pstride] * cc;
+        } else {
+            const double dminus = 2.0 * cc - dplus;
+            slope = limited_slope<LIM>(dminus, dplus);
+            if (freeze == 1) psi[k * pstride] = (cc != 0.0) ? slope / cc : 0.0;
+        }
+        Wf[k] = c.W[k] + sgn * 0.5 * slope;
     }
     if (!(Wf[0] > 0.0) || !(Wf[3] > 0.0)) {
         out = c;

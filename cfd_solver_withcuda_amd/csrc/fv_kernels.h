@@ -24,6 +24,10 @@ struct StageArgs {
     double *dt_partial;  // [ntiles] per-tile minimum of the local time steps
     double *err_partial; // [4*ntiles] per-tile sums of the reference error terms
     const double4 *Uexit;  // exit ghosts copy this state when set (reference RK stages)
+    // Limiter freezing (order 2): freeze[instance] = 0 off, 1 record psi, 2 use recorded psi;
+    // psi [8][nfe] = slope / (grad W . d) per face entry, side (L, R) and variable.
+    const int *freeze;
+    double *psi;
 };
 
 size_t grad_smem_bytes(const DevMesh &m);
@@ -38,6 +42,16 @@ hipError_t launch_err_finalize(const double *partial, int n, double *err4, hipSt
 hipError_t launch_ghosts(int n, const int *gcell, const double2 *gnrm, const int *gkind,
                          const DevCfg &c, const double4 *Us, const double4 *Uexit, double4 *Ug,
                          hipStream_t s);
+hipError_t launch_batch_status(int ninst, const int *inst_tile, const double *err_partial, double tol,
+                               int *status, int *steps, double *err4, int *freeze, double *best,
+                               int *since, double freeze_tol, int freeze_stall, hipStream_t s);
+hipError_t launch_batch_list(int ninst, int ntiles, const int *inst_tile, const int *tile_inst,
+                             const int *status, int *inst_off, int *tile_list, int *tile_count,
+                             hipStream_t s);
+// Freezing state machine of one solve after a step whose error norm is err4[0] (see
+// fv_kernels.hip, freeze_update_kernel).
+hipError_t launch_freeze_update(const double *err4, int *freeze, double *best, int *since,
+                                double freeze_tol, int freeze_stall, hipStream_t s);
 hipError_t launch_min_reduce(const double *partial, int n, double *out, hipStream_t s);
 hipError_t launch_ref_update(int nc, int kind, int lts, const double *inv_area, const double4 *Un,
                              const double4 *Us, const double4 *R, const double *dt_cell,

@@ -27,6 +27,26 @@ __device__ __forceinline__ int tile_of_block(int b, int ntiles, int remap) {
     return (b % 8) * per + b / 8;
 }
 
+// Tile of this workgroup: its XCD-aware position in the tile sequence, or in the list of tiles
+// still marching for a batched mesh. -1 when the workgroup has no tile.
+__device__ __forceinline__ int block_tile(const DevMesh &m, int remap) {
+    const int n = m.tile_list ? *m.tile_count : m.ntiles;
+    const int b = tile_of_block(blockIdx.x, n, remap);
+    if (b >= n) return -1;
+    return m.tile_list ? m.tile_list[b] : b;
+}
+
+// A batched mesh carries one prescribed (freestream / inlet) state per instance.
+__device__ __forceinline__ void tile_cfg(const DevMesh &m, int tile, DevCfg &c) {
+    if (m.inst_fs) {
+        const double4 f = m.inst_fs[m.tile_inst[tile]];
+        c.fs[0] = f.x;
+        c.fs[1] = f.y;
+        c.fs[2] = f.z;
+        c.fs[3] = f.w;
+    }
+}
+
 __device__ __forceinline__ void load_state(const double4 *__restrict__ U, int c, FState &s) {
     double4 u = U[c];
     state_from_U(u.x, u.y, u.z, u.w, s);
@@ -150,8 +170,9 @@ __global__ void __launch_bounds__(256) grad_tile_kernel(DevMesh m, DevCfg cfg,
                                                         const double4 *__restrict__ Us,
                                                         double *__restrict__ grad) {
     extern __shared__ double smem[];
-    const int tile = tile_of_block(blockIdx.x, m.ntiles, cfg.xcd_remap);
-    if (tile >= m.ntiles) return;
+    const int tile = block_tile(m, cfg.xcd_remap);
+    if (tile < 0) return;
+    tile_cfg(m, tile, cfg);
     const int T = m.tile;
     const int FT = m.max_tile_faces;
     const int B = blockDim.x;
@@ -266,7 +287,7 @@ __device__ __forceinline__ void flux_face(const DevMesh &m, const DevCfg &cfg,
                                           const double *__restrict__ grad, const double4 *Uexit,
                                           const double *sW, const double *sU, const double *sG,
                                           double *sF, bool glds, int T, int FT, int c0, int nct,
-                                          int j, const FaceRec &f) {
+                                          int j, const FaceRec &f, int fz, double *psi, int e) {
     const double nx = f.nx, ny = f.ny, dl = f.dl;
     const int lL = f.L - c0;
     const bool locL = (unsigned)lL < (unsigned)nct;
@@ -312,7 +333,8 @@ __device__ __forceinline__ void flux_face(const DevMesh &m, const DevCfg &cfg,
             load_grad(grad, f.L, gL);
         }
         FState fL, fR;
-        reconstruct_side<LIM>(SL, SR, gL, f.dx, f.dy, 1.0, fL);
+        double *pL = fz ? psi + e : nullptr;
+        reconstruct_side<LIM>(SL, SR, gL, f.dx, f.dy, 1.0, fz, pL, m.nfe, fL);
         if (boundary) {
             if (bc == BC_EXIT && Uexit) {
                 FState src;
@@ -328,7 +350,7 @@ __device__ __forceinline__ void flux_face(const DevMesh &m, const DevCfg &cfg,
             } else {
                 load_grad(grad, f.R, gR);
             }
-            reconstruct_side<LIM>(SR, SL, gR, f.dx, f.dy, -1.0, fR);
+            reconstruct_side<LIM>(SR, SL, gR, f.dx, f.dy, -1.0, fz, fz ? pL + 4 * (size_t)m.nfe : nullptr, m.nfe, fR);
         }
         face_flux<FLUX>(boundary, cfg.roe_bdiss, fL, fR, nx, ny, dl, F);
     } else {
@@ -345,8 +367,9 @@ __global__ void __launch_bounds__(256) flux_tile_kernel(DevMesh m, DevCfg cfg,
                                                         const double *__restrict__ grad,
                                                         StageArgs st) {
     extern __shared__ double smem[];
-    const int tile = tile_of_block(blockIdx.x, m.ntiles, cfg.xcd_remap);
-    if (tile >= m.ntiles) return;
+    const int tile = block_tile(m, cfg.xcd_remap);
+    if (tile < 0) return;
+    tile_cfg(m, tile, cfg);
     const int T = m.tile;
     const int FT = m.max_tile_faces;
     const int B = blockDim.x;
@@ -354,6 +377,7 @@ __global__ void __launch_bounds__(256) flux_tile_kernel(DevMesh m, DevCfg cfg,
     const int nct = m.tile_cstart[tile + 1] - c0;
     const int fs = m.tile_fstart[tile], nft = m.tile_fstart[tile + 1] - fs;
     const bool glds = (ORDER == 2) && cfg.grad_lds;
+    const int fz = (ORDER == 2 && st.freeze) ? st.freeze[m.tile_inst ? m.tile_inst[tile] : 0] : 0;
     double *sW = smem;            // [5][T]
     double *sU = sW + 5 * T;      // [4][T]
     double *sF = sU + 4 * T;      // [6][FT]: flux (4), spectral terms of L and R
@@ -396,12 +420,12 @@ __global__ void __launch_bounds__(256) flux_tile_kernel(DevMesh m, DevCfg cfg,
     __syncthreads();
 
     // ---- B: one flux per face ----------------------------------------------------------
-    if (h0) flux_face<FLUX, ORDER, LIM>(m, cfg, Us, grad, st.Uexit, sW, sU, sG, sF, glds, T, FT, c0, nct, k, f0);
-    if (h1) flux_face<FLUX, ORDER, LIM>(m, cfg, Us, grad, st.Uexit, sW, sU, sG, sF, glds, T, FT, c0, nct, k + B, f1);
+    if (h0) flux_face<FLUX, ORDER, LIM>(m, cfg, Us, grad, st.Uexit, sW, sU, sG, sF, glds, T, FT, c0, nct, k, f0, fz, st.psi, fs + k);
+    if (h1) flux_face<FLUX, ORDER, LIM>(m, cfg, Us, grad, st.Uexit, sW, sU, sG, sF, glds, T, FT, c0, nct, k + B, f1, fz, st.psi, fs + k + B);
     for (int j = k + 2 * B; j < nft; j += B) {
         FaceRec f;
         load_face<true, ORDER == 2>(m, fs + j, f);
-        flux_face<FLUX, ORDER, LIM>(m, cfg, Us, grad, st.Uexit, sW, sU, sG, sF, glds, T, FT, c0, nct, j, f);
+        flux_face<FLUX, ORDER, LIM>(m, cfg, Us, grad, st.Uexit, sW, sU, sG, sF, glds, T, FT, c0, nct, j, f, fz, st.psi, fs + j);
     }
     __syncthreads();
 
@@ -567,6 +591,120 @@ __global__ void __launch_bounds__(256) ghost_kernel(int n, const int *__restrict
     Ug[k] = make_double4(g.U[0], g.U[1], g.U[2], g.U[3]);
 }
 
+// ---- batched meshes ------------------------------------------------------------------------
+// After a step of a batch of independent instances: each marching instance (status 1) sums its
+// tiles' error partials [inst_tile[i], inst_tile[i+1]) in a fixed order (Estimate_Error of that
+// instance), counts the step and stops when Error[0] <= tol (status 0) or the norm is not finite
+// (status -1). One workgroup of 256 threads per instance.
+__device__ __forceinline__ void freeze_step(double e0, int &fz, double &best, int &since,
+                                            double freeze_tol, int freeze_stall) {
+    // Limiter freezing: a step that recorded psi (1) switches to frozen (2); otherwise freezing
+    // starts (record on the next step) once Error[0] <= freeze_tol, or once Error[0] has not
+    // dropped below 0.9 x its best value for freeze_stall steps (a limiter-switching stall).
+    if (fz == 1) {
+        fz = 2;
+    } else if (fz == 0 && (freeze_tol > 0.0 || freeze_stall > 0)) {
+        if (e0 < 0.9 * best) {
+            best = e0;
+            since = 0;
+        } else {
+            since += 1;
+        }
+        if (e0 <= freeze_tol || (freeze_stall > 0 && since >= freeze_stall)) fz = 1;
+    }
+}
+
+__global__ void __launch_bounds__(1) freeze_update_kernel(const double *__restrict__ err4, int *fz,
+                                                          double *best, int *since, double freeze_tol,
+                                                          int freeze_stall) {
+    int f = *fz, sn = *since;
+    double b = *best;
+    freeze_step(err4[0], f, b, sn, freeze_tol, freeze_stall);
+    *fz = f;
+    *best = b;
+    *since = sn;
+}
+
+__global__ void __launch_bounds__(256) batch_status_kernel(const int *__restrict__ inst_tile,
+                                                           const double *__restrict__ err_partial,
+                                                           double tol, int *status, int *steps,
+                                                           double *err4, int *freeze, double *best,
+                                                           int *since, double freeze_tol,
+                                                           int freeze_stall) {
+    __shared__ double red[4 * 256];
+    const int i = blockIdx.x, t = threadIdx.x;
+    if (status[i] != 1) return;
+    const int t0 = inst_tile[i], t1 = inst_tile[i + 1];
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int j = t0 + t; j < t1; j += 256)
+        for (int q = 0; q < 4; q++) v[q] += err_partial[4 * j + q];
+    for (int q = 0; q < 4; q++) red[q * 256 + t] = v[q];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (t < s)
+            for (int q = 0; q < 4; q++) red[q * 256 + t] += red[q * 256 + t + s];
+        __syncthreads();
+    }
+    if (t == 0) {
+        double e[4];
+        for (int q = 0; q < 4; q++) {
+            e[q] = sqrt(red[q * 256]);
+            err4[4 * i + q] = e[q];
+        }
+        steps[i] += 1;
+        if (!isfinite(e[0]) || !isfinite(e[1]) || !isfinite(e[2]) || !isfinite(e[3])) status[i] = -1;
+        else if (e[0] <= tol) status[i] = 0;
+        if (freeze) {
+            int f = freeze[i], sn = since[i];
+            double b = best[i];
+            freeze_step(e[0], f, b, sn, freeze_tol, freeze_stall);
+            freeze[i] = f;
+            best[i] = b;
+            since[i] = sn;
+        }
+    }
+}
+
+// Offsets of the marching instances' tiles in the compacted tile list, and its length. One
+// workgroup of 1024 threads scans the instances in chunks of 1024.
+__global__ void __launch_bounds__(1024) batch_list_scan_kernel(int ninst, const int *__restrict__ inst_tile,
+                                                               const int *__restrict__ status,
+                                                               int *__restrict__ inst_off,
+                                                               int *__restrict__ tile_count) {
+    __shared__ int buf[1024];
+    const int t = threadIdx.x;
+    int carry = 0;
+    for (int base = 0; base < ninst; base += 1024) {
+        const int i = base + t;
+        const int v = (i < ninst && status[i] == 1) ? inst_tile[i + 1] - inst_tile[i] : 0;
+        buf[t] = v;
+        __syncthreads();
+        for (int d = 1; d < 1024; d <<= 1) {
+            const int add = (t >= d) ? buf[t - d] : 0;
+            __syncthreads();
+            buf[t] += add;
+            __syncthreads();
+        }
+        if (i < ninst) inst_off[i] = carry + buf[t] - v;
+        carry += buf[1023];
+        __syncthreads();
+    }
+    if (t == 0) *tile_count = carry;
+}
+
+// Writes the compacted list: tile j of marching instance i goes to inst_off[i] + j - inst_tile[i].
+__global__ void __launch_bounds__(256) batch_list_fill_kernel(int ntiles, const int *__restrict__ tile_inst,
+                                                              const int *__restrict__ inst_tile,
+                                                              const int *__restrict__ status,
+                                                              const int *__restrict__ inst_off,
+                                                              int *__restrict__ tile_list) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ntiles) return;
+    const int i = tile_inst[j];
+    if (status[i] != 1) return;
+    tile_list[inst_off[i] + j - inst_tile[i]] = j;
+}
+
 // Minimum of n per-tile partial minima (fmin ignores NaN, as the reference's `<` scan does).
 __global__ void __launch_bounds__(1024) min_reduce_kernel(const double *partial, int n, double *out) {
     __shared__ double red[1024];
@@ -730,6 +868,31 @@ hipError_t launch_ghosts(int n, const int *gcell, const double2 *gnrm, const int
                          hipStream_t s) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(ghost_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, gcell, gnrm, gkind, c, Us, Uexit, Ug);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_status(int ninst, const int *inst_tile, const double *err_partial, double tol,
+                               int *status, int *steps, double *err4, int *freeze, double *best,
+                               int *since, double freeze_tol, int freeze_stall, hipStream_t s) {
+    if (ninst <= 0) return hipSuccess;
+    hipLaunchKernelGGL(batch_status_kernel, dim3(ninst), dim3(256), 0, s, inst_tile, err_partial, tol, status, steps, err4,
+                       freeze, best, since, freeze_tol, freeze_stall);
+    return hipGetLastError();
+}
+
+hipError_t launch_freeze_update(const double *err4, int *freeze, double *best, int *since,
+                                double freeze_tol, int freeze_stall, hipStream_t s) {
+    hipLaunchKernelGGL(freeze_update_kernel, dim3(1), dim3(1), 0, s, err4, freeze, best, since, freeze_tol, freeze_stall);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_list(int ninst, int ntiles, const int *inst_tile, const int *tile_inst,
+                             const int *status, int *inst_off, int *tile_list, int *tile_count,
+                             hipStream_t s) {
+    hipLaunchKernelGGL(batch_list_scan_kernel, dim3(1), dim3(1024), 0, s, ninst, inst_tile, status, inst_off, tile_count);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || ntiles <= 0) return e;
+    hipLaunchKernelGGL(batch_list_fill_kernel, dim3((ntiles + 255) / 256), dim3(256), 0, s, ntiles, tile_inst, inst_tile, status, inst_off, tile_list);
     return hipGetLastError();
 }
 

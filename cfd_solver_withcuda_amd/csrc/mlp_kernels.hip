@@ -1,0 +1,306 @@
+// Kernels of the coordinate-MLP surrogate (warm start of the FP64 pseudo-time march): BF16
+// weights and activations on gfx950 MFMA (v_mfma_f32_16x16x32_bf16) with FP32 accumulation,
+// fused bias/SiLU epilogues, split-K weight gradients with a fixed-order reduction, and Adam on
+// FP32 master weights. The reference has no counterpart; the oracle's FP64 MLP (tests) is the
+// checker.
+//
+// GEMM convention: C[r][c] = sum_k A[r][k] * B[c][k] with A [R x K] and B [C x K] both row-major
+// (K contiguous), R, C multiples of 32 and K a multiple of 32. One wave computes a 32 x 32 tile
+// of C as 2 x 2 MFMA 16x16 tiles. Operand lanes of v_mfma_f32_16x16x32_bf16: lane l holds
+// A[l % 16][8 (l / 16) .. +8] and B[l % 16][8 (l / 16) .. +8]; result lane l holds
+// C[4 (l / 16) + v][l % 16], v = 0..3.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mlp_kernels.h"
+
+namespace cfd {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// round-to-nearest-even float -> bf16 bits
+__device__ __forceinline__ uint16_t f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffff) ? 0x40 : 0));  // inf / nan
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+__device__ __forceinline__ float sigmoidf_(float z) { return 1.0f / (1.0f + __expf(-z)); }
+
+__device__ __forceinline__ bf16x8 load8(const uint16_t *p) {
+    return *reinterpret_cast<const bf16x8 *>(p);
+}
+
+// Epilogues
+enum { EPI_BIAS_SILU = MLP_EPI_BIAS_SILU, EPI_BIAS = MLP_EPI_BIAS, EPI_DSILU = MLP_EPI_DSILU, EPI_PARTIAL = MLP_EPI_PARTIAL };
+
+template <int EPI>
+__global__ void __launch_bounds__(64) gemm_nt_kernel(const uint16_t *__restrict__ A, int lda,
+                                                     const uint16_t *__restrict__ B, int ldb,
+                                                     int R, int C, int K, int ksplit,
+                                                     const float *__restrict__ bias,
+                                                     const float *__restrict__ Zprev,
+                                                     float *__restrict__ Zout,
+                                                     uint16_t *__restrict__ Y, uint16_t *__restrict__ Yt,
+                                                     int ldy, int ldyt, int c_valid) {
+    const int tilesC = C / 32;
+    const int tile = blockIdx.x;
+    const int r0 = (tile / tilesC) * 32, c0 = (tile % tilesC) * 32;
+    const int l = threadIdx.x;
+    const int lr = l % 16, lk = 8 * (l / 16);
+    // split-K: blockIdx.y selects the K range
+    const int kper = K / ksplit;
+    const int kb = blockIdx.y * kper, ke = kb + kper;
+    f32x4 acc[2][2];
+    for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const uint16_t *a0 = A + (size_t)(r0 + lr) * lda + lk;
+    const uint16_t *a1 = A + (size_t)(r0 + 16 + lr) * lda + lk;
+    const uint16_t *b0 = B + (size_t)(c0 + lr) * ldb + lk;
+    const uint16_t *b1 = B + (size_t)(c0 + 16 + lr) * ldb + lk;
+    for (int k = kb; k < ke; k += 32) {
+        const bf16x8 va0 = load8(a0 + k), va1 = load8(a1 + k);
+        const bf16x8 vb0 = load8(b0 + k), vb1 = load8(b1 + k);
+        acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va0, vb0, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va0, vb1, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va1, vb0, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va1, vb1, acc[1][1], 0, 0, 0);
+    }
+    for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++) {
+            const int c = c0 + 16 * j + lr;
+            const int rb = r0 + 16 * i + 4 * (l / 16);
+            float v[4] = {acc[i][j][0], acc[i][j][1], acc[i][j][2], acc[i][j][3]};
+            if constexpr (EPI == EPI_PARTIAL) {
+                float *P = Zout + (size_t)blockIdx.y * R * C;
+                for (int q = 0; q < 4; q++) P[(size_t)(rb + q) * C + c] = v[q];
+            } else if constexpr (EPI == EPI_BIAS) {
+                const float bb = (c < c_valid) ? bias[c] : 0.f;
+                for (int q = 0; q < 4; q++) Zout[(size_t)(rb + q) * C + c] = (c < c_valid) ? v[q] + bb : 0.f;
+            } else if constexpr (EPI == EPI_BIAS_SILU) {
+                const float bb = (c < c_valid) ? bias[c] : 0.f;
+                uint16_t ys[4];
+                for (int q = 0; q < 4; q++) {
+                    const float z = (c < c_valid) ? v[q] + bb : 0.f;
+                    Zout[(size_t)(rb + q) * C + c] = z;
+                    ys[q] = (c < c_valid) ? f2bf(z * sigmoidf_(z)) : (uint16_t)0;
+                    Y[(size_t)(rb + q) * ldy + c] = ys[q];
+                }
+                // transposed copy: 4 consecutive rows of column c
+                uint2 pk;
+                pk.x = (uint32_t)ys[0] | ((uint32_t)ys[1] << 16);
+                pk.y = (uint32_t)ys[2] | ((uint32_t)ys[3] << 16);
+                *reinterpret_cast<uint2 *>(Yt + (size_t)c * ldyt + rb) = pk;
+            } else {  // EPI_DSILU: dZprev = acc * silu'(Zprev)
+                uint16_t ys[4];
+                for (int q = 0; q < 4; q++) {
+                    float d = 0.f;
+                    if (c < c_valid) {
+                        const float z = Zprev[(size_t)(rb + q) * C + c];
+                        const float s = sigmoidf_(z);
+                        d = v[q] * (s * (1.0f + z * (1.0f - s)));
+                    }
+                    ys[q] = f2bf(d);
+                    Y[(size_t)(rb + q) * ldy + c] = ys[q];
+                }
+                uint2 pk;
+                pk.x = (uint32_t)ys[0] | ((uint32_t)ys[1] << 16);
+                pk.y = (uint32_t)ys[2] | ((uint32_t)ys[3] << 16);
+                *reinterpret_cast<uint2 *>(Yt + (size_t)c * ldyt + rb) = pk;
+            }
+        }
+}
+
+// dW[m][k] = sum_s P[s][m][k] (fixed order); db[m] = sum_n dZt[m][n] (one workgroup per m row).
+__global__ void __launch_bounds__(256) reduce_partials_kernel(const float *__restrict__ P, int S,
+                                                              int MK, float *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= MK) return;
+    float s = 0.f;
+    for (int q = 0; q < S; q++) s += P[(size_t)q * MK + i];
+    out[i] = s;
+}
+
+__global__ void __launch_bounds__(256) row_sum_kernel(const uint16_t *__restrict__ X, int ld, int n,
+                                                      float *__restrict__ out) {
+    __shared__ float red[256];
+    const int r = blockIdx.x, t = threadIdx.x;
+    float s = 0.f;
+    for (int j = t; j < n; j += 256) s += bf2f(X[(size_t)r * ld + j]);
+    red[t] = s;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (t < h) red[t] += red[t + h];
+        __syncthreads();
+    }
+    if (t == 0) out[r] = red[0];
+}
+
+// Normalised input features (n_in of them) of minibatch rows: row r takes sample idx[r] (or r
+// when idx is null), padded with zeros to K0 columns; row-major X and transposed Xt.
+__global__ void __launch_bounds__(256) encode_kernel(const float *__restrict__ feat, int n_in,
+                                                     const int *__restrict__ idx, int rows,
+                                                     const float *__restrict__ mean,
+                                                     const float *__restrict__ scale, int K0,
+                                                     uint16_t *__restrict__ X, uint16_t *__restrict__ Xt,
+                                                     int ldxt) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    const int s = idx ? idx[r] : r;
+    for (int k = 0; k < K0; k++) {
+        uint16_t h = 0;
+        if (k < n_in) h = f2bf((feat[(size_t)s * n_in + k] - mean[k]) * scale[k]);
+        X[(size_t)r * K0 + k] = h;
+        Xt[(size_t)k * ldxt + r] = h;
+    }
+}
+
+// Squared-error loss on normalised targets: dZ = 2 (Z - T) / (rows * n_out) for the first n_out
+// columns (bf16, both layouts); per-row loss into lrow.
+__global__ void __launch_bounds__(256) loss_kernel(const float *__restrict__ Z, int C,
+                                                   const float *__restrict__ targ, int n_out,
+                                                   const int *__restrict__ idx, int rows,
+                                                   const float *__restrict__ mean,
+                                                   const float *__restrict__ scale,
+                                                   uint16_t *__restrict__ dZ, uint16_t *__restrict__ dZt,
+                                                   int lddzt, float *__restrict__ lrow) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    const int s = idx ? idx[r] : r;
+    const float inv = 2.0f / (float)(rows * n_out);
+    float l = 0.f;
+    for (int c = 0; c < C; c++) {
+        float d = 0.f;
+        if (c < n_out) {
+            const float t = (targ[(size_t)s * n_out + c] - mean[c]) * scale[c];
+            const float e = Z[(size_t)r * C + c] - t;
+            l += e * e;
+            d = inv * e;
+        }
+        const uint16_t h = f2bf(d);
+        dZ[(size_t)r * C + c] = h;
+        dZt[(size_t)c * lddzt + r] = h;
+    }
+    lrow[r] = l / (float)n_out;
+}
+
+// Adam on FP32 master weights; refreshes the bf16 copies W [M][K] and Wt [K][M] (K valid columns
+// of K_pad, M valid rows of M_pad; padding stays zero).
+__global__ void __launch_bounds__(256) adam_kernel(float *__restrict__ W, float *__restrict__ m1,
+                                                   float *__restrict__ m2, const float *__restrict__ g,
+                                                   int n, float lr, float b1, float b2, float eps,
+                                                   float c1, float c2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float a = b1 * m1[i] + (1.0f - b1) * gi;
+    const float b = b2 * m2[i] + (1.0f - b2) * gi * gi;
+    m1[i] = a;
+    m2[i] = b;
+    W[i] -= lr * (a * c1) / (sqrtf(b * c2) + eps);
+}
+
+__global__ void __launch_bounds__(256) pack_weights_kernel(const float *__restrict__ W, int M, int K,
+                                                           int Mp, int Kp, uint16_t *__restrict__ Wb,
+                                                           uint16_t *__restrict__ Wt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Mp * Kp) return;
+    const int m = i / Kp, k = i % Kp;
+    const uint16_t h = (m < M && k < K) ? f2bf(W[(size_t)m * K + k]) : (uint16_t)0;
+    Wb[i] = h;
+    Wt[(size_t)k * Mp + m] = h;
+}
+
+// Gathers the valid [M x K] block of a padded [Mp x Kp] fp32 matrix.
+__global__ void __launch_bounds__(256) unpad_kernel(const float *__restrict__ P, int Kp, int M, int K,
+                                                    float *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M * K) return;
+    out[i] = P[(size_t)(i / K) * Kp + (i % K)];
+}
+
+// Minibatch indices from a counter-based hash (SplitMix64) of (seed, step, row).
+__global__ void __launch_bounds__(256) sample_kernel(unsigned long long seed, int step, int rows,
+                                                     int n, int *__restrict__ idx) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    unsigned long long z = seed + 0x9E3779B97F4A7C15ull * ((unsigned long long)step * (unsigned long long)rows + r + 1);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    idx[r] = (int)(z % (unsigned long long)n);
+}
+
+}  // namespace
+
+hipError_t mlp_gemm(int epi, const uint16_t *A, int lda, const uint16_t *B, int ldb, int R, int C,
+                    int K, int ksplit, const float *bias, const float *Zprev, float *Zout,
+                    uint16_t *Y, uint16_t *Yt, int ldy, int ldyt, int c_valid, hipStream_t s) {
+    if (R % 32 || C % 32 || K % 32 || ksplit < 1 || (K / 32) % ksplit) return hipErrorInvalidValue;
+    dim3 grid((R / 32) * (C / 32), ksplit);
+    switch (epi) {
+    case EPI_BIAS_SILU:
+        hipLaunchKernelGGL(gemm_nt_kernel<EPI_BIAS_SILU>, grid, dim3(64), 0, s, A, lda, B, ldb, R, C, K, ksplit, bias, Zprev, Zout, Y, Yt, ldy, ldyt, c_valid);
+        break;
+    case EPI_BIAS:
+        hipLaunchKernelGGL(gemm_nt_kernel<EPI_BIAS>, grid, dim3(64), 0, s, A, lda, B, ldb, R, C, K, ksplit, bias, Zprev, Zout, Y, Yt, ldy, ldyt, c_valid);
+        break;
+    case EPI_DSILU:
+        hipLaunchKernelGGL(gemm_nt_kernel<EPI_DSILU>, grid, dim3(64), 0, s, A, lda, B, ldb, R, C, K, ksplit, bias, Zprev, Zout, Y, Yt, ldy, ldyt, c_valid);
+        break;
+    default:
+        hipLaunchKernelGGL(gemm_nt_kernel<EPI_PARTIAL>, grid, dim3(64), 0, s, A, lda, B, ldb, R, C, K, ksplit, bias, Zprev, Zout, Y, Yt, ldy, ldyt, c_valid);
+        break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t mlp_reduce_partials(const float *P, int S, int MK, float *out, hipStream_t s) {
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3((MK + 255) / 256), dim3(256), 0, s, P, S, MK, out);
+    return hipGetLastError();
+}
+
+hipError_t mlp_row_sum(const uint16_t *X, int ld, int rows, int n, float *out, hipStream_t s) {
+    hipLaunchKernelGGL(row_sum_kernel, dim3(rows), dim3(256), 0, s, X, ld, n, out);
+    return hipGetLastError();
+}
+
+hipError_t mlp_encode(const float *feat, int n_in, const int *idx, int rows, const float *mean,
+                      const float *scale, int K0, uint16_t *X, uint16_t *Xt, int ldxt, hipStream_t s) {
+    hipLaunchKernelGGL(encode_kernel, dim3((rows + 255) / 256), dim3(256), 0, s, feat, n_in, idx, rows, mean, scale, K0, X, Xt, ldxt);
+    return hipGetLastError();
+}
+
+hipError_t mlp_loss(const float *Z, int C, const float *targ, int n_out, const int *idx, int rows,
+                    const float *mean, const float *scale, uint16_t *dZ, uint16_t *dZt, int lddzt,
+                    float *lrow, hipStream_t s) {
+    hipLaunchKernelGGL(loss_kernel, dim3((rows + 255) / 256), dim3(256), 0, s, Z, C, targ, n_out, idx, rows, mean, scale, dZ, dZt, lddzt, lrow);
+    return hipGetLastError();
+}
+
+hipError_t mlp_adam(float *W, float *m1, float *m2, const float *g, int n, float lr, float b1,
+                    float b2, float eps, float c1, float c2, hipStream_t s) {
+    hipLaunchKernelGGL(adam_kernel, dim3((n + 255) / 256), dim3(256), 0, s, W, m1, m2, g, n, lr, b1, b2, eps, c1, c2);
+    return hipGetLastError();
+}
+
+hipError_t mlp_pack(const float *W, int M, int K, int Mp, int Kp, uint16_t *Wb, uint16_t *Wt, hipStream_t s) {
+    hipLaunchKernelGGL(pack_weights_kernel, dim3((Mp * Kp + 255) / 256), dim3(256), 0, s, W, M, K, Mp, Kp, Wb, Wt);
+    return hipGetLastError();
+}
+
+hipError_t mlp_unpad(const float *P, int Kp, int M, int K, float *out, hipStream_t s) {
+    hipLaunchKernelGGL(unpad_kernel, dim3((M * K + 255) / 256), dim3(256), 0, s, P, Kp, M, K, out);
+    return hipGetLastError();
+}
+
+hipError_t mlp_sample(unsigned long long seed, int step, int rows, int n, int *idx, hipStream_t s) {
+    hipLaunchKernelGGL(sample_kernel, dim3((rows + 255) / 256), dim3(256), 0, s, seed, step, rows, n, idx);
+    return hipGetLastError();
+}
+
+}  // namespace cfd

@@ -29,7 +29,7 @@ REF_BCS = ("inlet_ref", "exit", "wall", "wall", "wall")
 
 def make_config(flux="roe", order=1, limiter="vanleer", gradient="gg", roe_bdiss=0, cfl=0.5,
                 bc=REF_BCS, fs=(1.0, 0.0, 0.0, 1.0), rk=3, lts=0, ref_mode=0, block=256,
-                xcd_remap=1, grad_lds=1) -> Config:
+                xcd_remap=1, grad_lds=1, freeze_tol=0.0, freeze_stall=0) -> Config:
     c = Config()
     c.flux = FLUX[flux]
     c.order = int(order)
@@ -50,6 +50,8 @@ def make_config(flux="roe", order=1, limiter="vanleer", gradient="gg", roe_bdiss
     c.block = int(block)
     c.xcd_remap = int(xcd_remap)
     c.grad_lds = int(grad_lds)
+    c.freeze_tol = float(freeze_tol)
+    c.freeze_stall = int(freeze_stall)
     return c
 
 
@@ -202,6 +204,10 @@ class Solver:
         check(lib().cfd_residual(self.h, dptr(R), dptr(dt), dptr(grad), ctypes.c_void_p(stream)),
               "cfd_residual")
         return R, dt, grad
+
+    def set_freeze(self, mode: int):
+        """Limiter freezing state: 0 off, 1 record on the next step, 2 use the recorded factors."""
+        check(lib().cfd_set_freeze(self.h, int(mode)), "cfd_set_freeze")
 
     def ghost_states(self, stream: int = 0) -> np.ndarray:
         """Conserved ghost states [nghost, 4] of the current state, ghost k = cell nphys + k."""

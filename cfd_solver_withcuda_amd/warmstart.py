@@ -1,0 +1,116 @@
+"""Learned warm start of batched steady solves on the bump family.
+
+  1. steady solutions of `n_train` bump cases (Latin hypercube in Mach x thickness) from the
+     batched FP64 solver;
+  2. a coordinate MLP (x, y, Mach, thickness) -> (rho, u, v, p) trained on their cell values
+     with the BF16-MFMA kernels;
+  3. `n_eval` held-out cases solved twice in one batch each: from freestream (plain) and from the
+     surrogate's field (warm, with per-instance fallback to freestream when the warm start's
+     first-step residual exceeds the freestream start's). Both marches run to Error[0] <= tol.
+
+The plain FP64 march stays the judge; the surrogate only moves its starting point.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from . import cases
+from .batch import Batch
+from .solver import cons_from_prim, make_config
+from .surrogate import MLP, features, predicted_state, prim_from_cons
+
+SCHEME = dict(flux="hllc", order=2, limiter="vanleer", gradient="wlsq", cfl=0.8, rk=3, lts=1,
+              roe_bdiss=1, bc=cases.BUMP_BCS)
+
+
+def freestream_states(cs):
+    return [np.tile(cons_from_prim(np.array([c.fs[0]]), c.fs[1], c.fs[2], c.fs[3]), (len(c.mesh.cells), 1))
+            for c in cs]
+
+
+def solve_batch(cs, tol, max_steps, check_every, block=128, start=None):
+    b = Batch([c.mesh for c in cs], np.array([c.fs for c in cs]), make_config(block=block, **SCHEME))
+    if start is not None:
+        b.set_state(start)
+    r = b.solve(tol=tol, max_steps=max_steps, check_every=check_every)
+    return b, r
+
+
+def training_set(cs, states):
+    feat, targ = [], []
+    for c, U in zip(cs, states):
+        feat.append(features(c.mesh.points, c.mesh.cells, c.mach, c.param))
+        targ.append(prim_from_cons(U).astype(np.float32))
+    return np.concatenate(feat), np.concatenate(targ)
+
+
+def run(n_train=64, n_eval=32, train_cells=30000, eval_cells=60000, tol=1e-8, train_tol=1e-8,
+        max_steps=60000, check_every=100, mlp_steps=3000, mlp_batch=16384, seed=5, log=print):
+    out = {}
+    t0 = time.perf_counter()
+    tr = cases.bump_family(n_train, seed=seed, cells=train_cells, stream=200)
+    b_tr, rt = solve_batch(tr, train_tol, max_steps, check_every)
+    states = b_tr.get_state()
+    ok = rt["status"] == 0
+    out["train"] = dict(cases=n_train, cells=int(sum(len(c.mesh.cells) for c in tr)), converged=int(ok.sum()),
+                        steps_max=int(rt["steps"].max()), wall_s=rt["wall_s"])
+    log(f"training solves: {out['train']}")
+    del b_tr
+    feat, targ = training_set([c for c, k in zip(tr, ok) if k], [U for U, k in zip(states, ok) if k])
+    mlp = MLP(4, 256, 6, 4, max_rows=mlp_batch, seed=seed)
+    mlp.fit_normalization(feat, targ)
+    t1 = time.perf_counter()
+    hist = mlp.train(feat, targ, steps=mlp_steps, batch=mlp_batch, lr=2e-3, lr_final=2e-5, seed=seed + 1,
+                     log_every=max(1, mlp_steps // 20))
+    t_train = time.perf_counter() - t1
+    out["mlp"] = dict(samples=int(len(feat)), steps=mlp_steps, batch=mlp_batch, train_s=t_train,
+                      loss_first=float(hist[0]), loss_last=float(hist[-1]))
+    log(f"mlp: {out['mlp']}")
+
+    ev = cases.bump_family(n_eval, seed=seed + 100, cells=eval_cells, stream=300)
+    fs_states = freestream_states(ev)
+    # plain: freestream start
+    b = Batch([c.mesh for c in ev], np.array([c.fs for c in ev]), make_config(block=128, **SCHEME))
+    b.set_state(fs_states)
+    rp = b.solve(tol=tol, max_steps=max_steps, check_every=check_every)
+    plain_states = b.get_state()
+    out["plain"] = dict(wall_s=rp["wall_s"], steps_max=int(rp["steps"].max()),
+                        steps_mean=float(rp["steps"].mean()), converged=int((rp["status"] == 0).sum()))
+    log(f"plain: {out['plain']}")
+    # warm: inference, fallback test on the first step's residual, march
+    tw = time.perf_counter()
+    warm = [predicted_state(mlp, c.mesh.points, c.mesh.cells, c.mach, c.param) for c in ev]
+    t_inf = time.perf_counter() - tw
+    b.set_state(warm)
+    e_w = b.solve(tol=-1.0, max_steps=1, check_every=1)["err"][:, 0]
+    b.set_state(fs_states)
+    e_f = b.solve(tol=-1.0, max_steps=1, check_every=1)["err"][:, 0]
+    use = np.isfinite(e_w) & (e_w <= e_f)
+    start = [w if u else f for w, f, u in zip(warm, fs_states, use)]
+    b.set_state(start)
+    t_check = time.perf_counter() - tw - t_inf
+    rw = b.solve(tol=tol, max_steps=max_steps, check_every=check_every)
+    warm_wall = time.perf_counter() - tw
+    warm_states = b.get_state()
+    out["warm"] = dict(wall_s=warm_wall, march_s=rw["wall_s"], inference_s=t_inf, fallback_check_s=t_check,
+                       steps_max=int(rw["steps"].max()), steps_mean=float(rw["steps"].mean()),
+                       converged=int((rw["status"] == 0).sum()), fallbacks=int((~use).sum()))
+    log(f"warm: {out['warm']}")
+    # surrogate error against the converged field, and agreement of the two converged fields
+    s_err, d_conv = [], []
+    for Uw0, Up, Uw in zip(warm, plain_states, warm_states):
+        Wp = prim_from_cons(Up)
+        s_err.append(np.linalg.norm(prim_from_cons(Uw0) - Wp) / np.linalg.norm(Wp))
+        d_conv.append(np.abs(Uw - Up).max() / np.abs(Up).max())
+    out["surrogate_rel_l2"] = dict(mean=float(np.mean(s_err)), max=float(np.max(s_err)))
+    out["converged_fields_max_rel_diff"] = float(np.max(d_conv))
+    out["speedup"] = out["plain"]["wall_s"] / out["warm"]["wall_s"]
+    out["total_s"] = time.perf_counter() - t0
+    log(f"surrogate rel L2 {out['surrogate_rel_l2']}, converged diff {out['converged_fields_max_rel_diff']:.3e}, "
+        f"speedup {out['speedup']:.2f}")
+    return out
+
+
+__all__ = ["run", "solve_batch", "training_set", "freestream_states", "SCHEME"]

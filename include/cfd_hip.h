@@ -53,6 +53,12 @@ typedef struct {
     int block;     /* workgroup size (64..1024, power of two; 256 default) */
     int xcd_remap; /* map consecutive tiles onto one XCD */
     int grad_lds;  /* stage the tile's gradients in LDS (1) or read them through L2 (0) */
+    /* Limiter freezing for steady marches (order 2, native mode): once Error[0] <= freeze_tol,
+     * or once Error[0] has not dropped below 0.9 x its best value for freeze_stall steps, the
+     * next step records each face's limiter factor psi = slope / (grad W . d) and later steps
+     * reuse it (slope = psi * grad W . d). 0 and 0 disable it. */
+    double freeze_tol;
+    int freeze_stall;
 } cfd_config_t;
 
 const char *cfd_last_error(void);
@@ -104,6 +110,10 @@ int cfd_state_device(cfd_solver_t *s, void **d_U);
  * gradients, order 2; may be NULL). Host outputs in the caller's order. */
 int cfd_residual(cfd_solver_t *s, double *R, double *dt, double *grad, void *stream);
 
+/* Sets the limiter-freezing state: 0 off (limiter evaluated), 1 record psi on the next step,
+ * 2 use the recorded psi. With freeze_tol = freeze_stall = 0 the state only changes here. */
+int cfd_set_freeze(cfd_solver_t *s, int mode);
+
 /* Ghost states of the current state (Apply_Boundary_Conditions, src/Boundary_Conditions.cpp:12-61):
  * U_ghost[4*nghost] (host, may be NULL to keep them on the device) in the reference's ghost
  * order, ghost k = cell id nphys + k. The residual and step entries form these states on the fly;
@@ -124,6 +134,66 @@ int cfd_run_steady(cfd_solver_t *s, double tol, int max_steps, int check_every, 
  * 1 the fused flux/stage pass (first stage, output to scratch; the state is unchanged),
  * 2 whole steps (advances the state). *ms = total elapsed milliseconds. */
 int cfd_time_pass(cfd_solver_t *s, int which, int reps, float *ms, void *stream);
+
+/* ---- batched steady solves ------------------------------------------------------------------
+ * ninst independent meshes concatenated into one device mesh and marched together with local
+ * time stepping until each instance's Error[0] <= tol. Instance i has points
+ * pts3[3*pt_off[i] .. 3*pt_off[i+1]) and cells cells4[4*cell_off[i] .. 4*cell_off[i+1]) (node ids
+ * local to the instance) and its own prescribed state fs4[4*i..4*i+3] (rho, u, v, p) for the
+ * inlet / freestream / far-field ghosts; the rest of cfg is shared (cfg->fs is ignored;
+ * cfg->lts must be 1, rk 3 or 4). cfg->block = 0 autotunes (tile, face budget, block) on the
+ * largest instances; 64/128/256 fixes block = tile and face budget = 2*block. Meshes are built
+ * on `nthreads` host threads. The state starts at each instance's freestream. */
+typedef struct cfd_batch_s cfd_batch_t;
+int cfd_batch_create(int ninst, const int *pt_off, const double *pts3, const int *cell_off,
+                     const int *cells4, const double *fs4, const cfd_config_t *cfg, int nthreads,
+                     cfd_batch_t **out);
+void cfd_batch_destroy(cfd_batch_t *b);
+/* out8 = ninst, cells, tiles, face entries, tile, face budget, block, max faces per tile;
+ * secs2 = mesh build seconds, autotune seconds (either may be NULL) */
+int cfd_batch_info(const cfd_batch_t *b, int *out8, double *secs2);
+/* U[4*total cells] conserved, instances concatenated in order, each in its caller's cell order */
+int cfd_batch_set_state(cfd_batch_t *b, const double *U);
+int cfd_batch_get_state(cfd_batch_t *b, double *U);
+/* March all instances (one hipGraph launch per step; the host reads the instance status every
+ * check_every steps). Per instance: steps[ninst] taken, err4[4*ninst] last Error[0..3],
+ * status[ninst] (0 converged, 1 still marching at max_steps, -1 non-finite), conv_s[ninst]
+ * wall seconds at the check where it was first seen converged; *wall_s total. Any output may be
+ * NULL. Returns 0 when every instance converged, CFD_E_NOTCONV otherwise. */
+int cfd_batch_solve(cfd_batch_t *b, double tol, int max_steps, int check_every, int *steps,
+                    double *err4, int *status, double *conv_s, double *wall_s, void *stream);
+/* Times nsteps graph-launched steps of all instances (advances the state). */
+int cfd_batch_time_steps(cfd_batch_t *b, int nsteps, float *ms);
+
+/* ---- coordinate-MLP surrogate (warm start) ---------------------------------------------------
+ * n_in features -> depth SiLU hidden layers of `width` -> n_out outputs; BF16 weights and
+ * activations on MFMA with FP32 accumulation; Adam on FP32 master weights. Parameters are
+ * concatenated per layer: W_l row-major [out x in], then b_l. Features and targets are float
+ * [n x n_in] / [n x n_out] host arrays, normalised inside as (v - mean) * scale (targets and
+ * predictions in physical units at this interface). max_rows bounds a minibatch (rounded up
+ * to a multiple of 512). */
+typedef struct cfd_mlp_s cfd_mlp_t;
+int cfd_mlp_create(int n_in, int width, int depth, int n_out, int max_rows, unsigned long long seed,
+                   cfd_mlp_t **out);
+void cfd_mlp_destroy(cfd_mlp_t *m);
+int cfd_mlp_num_params(const cfd_mlp_t *m, long long *n_weights, long long *n_biases);
+int cfd_mlp_set_params(cfd_mlp_t *m, const float *W, const float *b);  /* resets Adam moments */
+int cfd_mlp_get_params(cfd_mlp_t *m, float *W, float *b);
+int cfd_mlp_set_normalization(cfd_mlp_t *m, const float *in_mean, const float *in_scale,
+                              const float *out_mean, const float *out_scale);
+/* One forward + backward over all n rows (n a multiple of 512): gradients of the mean squared
+ * error of the normalised outputs, and that loss. */
+int cfd_mlp_gradients(cfd_mlp_t *m, const float *feat, const float *targ, int n, float *gW,
+                      float *gb, double *loss);
+/* `steps` Adam steps on minibatches of `batch` rows drawn (hash of seed, step, row) from the n
+ * samples; learning rate cosine-decays from lr to lr_final. loss_hist[ceil(steps/log_every)]
+ * (may be NULL) records the minibatch loss of every log_every-th step. */
+int cfd_mlp_train(cfd_mlp_t *m, const float *feat, const float *targ, int n, int steps, int batch,
+                  float lr, float lr_final, unsigned long long seed, float *loss_hist, int log_every);
+/* Predictions out[n x n_out] (physical units) for feat[n x n_in]. */
+int cfd_mlp_predict(cfd_mlp_t *m, const float *feat, int n, float *out);
+/* Times `reps` forward + backward passes of `batch` rows on the training data (HIP events). */
+int cfd_mlp_time_step(cfd_mlp_t *m, int batch, int reps, float *ms);
 
 #ifdef __cplusplus
 }

@@ -376,6 +376,8 @@ void om_free(OMesh *m) {
     free(m);
 }
 
+int om_nfaces(const OMesh *m) { return m->nfaces; }
+
 void om_counts(const OMesh *m, int *nphys, int *nghost, int *n_in, int *n_ex, int *n_wall) {
     *nphys = m->nc;
     *nghost = m->ng;
@@ -920,13 +922,27 @@ void on_gradients(const OMesh *m, const ONatCfg *cfg, const double *W5, double *
 /* ------------------------------------------------------------------------------------------
  * Face-loop residual
  * ---------------------------------------------------------------------------------------- */
+/* Limiter freezing: the limited slope equals psi * c with c = grad W . d the unlimited central
+ * slope ((d- + d+) / 2) and psi in [0, 1] for minmod and Van Leer (<= 4/3 for superbee). Mode 1
+ * records psi = slope / c (0 where c = 0); mode 2 uses slope = psi * c with the recorded psi,
+ * which makes the iteration near a converged state differentiable (limiter switching at shocks
+ * otherwise stalls steady convergence). */
 static void reconstruct_side(const ONatCfg *cfg, const FState *cellS, const FState *otherS,
-                             const double *g, double dx, double dy, double sgn, FState *out) {
+                             const double *g, double dx, double dy, double sgn, double *phi,
+                             FState *out) {
     double Wf[4];
     for (int k = 0; k < 4; k++) {
         double dplus = sgn * (otherS->W[k] - cellS->W[k]);
-        double dminus = 2.0 * (g[2 * k] * dx + g[2 * k + 1] * dy) - dplus;
-        Wf[k] = cellS->W[k] + sgn * 0.5 * or_limiter_slope(cfg->limiter, dminus, dplus);
+        double c = g[2 * k] * dx + g[2 * k + 1] * dy;
+        double slope;
+        if (cfg->freeze == 2) {
+            slope = phi[k] * c;
+        } else {
+            double dminus = 2.0 * c - dplus;
+            slope = or_limiter_slope(cfg->limiter, dminus, dplus);
+            if (cfg->freeze == 1) phi[k] = (c != 0.0) ? slope / c : 0.0;
+        }
+        Wf[k] = cellS->W[k] + sgn * 0.5 * slope;
     }
     if (!(Wf[0] > 0.0) || !(Wf[3] > 0.0)) {
         *out = *cellS;
@@ -967,13 +983,14 @@ void on_residual(const OMesh *m, const ONatCfg *cfg, const double *U, const doub
             double dx = m->ctr[3 * Rc] - m->ctr[3 * L];
             double dy = m->ctr[3 * Rc + 1] - m->ctr[3 * L + 1];
             FState fL, fR;
-            reconstruct_side(cfg, &SL, &SR, &grad[8 * L], dx, dy, 1.0, &fL);
+            double *phiL = cfg->phi ? &cfg->phi[8 * (size_t)fi] : NULL;
+            reconstruct_side(cfg, &SL, &SR, &grad[8 * L], dx, dy, 1.0, phiL, &fL);
             if (boundary) {
                 FState src = fL;
                 if (bc == OBC_EXIT && U_exit_src) src = srcL;
                 ghost_fstate(bc, &src, nx, ny, cfg->fs, &fR);
             } else {
-                reconstruct_side(cfg, &SR, &SL, &grad[8 * Rc], dx, dy, -1.0, &fR);
+                reconstruct_side(cfg, &SR, &SL, &grad[8 * Rc], dx, dy, -1.0, phiL ? phiL + 4 : NULL, &fR);
             }
             face_flux(cfg, boundary, &fL, &fR, nx, ny, dl, &Ff[4 * fi]);
         } else {

@@ -35,6 +35,7 @@ typedef struct OMesh OMesh;
 /* ---- mesh ---------------------------------------------------------------------------- */
 OMesh *om_build(int npts, const double *pts3, int ncells, const int *cells4);
 void om_free(OMesh *m);
+int om_nfaces(const OMesh *m);
 void om_counts(const OMesh *m, int *nphys, int *nghost, int *n_in, int *n_ex, int *n_wall);
 void om_export(const OMesh *m, int *nf, int *nodes4, int *nbr4, double *nrm8, double *len4,
                double *ctr3_all, double *area_all, double *inv_area_all, double *dist4, int *kind4,
@@ -63,6 +64,9 @@ typedef struct {
     double cfl;
     int bc[6];       /* bc[kind] for face kinds 1..5 (LEFT, RIGHT, TOP, BOTTOM, WALL) */
     double fs[4];    /* prescribed primitive state (rho, u, v, p) for inlet/freestream/far field */
+    int freeze;      /* limiter freezing (order 2): 0 off, 1 record phi = slope / d+ into phi,
+                        2 use the recorded phi (slope = phi * d+) */
+    double *phi;     /* [8 * faces]: per face, side L (4 variables) then side R */
 } ONatCfg;
 
 /* ---- primitive / ghost helpers --------------------------------------------------------- */

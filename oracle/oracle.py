@@ -40,7 +40,8 @@ def _i(a: np.ndarray):
 class ONatCfg(ctypes.Structure):
     _fields_ = [("flux", ctypes.c_int), ("order", ctypes.c_int), ("limiter", ctypes.c_int),
                 ("gradient", ctypes.c_int), ("roe_bdiss", ctypes.c_int), ("cfl", ctypes.c_double),
-                ("bc", ctypes.c_int * 6), ("fs", ctypes.c_double * 4)]
+                ("bc", ctypes.c_int * 6), ("fs", ctypes.c_double * 4), ("freeze", ctypes.c_int),
+                ("phi", ctypes.c_void_p)]
 
 
 def make_cfg(flux="roe", order=1, limiter="vanleer", gradient="gg", roe_bdiss=0, cfl=0.5,
@@ -125,6 +126,8 @@ class Oracle:
         L.om_export.argtypes = [ctypes.c_void_p] + [_ip, _ip, _ip, _dp, _dp, _dp, _dp, _dp, _dp, _ip, _ip, _ip, _ip]
         L.or_prim.argtypes = [_dp, _dp]
         L.on_ghost_state.argtypes = [ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double, _dp, _dp]
+        L.om_nfaces.restype = ctypes.c_int
+        L.om_nfaces.argtypes = [ctypes.c_void_p]
         L.on_ghost_cons.argtypes = [ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double, _dp, _dp]
         L.or_limiter_slope.restype = ctypes.c_double
         L.or_limiter_slope.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double]
@@ -223,6 +226,10 @@ class OracleMesh:
             self.orc.lib.om_free(self.h)
         except Exception:
             pass
+
+    @property
+    def nfaces(self) -> int:
+        return self.orc.lib.om_nfaces(self.h)
 
     def arrays(self) -> MeshArrays:
         if self._arrays is None:

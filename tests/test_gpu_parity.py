@@ -214,3 +214,26 @@ def test_freestream_preserved(bump):
     s.set_state(U)
     R, _, _ = s.residual()
     assert np.abs(R).max() < 1e-12
+
+
+@pytest.mark.parametrize("flux,lim,grad", [("hllc", "vanleer", "wlsq"), ("ausm", "minmod", "gg")])
+def test_limiter_freezing_vs_oracle(orc, bump, flux, lim, grad):
+    """Record-then-frozen limiter factors (cfd_set_freeze): states bit-exact with the oracle's
+    freezing over 3 plain, 1 recording and 4 frozen LTS RK3 steps."""
+    print(f"case freezing {flux} {lim} {grad}", flush=True)
+    om = orc.mesh(bump.points, bump.cells)
+    U0 = smooth_state(om.arrays().ctr, om.nphys)
+    kw = dict(flux=flux, order=2, limiter=lim, gradient=grad, cfl=0.7, bc=BCS_NATIVE, fs=FS)
+    s = Solver(Mesh(bump.points, bump.cells, tile=64), make_config(rk=3, lts=1, **kw))
+    s.set_state(U0)
+    U = np.ascontiguousarray(U0.copy())
+    ocfg = make_cfg(**kw)
+    phi = np.zeros(8 * om.nfaces)
+    ocfg.phi = phi.ctypes.data
+    for mode, n in ((0, 3), (1, 1), (2, 4)):
+        s.set_freeze(mode)
+        s.step(n)
+        ocfg.freeze = mode
+        for _ in range(n):
+            om.step(ocfg, U, rk=3, lts=1)
+        assert np.array_equal(s.get_state(), U), mode
