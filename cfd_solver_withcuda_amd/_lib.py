@@ -8,7 +8,8 @@ import ctypes
 import os
 
 PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG, "libcfdhip.so")
+# CFD_HIP_LIB selects a separately built (e.g. differently tuned) library
+LIB_PATH = os.environ.get("CFD_HIP_LIB") or os.path.join(PKG, "libcfdhip.so")
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)

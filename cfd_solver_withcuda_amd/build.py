@@ -1,8 +1,9 @@
 """Builds libcfdhip.so in-tree with hipcc for gfx950.
 
-    python -m cfd_solver_withcuda_amd.build
+    python -m cfd_solver_withcuda_amd.build [--out PATH] [-D NAME=VALUE ...]
 
-HIP sources are compiled with floating-point contraction off, so that every FP64 kernel
+-D sets the kernels' compile-time tuning macros (e.g. CFD_FLUX_WAVES_PER_EU, CFD_FLUX_PREFETCH_CELL);
+--out writes a separately tuned library elsewhere (load it with CFD_HIP_LIB=PATH). HIP sources are compiled with floating-point contraction off, so that every FP64 kernel
 reproduces the CPU restatement's rounding, and with packed FP32 ops disabled.
 """
 from __future__ import annotations
@@ -34,10 +35,10 @@ def _hipcc() -> str:
     return "hipcc"
 
 
-def _compile(src: str) -> str:
+def _compile(src: str, build_dir: str = BUILD, defines: tuple = ()) -> str:
     path = os.path.join(CSRC, src)
-    obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-    cmd = [_hipcc()] + COMMON
+    obj = os.path.join(build_dir, os.path.splitext(src)[0] + ".o")
+    cmd = [_hipcc()] + COMMON + [f"-D{d}" for d in defines]
     if src.endswith(".hip"):
         cmd += HIP_FLAGS
     cmd += ["-c", path, "-o", obj]
@@ -51,22 +52,29 @@ def sources() -> list[str]:
     return [s for s in HIP_SOURCES + CPP_SOURCES if os.path.exists(os.path.join(CSRC, s))]
 
 
-def build(verbose: bool = True) -> str:
-    os.makedirs(BUILD, exist_ok=True)
+def build(verbose: bool = True, out: str = LIB, defines: tuple = ()) -> str:
+    build_dir = BUILD if out == LIB else os.path.join(os.path.dirname(os.path.abspath(out)), "_build")
+    os.makedirs(build_dir, exist_ok=True)
     srcs = sources()
     with ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
-        objs = list(ex.map(_compile, srcs))
-    tmp = LIB + ".tmp"
+        objs = list(ex.map(lambda src: _compile(src, build_dir, tuple(defines)), srcs))
+    LIB_OUT = os.path.abspath(out)
+    tmp = LIB_OUT + ".tmp"
     cmd = [_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
-    os.replace(tmp, LIB)
+    os.replace(tmp, LIB_OUT)
     if verbose:
-        print(f"built {LIB} from {', '.join(srcs)}")
-    return LIB
+        print(f"built {LIB_OUT} from {', '.join(srcs)}" + (f" with {' '.join(defines)}" if defines else ""))
+    return LIB_OUT
 
 
 if __name__ == "__main__":
-    build()
-    sys.exit(0)
+    import argparse
+
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=LIB)
+    ap.add_argument("-D", dest="defines", action="append", default=[])
+    a = ap.parse_args()
+    build(out=a.out, defines=tuple(a.defines))

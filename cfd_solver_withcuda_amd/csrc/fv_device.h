@@ -226,9 +226,10 @@ __device__ __forceinline__ void hllc_flux(const FState &L, const FState &R, doub
     double un_R = nx * u_R + ny * v_R;
     double sqrt_rho_L = sqrt(rho_L);
     double sqrt_rho_R = sqrt(rho_R);
-    double a_roe = (a_L * sqrt_rho_L + a_R * sqrt_rho_R) / (sqrt_rho_L + sqrt_rho_R);
-    double u_roe = (u_L * sqrt_rho_L + u_R * sqrt_rho_R) / (sqrt_rho_L + sqrt_rho_R);
-    double v_roe = (v_L * sqrt_rho_L + v_R * sqrt_rho_R) / (sqrt_rho_L + sqrt_rho_R);
+    double inv_s = 1.0 / (sqrt_rho_L + sqrt_rho_R);
+    double a_roe = (a_L * sqrt_rho_L + a_R * sqrt_rho_R) * inv_s;
+    double u_roe = (u_L * sqrt_rho_L + u_R * sqrt_rho_R) * inv_s;
+    double v_roe = (v_L * sqrt_rho_L + v_R * sqrt_rho_R) * inv_s;
     double un_roe = nx * u_roe + ny * v_roe;
     double S_L = fmin(un_L - a_L, un_roe - a_roe);
     double S_R = fmax(un_R + a_R, un_roe + a_roe);
@@ -252,11 +253,14 @@ __device__ __forceinline__ void hllc_flux(const FState &L, const FState &R, doub
         double S_K = left ? S_L : S_R;
         double rho = K.W[0], u = K.W[1], v = K.W[2], P = K.W[3], E = K.U[3];
         double un = nx * u + ny * v;
-        double factor = rho * (S_K - un) / (S_K - S_star);
+        // star state with one reciprocal: U*_3 = (E (S_K - un) + (S* - un)(rho (S_K - un) S* + P)) / (S_K - S*)
+        double inv_k = 1.0 / (S_K - S_star);
+        double mK = rho * (S_K - un);
+        double factor = mK * inv_k;
         double Us0 = factor;
         double Us1 = factor * (u + (S_star - un) * nx);
         double Us2 = factor * (v + (S_star - un) * ny);
-        double Us3 = factor * (E / rho + (S_star - un) * (S_star + P / (rho * (S_K - un))));
+        double Us3 = (E * (S_K - un) + (S_star - un) * (mK * S_star + P)) * inv_k;
         f[0] = rho * un + S_K * (Us0 - K.U[0]);
         f[1] = (rho * u * un + P * nx) + S_K * (Us1 - K.U[1]);
         f[2] = (rho * v * un + P * ny) + S_K * (Us2 - K.U[2]);
@@ -372,7 +376,9 @@ __device__ __forceinline__ void reconstruct_side(const FState &c, const FState &
         Wf[k] = c.W[k] + sgn * 0.5 * slope;
     }
     if (!(Wf[0] > 0.0) || !(Wf[3] > 0.0)) {
-        out = c;
+        // first order on this side: the cell's primitives, conserved values formed from them
+        for (int k = 0; k < 5; k++) out.W[k] = c.W[k];
+        cons_from_prim(c.W[0], c.W[1], c.W[2], c.W[3], out.U);
         return;
     }
     out.W[0] = Wf[0];

@@ -15,6 +15,21 @@
 #include "fv_device.h"
 #include "fv_kernels.h"
 
+// Compile-time tuning of the flux pass (defaults chosen by measurement on MI355X):
+//   CFD_FLUX_WAVES_PER_EU   waves per SIMD the register allocation must allow (0: compiler's choice)
+//   CFD_FLUX_PREFETCH_CELL  1: load the gather phase's cell inputs at kernel entry, 0: in the gather
+#ifndef CFD_FLUX_WAVES_PER_EU
+#define CFD_FLUX_WAVES_PER_EU 0
+#endif
+#ifndef CFD_FLUX_PREFETCH_CELL
+#define CFD_FLUX_PREFETCH_CELL 1
+#endif
+#if CFD_FLUX_WAVES_PER_EU > 0
+#define CFD_FLUX_ATTR __attribute__((amdgpu_waves_per_eu(CFD_FLUX_WAVES_PER_EU)))
+#else
+#define CFD_FLUX_ATTR
+#endif
+
 namespace cfd {
 
 namespace {
@@ -36,14 +51,17 @@ __device__ __forceinline__ int block_tile(const DevMesh &m, int remap) {
     return m.tile_list ? m.tile_list[b] : b;
 }
 
-// A batched mesh carries one prescribed (freestream / inlet) state per instance.
-__device__ __forceinline__ void tile_cfg(const DevMesh &m, int tile, DevCfg &c) {
+// Prescribed (freestream / inlet) state of a tile: the configuration's, or on a batched mesh the
+// tile's instance's. Kept in a local array (the kernel parameters stay read-only).
+__device__ __forceinline__ void tile_fs(const DevMesh &m, const DevCfg &cfg, int tile, double *fsv) {
     if (m.inst_fs) {
         const double4 f = m.inst_fs[m.tile_inst[tile]];
-        c.fs[0] = f.x;
-        c.fs[1] = f.y;
-        c.fs[2] = f.z;
-        c.fs[3] = f.w;
+        fsv[0] = f.x;
+        fsv[1] = f.y;
+        fsv[2] = f.z;
+        fsv[3] = f.w;
+    } else {
+        for (int q = 0; q < 4; q++) fsv[q] = cfg.fs[q];
     }
 }
 
@@ -120,7 +138,8 @@ template <int GRAD>
 __device__ __forceinline__ void grad_face(const DevMesh &m, const DevCfg &cfg,
                                           const double4 *__restrict__ Us, const double *sW,
                                           double *sX, int T, int FT, int c0, int nct, int e,
-                                          int j, const FaceRec &f, double2 wl, double2 wr) {
+                                          int j, const FaceRec &f, double2 wl, double2 wr,
+                                          const double *fsv) {
     const int lL = f.L - c0;
     double WL[5], WR[4];
     if ((unsigned)lL < (unsigned)nct) {
@@ -137,7 +156,7 @@ __device__ __forceinline__ void grad_face(const DevMesh &m, const DevCfg &cfg,
             FState in, g;
             for (int q = 0; q < 5; q++) in.W[q] = WL[q];
             for (int q = 0; q < 4; q++) in.U[q] = 0.0;
-            ghost_state(cfg.bc[-f.R], in, m.fnx[e], m.fny[e], cfg.fs, g);
+            ghost_state(cfg.bc[-f.R], in, m.fnx[e], m.fny[e], fsv, g);
             for (int q = 0; q < 4; q++) WR[q] = g.W[q];
         }
     } else {
@@ -172,7 +191,8 @@ __global__ void __launch_bounds__(256) grad_tile_kernel(DevMesh m, DevCfg cfg,
     extern __shared__ double smem[];
     const int tile = block_tile(m, cfg.xcd_remap);
     if (tile < 0) return;
-    tile_cfg(m, tile, cfg);
+    double fsv[4];
+    tile_fs(m, cfg, tile, fsv);
     const int T = m.tile;
     const int FT = m.max_tile_faces;
     const int B = blockDim.x;
@@ -220,8 +240,8 @@ __global__ void __launch_bounds__(256) grad_tile_kernel(DevMesh m, DevCfg cfg,
     }
     __syncthreads();
     // ---- B ------------------------------------------------------------------------------
-    if (h0) grad_face<GRAD>(m, cfg, Us, sW, sX, T, FT, c0, nct, fs + k, k, f0, w0l, w0r);
-    if (h1) grad_face<GRAD>(m, cfg, Us, sW, sX, T, FT, c0, nct, fs + k + B, k + B, f1, w1l, w1r);
+    if (h0) grad_face<GRAD>(m, cfg, Us, sW, sX, T, FT, c0, nct, fs + k, k, f0, w0l, w0r, fsv);
+    if (h1) grad_face<GRAD>(m, cfg, Us, sW, sX, T, FT, c0, nct, fs + k + B, k + B, f1, w1l, w1r, fsv);
     for (int j = k + 2 * B; j < nft; j += B) {
         FaceRec f;
         load_face<GG, false>(m, fs + j, f);
@@ -230,7 +250,7 @@ __global__ void __launch_bounds__(256) grad_tile_kernel(DevMesh m, DevCfg cfg,
             wl = m.fwL[fs + j];
             if (f.R >= 0) wr = m.fwR[fs + j];
         }
-        grad_face<GRAD>(m, cfg, Us, sW, sX, T, FT, c0, nct, fs + j, j, f, wl, wr);
+        grad_face<GRAD>(m, cfg, Us, sW, sX, T, FT, c0, nct, fs + j, j, f, wl, wr, fsv);
     }
     __syncthreads();
     // ---- C ------------------------------------------------------------------------------
@@ -287,14 +307,18 @@ __device__ __forceinline__ void flux_face(const DevMesh &m, const DevCfg &cfg,
                                           const double *__restrict__ grad, const double4 *Uexit,
                                           const double *sW, const double *sU, const double *sG,
                                           double *sF, bool glds, int T, int FT, int c0, int nct,
-                                          int j, const FaceRec &f, int fz, double *psi, int e) {
+                                          int j, const FaceRec &f, int fz, double *psi, int e,
+                                          const double *fsv) {
     const double nx = f.nx, ny = f.ny, dl = f.dl;
     const int lL = f.L - c0;
     const bool locL = (unsigned)lL < (unsigned)nct;
     FState SL, SR;
+    // Tile cells come from LDS: primitives always, conserved values at first order only (the
+    // second-order face states form their conserved values from the reconstructed primitives).
     if (locL) {
         for (int q = 0; q < 5; q++) SL.W[q] = sW[q * T + lL];
-        for (int q = 0; q < 4; q++) SL.U[q] = sU[q * T + lL];
+        if constexpr (ORDER == 1)
+            for (int q = 0; q < 4; q++) SL.U[q] = sU[q * T + lL];
     } else {
         load_state(Us, f.L, SL);
     }
@@ -304,26 +328,26 @@ __device__ __forceinline__ void flux_face(const DevMesh &m, const DevCfg &cfg,
     int lR = 0;
     if (boundary) {
         bc = cfg.bc[-f.R];
+        // ghost of the cell-centre state (at second order only its primitives are used: the
+        // L side's reconstruction differences toward it)
         if (bc == BC_EXIT && Uexit) {
             FState src;
             load_state(Uexit, f.L, src);
-            ghost_state(bc, src, nx, ny, cfg.fs, SR);
+            ghost_state(bc, src, nx, ny, fsv, SR);
         } else {
-            ghost_state(bc, SL, nx, ny, cfg.fs, SR);
+            ghost_state(bc, SL, nx, ny, fsv, SR);
         }
     } else {
         lR = f.R - c0;
         locR = (unsigned)lR < (unsigned)nct;
         if (locR) {
             for (int q = 0; q < 5; q++) SR.W[q] = sW[q * T + lR];
-            for (int q = 0; q < 4; q++) SR.U[q] = sU[q * T + lR];
+            if constexpr (ORDER == 1)
+                for (int q = 0; q < 4; q++) SR.U[q] = sU[q * T + lR];
         } else {
             load_state(Us, f.R, SR);
         }
     }
-    // spectral terms (|u.n| + a) dl of the two cell-centre states for the local time step
-    const double lamL = (fabs(SL.W[1] * nx + SL.W[2] * ny) + SL.W[4]) * dl;
-    const double lamR = boundary ? 0.0 : (fabs(SR.W[1] * nx + SR.W[2] * ny) + SR.W[4]) * dl;
     double F[4];
     if constexpr (ORDER == 2) {
         double gL[8];
@@ -339,9 +363,9 @@ __device__ __forceinline__ void flux_face(const DevMesh &m, const DevCfg &cfg,
             if (bc == BC_EXIT && Uexit) {
                 FState src;
                 load_state(Uexit, f.L, src);
-                ghost_state(bc, src, nx, ny, cfg.fs, fR);
+                ghost_state(bc, src, nx, ny, fsv, fR);
             } else {
-                ghost_state(bc, fL, nx, ny, cfg.fs, fR);
+                ghost_state(bc, fL, nx, ny, fsv, fR);
             }
         } else {
             double gR[8];
@@ -357,19 +381,18 @@ __device__ __forceinline__ void flux_face(const DevMesh &m, const DevCfg &cfg,
         face_flux<FLUX>(boundary, cfg.roe_bdiss, SL, SR, nx, ny, dl, F);
     }
     for (int q = 0; q < 4; q++) sF[q * FT + j] = F[q];
-    sF[4 * FT + j] = lamL;
-    sF[5 * FT + j] = lamR;
 }
 
 template <int FLUX, int ORDER, int LIM, int MODE>
-__global__ void __launch_bounds__(256) flux_tile_kernel(DevMesh m, DevCfg cfg,
+__global__ void __launch_bounds__(256) CFD_FLUX_ATTR flux_tile_kernel(DevMesh m, DevCfg cfg,
                                                         const double4 *__restrict__ Us,
                                                         const double *__restrict__ grad,
                                                         StageArgs st) {
     extern __shared__ double smem[];
     const int tile = block_tile(m, cfg.xcd_remap);
     if (tile < 0) return;
-    tile_cfg(m, tile, cfg);
+    double fsv[4];
+    tile_fs(m, cfg, tile, fsv);
     const int T = m.tile;
     const int FT = m.max_tile_faces;
     const int B = blockDim.x;
@@ -378,10 +401,10 @@ __global__ void __launch_bounds__(256) flux_tile_kernel(DevMesh m, DevCfg cfg,
     const int fs = m.tile_fstart[tile], nft = m.tile_fstart[tile + 1] - fs;
     const bool glds = (ORDER == 2) && cfg.grad_lds;
     const int fz = (ORDER == 2 && st.freeze) ? st.freeze[m.tile_inst ? m.tile_inst[tile] : 0] : 0;
-    double *sW = smem;            // [5][T]
-    double *sU = sW + 5 * T;      // [4][T]
-    double *sF = sU + 4 * T;      // [6][FT]: flux (4), spectral terms of L and R
-    double *sG = sF + 6 * FT;     // [8][T] (second order, LDS-staged gradients)
+    double *sW = smem;                          // [5][T] primitives (rho, u, v, p, a)
+    double *sU = sW + 5 * T;                    // [4][T] conserved (first order only)
+    double *sF = sU + (ORDER == 1 ? 4 * T : 0); // [4][FT] face fluxes
+    double *sG = sF + 4 * FT;                   // [8][T] (second order, LDS-staged gradients)
     const int k = threadIdx.x;
     const bool hasCell = k < nct;
     const int c = c0 + k;
@@ -390,8 +413,8 @@ __global__ void __launch_bounds__(256) flux_tile_kernel(DevMesh m, DevCfg cfg,
     double4 u4 = make_double4(1.0, 0.0, 0.0, 1.0), un4 = make_double4(0, 0, 0, 0);
     ushort4 sl4 = make_ushort4(0xFFFF, 0xFFFF, 0xFFFF, 0xFFFF);
     double ar = 0.0, ia = 0.0, hprev = 0.0;
-    if (hasCell) {
-        u4 = Us[c];
+    // the gather phase's cell inputs: loaded here (in flight during phases A and B) or there
+    auto load_cell_inputs = [&]() {
         sl4 = reinterpret_cast<const ushort4 *>(m.slots)[c];
         ar = m.area[c];
         if constexpr (MODE == MODE_STAGE) {
@@ -399,6 +422,10 @@ __global__ void __launch_bounds__(256) flux_tile_kernel(DevMesh m, DevCfg cfg,
             ia = m.inv_area[c];
             if (st.lts && st.stage > 0) hprev = st.hbuf[c];
         }
+    };
+    if (hasCell) {
+        u4 = Us[c];
+        if (CFD_FLUX_PREFETCH_CELL) load_cell_inputs();
     }
     FaceRec f0{}, f1{};
     const bool h0 = k < nft, h1 = k + B < nft;
@@ -410,7 +437,8 @@ __global__ void __launch_bounds__(256) flux_tile_kernel(DevMesh m, DevCfg cfg,
         FState s;
         state_from_U(u4.x, u4.y, u4.z, u4.w, s);
         for (int q = 0; q < 5; q++) sW[q * T + k] = s.W[q];
-        for (int q = 0; q < 4; q++) sU[q * T + k] = s.U[q];
+        if constexpr (ORDER == 1)
+            for (int q = 0; q < 4; q++) sU[q * T + k] = s.U[q];
         if (glds) {
             double g[8];
             load_grad(grad, c, g);
@@ -420,12 +448,12 @@ __global__ void __launch_bounds__(256) flux_tile_kernel(DevMesh m, DevCfg cfg,
     __syncthreads();
 
     // ---- B: one flux per face ----------------------------------------------------------
-    if (h0) flux_face<FLUX, ORDER, LIM>(m, cfg, Us, grad, st.Uexit, sW, sU, sG, sF, glds, T, FT, c0, nct, k, f0, fz, st.psi, fs + k);
-    if (h1) flux_face<FLUX, ORDER, LIM>(m, cfg, Us, grad, st.Uexit, sW, sU, sG, sF, glds, T, FT, c0, nct, k + B, f1, fz, st.psi, fs + k + B);
+    if (h0) flux_face<FLUX, ORDER, LIM>(m, cfg, Us, grad, st.Uexit, sW, sU, sG, sF, glds, T, FT, c0, nct, k, f0, fz, st.psi, fs + k, fsv);
+    if (h1) flux_face<FLUX, ORDER, LIM>(m, cfg, Us, grad, st.Uexit, sW, sU, sG, sF, glds, T, FT, c0, nct, k + B, f1, fz, st.psi, fs + k + B, fsv);
     for (int j = k + 2 * B; j < nft; j += B) {
         FaceRec f;
         load_face<true, ORDER == 2>(m, fs + j, f);
-        flux_face<FLUX, ORDER, LIM>(m, cfg, Us, grad, st.Uexit, sW, sU, sG, sF, glds, T, FT, c0, nct, j, f, fz, st.psi, fs + j);
+        flux_face<FLUX, ORDER, LIM>(m, cfg, Us, grad, st.Uexit, sW, sU, sG, sF, glds, T, FT, c0, nct, j, f, fz, st.psi, fs + j, fsv);
     }
     __syncthreads();
 
@@ -433,8 +461,8 @@ __global__ void __launch_bounds__(256) flux_tile_kernel(DevMesh m, DevCfg cfg,
     double errp[4] = {0.0, 0.0, 0.0, 0.0};
     double dtmin = 1.7976931348623157e308;
     if (hasCell) {
+        if (!CFD_FLUX_PREFETCH_CELL) load_cell_inputs();
         double r[4] = {0.0, 0.0, 0.0, 0.0};
-        double denom = 0.0;
         const uint16_t sls[4] = {sl4.x, sl4.y, sl4.z, sl4.w};
         for (int f = 0; f < 4; f++) {
             const uint16_t sl = sls[f];
@@ -445,7 +473,18 @@ __global__ void __launch_bounds__(256) flux_tile_kernel(DevMesh m, DevCfg cfg,
                 double v = sF[q * FT + j];
                 r[q] += isR ? -v : v;
             }
-            denom += isR ? sF[5 * FT + j] : sF[4 * FT + j];
+        }
+        // spectral radius sum_f (|u.n_f| + a) dl_f of the cell's own state, in its face order,
+        // only where a time step is formed (residual mode, first stage of local time stepping)
+        double denom = 0.0;
+        if (MODE == MODE_RESID || (st.lts && st.stage == 0)) {
+            const double w1 = sW[1 * T + k], w2 = sW[2 * T + k], w4 = sW[4 * T + k];
+            for (int f = 0; f < 4; f++) {
+                const uint16_t sl = sls[f];
+                if (sl == 0xFFFF) break;
+                const int e = fs + (sl >> 1);
+                denom += (fabs(w1 * m.fnx[e] + w2 * m.fny[e]) + w4) * m.fdl[e];
+            }
         }
         // Inviscid_Time_Step (src/Time_Step.cpp:293-329)
         const double dtc = (denom <= 0.0 || !isfinite(denom)) ? 0.0 : cfg.cfl * ar / denom;
@@ -792,7 +831,7 @@ size_t grad_smem_bytes(const DevMesh &m) {
 }
 
 size_t flux_smem_bytes(const DevMesh &m, int order, int grad_lds) {
-    size_t d = 5 * (size_t)m.tile + 4 * (size_t)m.tile + 6 * (size_t)m.max_tile_faces;
+    size_t d = 5 * (size_t)m.tile + (order == 1 ? 4 * (size_t)m.tile : 0) + 4 * (size_t)m.max_tile_faces;
     if (order == 2 && grad_lds) d += 8 * (size_t)m.tile;
     // the block reductions of phase C need 4 doubles per thread (blockDim <= 256)
     if (d < 4 * 256) d = 4 * 256;

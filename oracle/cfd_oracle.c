@@ -701,9 +701,11 @@ static void hllc_flux(const FState *L, const FState *R, double nx, double ny, do
     double un_R = nx * u_R + ny * v_R;
     double sqrt_rho_L = sqrt(rho_L);
     double sqrt_rho_R = sqrt(rho_R);
-    double a_roe = (a_L * sqrt_rho_L + a_R * sqrt_rho_R) / (sqrt_rho_L + sqrt_rho_R);
-    double u_roe = (u_L * sqrt_rho_L + u_R * sqrt_rho_R) / (sqrt_rho_L + sqrt_rho_R);
-    double v_roe = (v_L * sqrt_rho_L + v_R * sqrt_rho_R) / (sqrt_rho_L + sqrt_rho_R);
+    /* one reciprocal for the three sqrt(rho)-weighted averages */
+    double inv_s = 1.0 / (sqrt_rho_L + sqrt_rho_R);
+    double a_roe = (a_L * sqrt_rho_L + a_R * sqrt_rho_R) * inv_s;
+    double u_roe = (u_L * sqrt_rho_L + u_R * sqrt_rho_R) * inv_s;
+    double v_roe = (v_L * sqrt_rho_L + v_R * sqrt_rho_R) * inv_s;
     double un_roe = nx * u_roe + ny * v_roe;
     double S_L = fmin(un_L - a_L, un_roe - a_roe);
     double S_R = fmax(un_R + a_R, un_roe + a_roe);
@@ -731,11 +733,15 @@ static void hllc_flux(const FState *L, const FState *R, double nx, double ny, do
         FK[1] = rho * u * un + P * nx;
         FK[2] = rho * v * un + P * ny;
         FK[3] = (E + P) * un;
-        double factor = rho * (S_K - un) / (S_K - S_star);
+        /* star state (Toro eq. 10.39) with one reciprocal of S_K - S*:
+         * U*_3 = (E (S_K - un) + (S* - un)(rho (S_K - un) S* + P)) / (S_K - S*) */
+        double inv_k = 1.0 / (S_K - S_star);
+        double mK = rho * (S_K - un);
+        double factor = mK * inv_k;
         Us[0] = factor;
         Us[1] = factor * (u + (S_star - un) * nx);
         Us[2] = factor * (v + (S_star - un) * ny);
-        Us[3] = factor * (E / rho + (S_star - un) * (S_star + P / (rho * (S_K - un))));
+        Us[3] = (E * (S_K - un) + (S_star - un) * (mK * S_star + P)) * inv_k;
         for (int i = 0; i < 4; i++) f[i] = FK[i] + S_K * (Us[i] - K->U[i]);
     }
     for (int i = 0; i < 4; i++) F[i] = f[i] * dl;
@@ -945,7 +951,9 @@ static void reconstruct_side(const ONatCfg *cfg, const FState *cellS, const FSta
         Wf[k] = cellS->W[k] + sgn * 0.5 * slope;
     }
     if (!(Wf[0] > 0.0) || !(Wf[3] > 0.0)) {
-        *out = *cellS;
+        /* first order on this side: the cell's primitives, conserved values formed from them */
+        for (int k = 0; k < 5; k++) out->W[k] = cellS->W[k];
+        cons_from_prim(cellS->W[0], cellS->W[1], cellS->W[2], cellS->W[3], out->U);
         return;
     }
     out->W[0] = Wf[0];
