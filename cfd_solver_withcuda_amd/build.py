@@ -70,6 +70,36 @@ def build(verbose: bool = True, out: str = LIB, defines: tuple = ()) -> str:
     return LIB_OUT
 
 
+HOST_LIB = os.path.join(PKG, "libcfdhost.so")
+HOST_TESTS = os.path.join(ROOT, "tests", "host")
+
+
+def build_host(verbose: bool = True) -> str:
+    """libcfdhost.so: the reference-named C++ host driver (include/cfd_host.hpp) over
+    libcfdhip.so, and the C++ test drivers under tests/host (built into tests/host/_build)."""
+    cmd = [_hipcc()] + COMMON + ["-shared", "-o", HOST_LIB, os.path.join(CSRC, "host_driver.cpp"),
+                                 f"-L{PKG}", "-lcfdhip", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    out_dir = os.path.join(HOST_TESTS, "_build")
+    os.makedirs(out_dir, exist_ok=True)
+    built = [HOST_LIB]
+    for src in sorted(os.listdir(HOST_TESTS)):
+        if not src.endswith(".cpp"):
+            continue
+        exe = os.path.join(out_dir, os.path.splitext(src)[0])
+        cmd = [_hipcc()] + COMMON + ["-o", exe, os.path.join(HOST_TESTS, src), f"-L{PKG}", "-lcfdhost",
+                                     "-lcfdhip", "-Wl,-rpath,$ORIGIN/../../../cfd_solver_withcuda_amd"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        built.append(exe)
+    if verbose:
+        print("built " + ", ".join(built))
+    return HOST_LIB
+
+
 if __name__ == "__main__":
     import argparse
 
@@ -78,3 +108,5 @@ if __name__ == "__main__":
     ap.add_argument("-D", dest="defines", action="append", default=[])
     a = ap.parse_args()
     build(out=a.out, defines=tuple(a.defines))
+    if a.out == LIB:
+        build_host()

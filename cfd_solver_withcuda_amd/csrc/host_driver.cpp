@@ -1,0 +1,221 @@
+// Reference-named C++ host driver (include/cfd_host.hpp) over the C-ABI of libcfdhip.so.
+#include "../../include/cfd_host.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <fstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/cfd_hip.h"
+
+std::vector<V_D> U_Cells, Cells_Net_Flux, Cells_DelU;
+V_D Error(4, 0.0), Cells_del_t;
+double Min_dt = 0.0;
+int No_Physical_Cells = 0, Total_No_Cells = 0;
+double CFL = 0.5;
+int Dissipation_Type = 3;
+bool Is_Second_Order = false, Local_Time_Stepping = false, Is_Inlet_SubSonic = false, Is_Exit_SubSonic = false;
+int Limiter_Type = 2, Gradient_Type = 1;
+InletCondition inletCond;
+
+namespace {
+
+cfd_mesh_t *g_mesh = nullptr;
+cfd_solver_t *g_solver = nullptr;
+std::vector<double> g_dev;   // physical state the device holds (flattened [N][4])
+std::vector<double> g_next;  // state produced by the last iteration, not yet moved by Update()
+bool g_pending = false, g_have_err = false;
+double g_err[4] = {0, 0, 0, 0};
+
+void check(int rc, const char *where) {
+    if (rc) throw std::runtime_error(std::string(where) + ": " + cfd_last_error());
+}
+
+void need_mesh(const char *where) {
+    if (!g_solver) throw std::runtime_error(std::string(where) + ": no mesh (call Read_VTK_Grid or Form_Cells)");
+}
+
+cfd_config_t config(int order, int rk, int ref_mode) {
+    cfd_config_t c;
+    std::memset(&c, 0, sizeof(c));
+    switch (Dissipation_Type) {
+    case 1: c.flux = 2; break;    // LLF
+    case 3: c.flux = 0; break;    // ROE (Harten entropy fix, no dissipation on ghost faces)
+    case 101: c.flux = 1; break;  // HLLC
+    case 102: c.flux = 3; break;  // AUSM
+    default: throw std::runtime_error("Dissipation_Type must be 1 (LLF), 3 (ROE), 101 (HLLC) or 102 (AUSM)");
+    }
+    c.order = order;
+    c.limiter = (order == 2) ? Limiter_Type : 0;
+    c.gradient = Gradient_Type;
+    c.roe_bdiss = 0;
+    c.cfl = CFL;
+    // face kinds LEFT -> inlet, RIGHT -> exit, TOP / BOTTOM / WALL -> slip wall (src/Read_Gmsh_File.cpp:579-621).
+    // The reference's subsonic exit leaves the ghost untouched (src/Exit_Boundary_Conditions.cpp:83-85);
+    // here both exits copy the interior state.
+    c.bc[1] = Is_Inlet_SubSonic ? 7 : 1;
+    c.bc[2] = 2;
+    c.bc[3] = c.bc[4] = c.bc[5] = 3;
+    c.fs[0] = inletCond.Rho;
+    c.fs[1] = inletCond.u;
+    c.fs[2] = inletCond.v;
+    c.fs[3] = inletCond.P;
+    c.rk = rk;
+    c.lts = Local_Time_Stepping ? 1 : 0;
+    c.ref_mode = ref_mode;
+    c.block = 128;
+    c.xcd_remap = 1;
+    c.grad_lds = 0;
+    return c;
+}
+
+void configure(int order, int rk, int ref_mode) {
+    cfd_config_t c = config(order, rk, ref_mode);
+    check(cfd_solver_configure(g_solver, &c), "configure");
+}
+
+// Uploads U_Cells' physical rows when they differ from what the device holds.
+void sync_state() {
+    const int n = No_Physical_Cells;
+    std::vector<double> h(4 * (size_t)n);
+    for (int i = 0; i < n; i++) {
+        if (U_Cells[i].size() < 4) throw std::runtime_error("U_Cells rows need 4 conserved variables");
+        std::memcpy(&h[4 * (size_t)i], U_Cells[i].data(), 4 * sizeof(double));
+    }
+    if (h != g_dev) {
+        check(cfd_set_state(g_solver, h.data()), "set state");
+        g_dev.swap(h);
+    }
+}
+
+void one_iteration(int rk, const char *where) {
+    need_mesh(where);
+    configure(Is_Second_Order ? 2 : 1, rk, 1);
+    sync_state();
+    const std::vector<double> before = g_dev;
+    double dt = 0.0;
+    check(cfd_step(g_solver, 1, g_err, &dt, nullptr), where);
+    Min_dt = dt;
+    g_next.resize(before.size());
+    check(cfd_get_state(g_solver, g_next.data()), where);
+    g_dev = g_next;
+    for (int i = 0; i < No_Physical_Cells; i++)
+        for (int k = 0; k < 4; k++) Cells_DelU[i][k] = g_next[4 * (size_t)i + k] - before[4 * (size_t)i + k];
+    g_pending = true;
+    g_have_err = true;
+}
+
+void residual(int order, const char *where) {
+    need_mesh(where);
+    configure(order, 3, 0);
+    sync_state();
+    const int n = No_Physical_Cells;
+    std::vector<double> R(4 * (size_t)n), dt(n);
+    check(cfd_residual(g_solver, R.data(), dt.data(), nullptr, nullptr), where);
+    for (int i = 0; i < n; i++) {
+        for (int k = 0; k < 4; k++) Cells_Net_Flux[i][k] = R[4 * (size_t)i + k];
+        Cells_del_t[i] = dt[i];
+    }
+}
+
+}  // namespace
+
+bool Form_Cells(int npts, const double *pts3, int ncells, const int *cells4) {
+    Release_Solver();
+    if (cfd_mesh_create(npts, pts3, ncells, cells4, 128, 256, 1, &g_mesh)) return false;
+    cfd_config_t c = config(1, 3, 1);
+    if (cfd_solver_create(g_mesh, &c, &g_solver)) {
+        cfd_mesh_destroy(g_mesh);
+        g_mesh = nullptr;
+        return false;
+    }
+    int cnt[9];
+    check(cfd_mesh_counts(g_mesh, cnt), "Form_Cells");
+    No_Physical_Cells = cnt[0];
+    Total_No_Cells = cnt[0] + cnt[1];
+    U_Cells.assign(Total_No_Cells, V_D(4, 0.0));
+    Cells_Net_Flux.assign(No_Physical_Cells, V_D(4, 0.0));
+    Cells_DelU.assign(No_Physical_Cells, V_D(4, 0.0));
+    Cells_del_t.assign(No_Physical_Cells, 0.0);
+    Error.assign(4, 0.0);
+    g_dev.clear();
+    g_pending = g_have_err = false;
+    return true;
+}
+
+bool Read_VTK_Grid(const std::string &file) {
+    std::ifstream in(file);
+    if (!in) return false;
+    std::string tok;
+    std::vector<double> pts;
+    std::vector<std::vector<int>> conn;
+    std::vector<int> types;
+    while (in >> tok) {
+        if (tok == "POINTS") {
+            size_t n;
+            in >> n >> tok;
+            pts.resize(3 * n);
+            for (size_t i = 0; i < 3 * n; i++) in >> pts[i];
+        } else if (tok == "CELLS") {
+            size_t n, total;
+            in >> n >> total;
+            conn.resize(n);
+            for (size_t i = 0; i < n; i++) {
+                int k;
+                in >> k;
+                conn[i].resize(k);
+                for (int j = 0; j < k; j++) in >> conn[i][j];
+            }
+        } else if (tok == "CELL_TYPES") {
+            size_t n;
+            in >> n;
+            types.resize(n);
+            for (size_t i = 0; i < n; i++) in >> types[i];
+        }
+    }
+    if (pts.empty() || conn.empty() || types.size() != conn.size()) return false;
+    std::vector<int> cells;
+    for (size_t i = 0; i < conn.size(); i++) {
+        if ((types[i] == 5 && conn[i].size() == 3) || (types[i] == 9 && conn[i].size() == 4)) {
+            for (int j = 0; j < 4; j++) cells.push_back(j < (int)conn[i].size() ? conn[i][j] : -1);
+        }
+    }
+    return Form_Cells((int)(pts.size() / 3), pts.data(), (int)(cells.size() / 4), cells.data());
+}
+
+void Apply_Boundary_Conditions() {
+    need_mesh("Apply_Boundary_Conditions");
+    configure(Is_Second_Order ? 2 : 1, 3, 0);
+    sync_state();
+    const int ng = Total_No_Cells - No_Physical_Cells;
+    std::vector<double> g(4 * (size_t)ng);
+    check(cfd_apply_bc(g_solver, g.data(), nullptr), "Apply_Boundary_Conditions");
+    for (int k = 0; k < ng; k++)
+        for (int q = 0; q < 4; q++) U_Cells[No_Physical_Cells + k][q] = g[4 * (size_t)k + q];
+}
+
+void Evaluate_Cell_Net_Flux_1O() { residual(1, "Evaluate_Cell_Net_Flux_1O"); }
+void Evaluate_Cell_Net_Flux_2O() { residual(2, "Evaluate_Cell_Net_Flux_2O"); }
+void Explicit_Method() { one_iteration(1, "Explicit_Method"); }
+void Runge_Kutta_Method() { one_iteration(3, "Runge_Kutta_Method"); }
+
+void Estimate_Error() {
+    if (!g_have_err) throw std::runtime_error("Estimate_Error: no iteration has run");
+    Error.assign(g_err, g_err + 4);
+}
+
+void Update() {
+    if (!g_pending) return;
+    for (int i = 0; i < No_Physical_Cells; i++)
+        for (int k = 0; k < 4; k++) U_Cells[i][k] = g_next[4 * (size_t)i + k];
+    g_pending = false;
+}
+
+void Release_Solver() {
+    if (g_solver) cfd_solver_destroy(g_solver);
+    if (g_mesh) cfd_mesh_destroy(g_mesh);
+    g_solver = nullptr;
+    g_mesh = nullptr;
+}

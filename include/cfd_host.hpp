@@ -1,0 +1,67 @@
+// C++ host driver with the reference's driver names and globals (rameshkolluru43/CFD_Solver_withCUDA),
+// as thin adapters over libcfdhip.so's device-resident path. Link with -lcfdhost -lcfdhip.
+//
+//   Read_VTK_Grid(file) + Form_Cells()  src/Read_Gmsh_File.cpp:11-144, src/Grid_Computations.cpp:319-392
+//   Apply_Boundary_Conditions()         src/Boundary_Conditions.cpp:12-61     (include/Boundary_Conditions.h:11)
+//   Evaluate_Cell_Net_Flux_1O/2O()      src/Net_Flux.cpp:46-128               (include/Flux.h:11-12)
+//   Explicit_Method()                   src/Numerical_Method.cpp:26-103       (include/Solver.h:11)
+//   Runge_Kutta_Method()                src/Numerical_Method.cpp:342-457      (include/Solver.h:12)
+//   Estimate_Error(), Update()          src/Error_Estimate_Update.cpp:10-44   (include/Error_Update.h:10-11)
+//
+// Globals keep the reference's names (include/Globals.h:332-399). The state lives on the GPU;
+// the host vectors are the interface, as in the reference:
+//   - a call that computes reads U_Cells' physical rows (uploading them when they changed since
+//     the device last produced them) and writes its outputs back into the globals;
+//   - Explicit_Method / Runge_Kutta_Method run one whole reference iteration on the device
+//     (ghost states, stages, error norm, update); Estimate_Error() then reports that iteration's
+//     Error[0..3] and Update() moves the device's new state into U_Cells. Cells_DelU holds the
+//     iteration's increment U^{n+1} - U^n (as the reference leaves it).
+// Second order: Evaluate_Cell_Net_Flux_2O is the device path's MUSCL (limiter
+// Limiter_Type, gradients Gradient_Type), not the reference's Roe-2O limited differences.
+// Fatal conditions throw std::runtime_error (the reference calls exit()).
+#pragma once
+
+#include <string>
+#include <vector>
+
+typedef std::vector<double> V_D;
+
+// ---- state (include/Globals.h:332-340) ------------------------------------------------------
+extern std::vector<V_D> U_Cells;         // [Total_No_Cells][4]: physical cells, then ghosts
+extern std::vector<V_D> Cells_Net_Flux;  // [No_Physical_Cells][4]
+extern std::vector<V_D> Cells_DelU;      // [No_Physical_Cells][4]
+extern V_D Error;                        // [4]
+extern V_D Cells_del_t;                  // [No_Physical_Cells] the reference's Cells[i].del_t
+extern double Min_dt;
+extern int No_Physical_Cells, Total_No_Cells;
+
+// ---- configuration (include/Globals.h:349-399) ----------------------------------------------
+extern double CFL;
+extern int Dissipation_Type;  // 1 LLF, 3 ROE (reference); 101 HLLC, 102 AUSM (this library)
+extern bool Is_Second_Order, Local_Time_Stepping, Is_Inlet_SubSonic, Is_Exit_SubSonic;
+extern int Limiter_Type;      // second order: 1 minmod, 2 Van Leer, 3 superbee
+extern int Gradient_Type;     // second order: 0 Green-Gauss, 1 weighted least squares
+
+// Inlet state used by the inlet ghosts (the reference's InletCondition inletCond, Globals.h:167-190)
+struct InletCondition {
+    double P = 1.0, Rho = 1.0, M = 0.0, u = 0.0, v = 0.0;
+};
+extern InletCondition inletCond;
+
+// ---- driver ----------------------------------------------------------------------------------
+// Reads a legacy ASCII VTK unstructured grid (triangles and quads; other cell types skipped)
+// and forms the cells (connectivity, geometry, ghosts, boundary lists) on the device.
+bool Read_VTK_Grid(const std::string &file);
+// Forms the cells from points [3*npts] and cells [4*ncells] (triangles padded with -1).
+bool Form_Cells(int npts, const double *pts3, int ncells, const int *cells4);
+
+void Apply_Boundary_Conditions();
+void Evaluate_Cell_Net_Flux_1O();
+void Evaluate_Cell_Net_Flux_2O();
+void Explicit_Method();
+void Runge_Kutta_Method();
+void Estimate_Error();
+void Update();
+
+// Releases the device mesh and solver.
+void Release_Solver();

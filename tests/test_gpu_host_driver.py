@@ -1,0 +1,42 @@
+"""GPU test of the reference-named C++ host driver (include/cfd_host.hpp, libcfdhost.so): a C++
+program (tests/host/sod_driver.cpp) runs the reference's Inviscid_Solver loop -- Read_VTK_Grid,
+Apply_Boundary_Conditions, Runge_Kutta_Method, Estimate_Error, Update -- on the Sod tube and
+must reproduce the reference's own 25-step outputs (tests/golden/sod_roe.npz)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from cfd_solver_withcuda_amd import meshgen
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "tests", "host", "_build", "sod_driver")
+
+
+def test_host_driver_sod_vs_reference_golden(tmp_path):
+    assert os.path.exists(EXE), "build first: python -m cfd_solver_withcuda_amd.build"
+    g = np.load(os.path.join(ROOT, "tests", "golden", "sod_roe.npz"), allow_pickle=False)
+    m = meshgen.shock_tube_mesh(100, 10)
+    vtk = tmp_path / "sod.vtk"
+    meshgen.write_vtk(str(vtk), m)
+    u0 = tmp_path / "U0.bin"
+    np.ascontiguousarray(g["U0"], np.float64).tofile(u0)
+    out = tmp_path / "out.bin"
+    r = subprocess.run([EXE, str(vtk), str(u0), "25", str(out)], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    n = len(m.cells)
+    d = np.fromfile(out, np.float64)
+    U = d[:4 * n].reshape(n, 4)
+    hist = d[4 * n:4 * n + 25 * 5].reshape(25, 5)
+    R0 = d[4 * n + 125:8 * n + 125].reshape(n, 4)
+    dt0 = d[8 * n + 125:]
+    assert np.allclose(hist[:, :4], g["err_hist"], rtol=1e-10, atol=1e-13)
+    assert np.allclose(hist[:, 4], g["dt_hist"], rtol=1e-13, atol=0)
+    assert np.abs(U - g["U_step25"]).max() <= 1e-12 * np.abs(g["U_step25"]).max()
+    # Evaluate_Cell_Net_Flux_1O of the initial state against the reference's net flux and del_t
+    assert np.allclose(R0, g["R_roe"], rtol=0, atol=1e-13 * np.abs(g["R_roe"]).max())
+    assert np.allclose(dt0, g["dt_roe"], rtol=1e-14, atol=0)
