@@ -5,7 +5,7 @@ Workload (default): steady supersonic flow over a 10%-thick circular-arc bump, c
 (~1.27M cells); HLLC flux, MUSCL with the Van Leer limiter, weighted-least-squares gradients,
 local time stepping, TVD-RK3, FP64. One step = one RK3 step (3 gradient + 3 flux passes).
 
-    python bench.py [--steps K] [--warmup W] [--workload bump|ffs] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--workload bump|ffs|batch] [--no-cpu] [--warmstart 1]
 
 Prints one JSON line. `value` is FP64 face-flux evaluations per second (unique faces times
 residual evaluations per second) with the state resident in HBM. The roofline entry is the
@@ -51,7 +51,19 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=40)
     p.add_argument("--warmup", type=int, default=5)
-    p.add_argument("--workload", default="bump", choices=["bump", "ffs"])
+    p.add_argument("--workload", default="bump", choices=["bump", "ffs", "batch"])
+    p.add_argument("--warmstart", type=int, default=0,
+                   help="1: also measure time to Error[0] <= 1e-8 of batched bump-family solves, "
+                        "surrogate-warm vs freestream starts (sub-object time_to_residual)")
+    p.add_argument("--ws-train", type=int, default=32, help="training cases (bump family)")
+    p.add_argument("--ws-eval", type=int, default=32, help="held-out cases, solved in one batch")
+    p.add_argument("--ws-train-cells", type=int, default=20000)
+    p.add_argument("--ws-eval-cells", type=int, default=50000)
+    p.add_argument("--ws-mlp-steps", type=int, default=2000)
+    p.add_argument("--ws-max-steps", type=int, default=40000)
+    p.add_argument("--instances", type=int, default=32, help="batch workload: ramp instances")
+    p.add_argument("--min-cells", type=int, default=20000)
+    p.add_argument("--max-cells", type=int, default=100000)
     p.add_argument("--tile", type=int, default=128, help="max cells per tile")
     p.add_argument("--face-budget", type=int, default=256, help="max faces per tile")
     p.add_argument("--block", type=int, default=128)
@@ -125,6 +137,61 @@ def cpu_baseline(m, cfg, U0, budget_s=15.0):
                        f"({dt:.1f} s)")
 
 
+def warmstart_bench(args):
+    """Time to Error[0] <= 1e-8 of held-out bump-family solves, batched, from the surrogate's
+    field vs from freestream (the surrogate is trained first on the solver's own solutions)."""
+    from cfd_solver_withcuda_amd import warmstart
+
+    r = warmstart.run(n_train=args.ws_train, n_eval=args.ws_eval, train_cells=args.ws_train_cells,
+                      eval_cells=args.ws_eval_cells, mlp_steps=args.ws_mlp_steps, max_steps=args.ws_max_steps,
+                      log=lambda msg: print(msg, file=sys.stderr, flush=True))
+    return {"plain_s": r["plain"]["wall_s"], "warm_s": r["warm"]["wall_s"], "speedup": r["speedup"],
+            "plain_steps_max": r["plain"]["steps_max"], "warm_steps_max": r["warm"]["steps_max"],
+            "plain_converged": r["plain"]["converged"], "warm_converged": r["warm"]["converged"],
+            "instances": args.ws_eval, "cells_per_instance": args.ws_eval_cells,
+            "fallbacks": r["warm"]["fallbacks"], "surrogate_rel_l2": r["surrogate_rel_l2"],
+            "converged_fields_max_rel_diff": r["converged_fields_max_rel_diff"],
+            "training": {"cases": r["train"]["cases"], "solve_s": r["train"]["wall_s"],
+                         "mlp_steps": r["mlp"]["steps"], "mlp_s": r["mlp"]["train_s"],
+                         "loss_first": r["mlp"]["loss_first"], "loss_last": r["mlp"]["loss_last"]},
+            "scheme": "HLLC + MUSCL/Van Leer + WLSQ, LTS TVD-RK3, limiter freezing on stall"}
+
+
+def batch_bench(args):
+    """Batched steady compression-ramp solves (AUSM + MUSCL/Van Leer + WLSQ, LTS TVD-RK3) to
+    Error[0] <= 1e-8 per instance, in one batch with an autotuned launch shape."""
+    from cfd_solver_withcuda_amd import cases
+    from cfd_solver_withcuda_amd.batch import Batch
+    from cfd_solver_withcuda_amd.solver import make_config
+
+    t = time.perf_counter()
+    cs = cases.ramp_batch(args.instances, min_cells=args.min_cells, max_cells=args.max_cells)
+    cfg = make_config(flux="ausm", order=2, limiter="vanleer", gradient="wlsq", cfl=0.8, rk=3, lts=1,
+                      roe_bdiss=1, bc=cases.RAMP_BCS, block=0, freeze_stall=300)
+    b = Batch([c.mesh for c in cs], np.array([c.fs for c in cs]), cfg)
+    info = b.info()
+    setup = time.perf_counter() - t
+    ms = b.time_steps(args.steps)
+    b.set_state([np.tile(np.array([c.fs[0], c.fs[0] * c.fs[1], c.fs[0] * c.fs[2],
+                                   c.fs[3] / 0.4 + 0.5 * c.fs[0] * (c.fs[1] ** 2 + c.fs[2] ** 2)]),
+                         (len(c.mesh.cells), 1)) for c in cs])
+    r = b.solve(tol=1e-8, max_steps=args.ws_max_steps, check_every=100)
+    out = {
+        "metric": "time to residual 1e-8, batched plain FP64 solves",
+        "value": r["wall_s"], "unit": "s", "n_gpus": 1, "steps": int(r["steps"].max()), "warmup": 0,
+        "ms_per_step": ms / args.steps, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (seeded ramp family, freestream initial states)",
+        "config": {"workload": f"ramp_batch_{args.instances}x[{args.min_cells},{args.max_cells}]_ausm_muscl_vanleer_wlsq_lts_rk3",
+                   "cells": info["cells"], "tiles": info["tiles"], "autotuned": {k: info[k] for k in ("tile", "face_budget", "block")}},
+        "cell_updates_per_s": info["cells"] / (ms / args.steps / 1e3),
+        "converged": int((r["status"] == 0).sum()), "instances": args.instances,
+        "steps_per_instance": {"min": int(r["steps"].min()), "max": int(r["steps"].max())},
+        "setup_s": setup, "autotune_s": info["autotune_s"],
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -145,6 +212,8 @@ def main():
 
     from cfd_solver_withcuda_amd.solver import Mesh, Solver, cons_from_prim, make_config
 
+    if args.workload == "batch":
+        return batch_bench(args)
     m, cfg, desc = workload(args.workload)
     t_setup = time.perf_counter()
     mesh = Mesh(m.points, m.cells, tile=args.tile, renumber=1, face_budget=args.face_budget)
@@ -207,6 +276,9 @@ def main():
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(m, cfg, U0)
+    if args.warmstart:
+        del s, mesh
+        out["time_to_residual"] = warmstart_bench(args)
     print(json.dumps(out), flush=True)
     return 0
 
