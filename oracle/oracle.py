@@ -314,6 +314,8 @@ class Reference:
         L.ref_get_delU.argtypes = [_dp]
         L.ref_gg_gradient.argtypes = [ctypes.c_int, _dp]
         L.ref_prim.argtypes = [_dp, _dp]
+        L.ref_write_solution.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.ref_write_error.argtypes = [ctypes.c_char_p]
         self.counts = None
 
     def load_vtk(self, path: str):
@@ -363,6 +365,13 @@ class Reference:
         tt = ctypes.c_double()
         self.lib.ref_step(_d(err), ctypes.byref(mdt), ctypes.byref(tt))
         return err, mdt.value, tt.value
+
+    def write_solution(self, path: str, iterations: int, header=(0, 0, 0)):
+        """Write_Solution(path, 1) of the reference's current state."""
+        self.lib.ref_write_solution(path.encode(), iterations, *[int(h) for h in header])
+
+    def write_error(self, path: str):
+        self.lib.ref_write_error(path.encode())
 
     def state(self):
         nt = self.nphys + self.nghost

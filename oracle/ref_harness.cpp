@@ -94,6 +94,9 @@ void set_cell_state(int c, const double *U4) {
 }
 }  // namespace
 
+void Write_Solution(const string &, const int &);
+void Write_Error_File(const string &);
+
 extern "C" {
 
 // Reads a legacy ASCII VTK mesh through Form_Cells (src/Grid_Computations.cpp:319-392).
@@ -262,4 +265,16 @@ void ref_prim(const double *U4, double *prim11) {
     for (int k = 0; k < 11; k++) prim11[k] = P[k];
 }
 
+
+// Write_Solution(file, 1) and Write_Error_File (src/output_files.cpp:492-575) of the current
+// state, with the header fields and iteration count given.
+void ref_write_solution(const char *path, int iters, int n1, int n2, int n3) {
+    Grid_Type = 0;
+    nx_c = n1;
+    ny_c = n2;
+    nz_c = n3;
+    iterations = iters;
+    Write_Solution(std::string(path), 1);
+}
+void ref_write_error(const char *path) { Write_Error_File(std::string(path)); }
 }  // extern "C"

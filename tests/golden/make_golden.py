@@ -12,6 +12,11 @@ Fixtures:
   sod_roe.npz       Sod tube 100x10: Roe-1O and LLF-1O residuals at t=0, 25 TVD-RK3 steps
   bump_mixed.npz    jittered mixed bump mesh: ghost BC arrays, Roe-1O/LLF-1O/Roe-2O/AUSM
                     residuals, Green-Gauss gradients, LTS RK3 and forward-Euler steps
+  sod_solution_ref.txt, sod_error_ref.txt
+                    the reference's Write_Solution (type 1) of the Sod initial state after one
+                    Roe-1O residual, and its Write_Error_File rows of 3 TVD-RK3 steps
+
+    python tests/golden/make_golden.py io    # only the two text files
 """
 from __future__ import annotations
 
@@ -67,8 +72,34 @@ def smooth_state(ctr, nc, rho0=1.4, u0=1.65, p0=1.0):
     return np.ascontiguousarray(cons_from_prim(rho, u, v, p))
 
 
+def io_files(ref):
+    m = gen_mesh("sod_100x10")
+    load(ref, m)
+    a = ref.arrays()
+    nc = a.nphys
+    xc = a.ctr[:nc, 0]
+    left = xc < 0.5
+    U0 = np.ascontiguousarray(cons_from_prim(np.where(left, 1.0, 0.125), 0 * xc, 0 * xc,
+                                             np.where(left, 1.0, 0.1)))
+    ref.configure(dissipation=3, second_order=0, time_accurate=1, lts=0, cfl=0.5, rho_in=1.0, p_in=1.0)
+    ref.init_state(U0)
+    ref.apply_bc()
+    ref.residual()
+    ref.write_solution(os.path.join(HERE, "sod_solution_ref.txt"), 0, (101, 11, 1))
+    err_path = os.path.join(HERE, "sod_error_ref.txt")
+    if os.path.exists(err_path):
+        os.remove(err_path)
+    ref.init_state(U0)
+    for _ in range(3):
+        ref.step()
+        ref.write_error(err_path)
+
+
 def main():
     ref = Reference()
+    if sys.argv[1:] == ["io"]:
+        io_files(ref)
+        return
     out = {}
     for name in GEN_MESHES:
         load(ref, gen_mesh(name))

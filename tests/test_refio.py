@@ -1,0 +1,62 @@
+"""Reference-compatible files (cfd_solver_withcuda_amd/refio.py) against the reference's own
+writers: Write_Solution type 1 and Write_Error_File outputs on the Sod tube
+(tests/golden/sod_solution_ref.txt, sod_error_ref.txt), the restart round trip, the VTK solution
+layout and the JSON configuration mapping. CPU only."""
+import os
+
+import numpy as np
+
+from cfd_solver_withcuda_amd import meshgen, refio
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def sod():
+    return np.load(os.path.join(GOLD, "sod_roe.npz"), allow_pickle=False)
+
+
+def test_write_solution_matches_reference_text(tmp_path):
+    g = sod()
+    p = tmp_path / "sol.txt"
+    refio.write_solution(str(p), g["U0"], g["dt_roe"], 0, (101, 11, 1))
+    assert p.read_text() == open(os.path.join(GOLD, "sod_solution_ref.txt")).read()
+
+
+def test_error_rows_match_reference_text(tmp_path):
+    g = sod()
+    p = tmp_path / "err.txt"
+    for k in range(3):
+        refio.append_error(str(p), g["err_hist"][k])
+    assert p.read_text() == open(os.path.join(GOLD, "sod_error_ref.txt")).read()
+
+
+def test_restart_round_trip(tmp_path):
+    g = sod()
+    p = tmp_path / "sol.txt"
+    U = g["U_step25"]
+    refio.write_solution(str(p), U, g["dt_roe"], 25, (101, 11, 1))
+    U2, dt, it, hdr = refio.read_solution(str(p))
+    assert it == 25 and hdr == [101, 11, 1] and U2.shape == U.shape
+    assert np.allclose(U2, U, rtol=2e-5, atol=1e-6)
+
+
+def test_vtk_solution_layout(tmp_path):
+    m = meshgen.bump_mesh(12, 6, 0.1, seed=2)
+    U = np.tile([1.4, 1.4 * 1.65, 0.0, 1.0 / 0.4 + 0.5 * 1.4 * 1.65 ** 2], (len(m.cells), 1))
+    p = tmp_path / "sol.vtk"
+    refio.write_vtk_solution(str(p), m.points, m.cells, U)
+    text = p.read_text()
+    back = meshgen.read_vtk(str(p))
+    assert np.array_equal(back.cells, m.cells)
+    for name in ("Pressure", "Temperature", "Density", "U_Velocity", "V_Velocity", "Total_Pressure",
+                 "Entropy", "Mach_Number"):
+        assert f"SCALARS\t {name}\t double" in text
+    assert f"CELL_DATA\t{len(m.cells)}" in text and "Vectors\t Velocity" in text
+
+
+def test_config_from_reference_json(tmp_path):
+    p = tmp_path / "cfg.json"
+    p.write_text('{ "Simulation": {"CFL": 0.4, // comment\n "Total_Iterations": 10},'
+                 ' "Solver": {"Dissipation_Type": 3, "Is_Second_Order": false, "Local_Time_Stepping": true} }')
+    kw = refio.config_from_json(str(p))
+    assert kw == dict(flux="roe", order=1, cfl=0.4, lts=1, rk=3)
