@@ -64,9 +64,14 @@ struct cfd_mesh_s {
     double *d_area = nullptr, *d_inv_area = nullptr;
     int *d_gcell = nullptr, *d_gkind = nullptr;  // ghost k: renumbered owner cell, face kind
     double *d_gnrm = nullptr;                    // ghost k: unit normal of the owner's face
+    // the reference's per-cell arrays in renumbered order (built on first use by the reference's
+    // second order)
+    int *d_rc_nf = nullptr, *d_rc_nbr = nullptr;
+    double *d_rc_dist = nullptr, *d_rc_nrm = nullptr, *d_rc_len = nullptr, *d_rc_area = nullptr;
     DevMesh dev{};
     ~cfd_mesh_s() {
-        void *ps[] = {d_tile_cstart, d_tile_fstart, d_fL, d_fR, d_fnx, d_fny, d_fdl, d_fdx, d_fdy, d_fwL, d_fwR, d_slots, d_area, d_inv_area, d_gcell, d_gkind, d_gnrm};
+        void *ps[] = {d_tile_cstart, d_tile_fstart, d_fL, d_fR, d_fnx, d_fny, d_fdl, d_fdx, d_fdy, d_fwL, d_fwR, d_slots, d_area, d_inv_area, d_gcell, d_gkind, d_gnrm,
+                      d_rc_nf, d_rc_nbr, d_rc_dist, d_rc_nrm, d_rc_len, d_rc_area};
         for (void *p : ps)
             if (p) (void)hipFree(p);
     }
@@ -81,13 +86,14 @@ struct cfd_solver_s {
     double *scal = nullptr;  // [0],[1] global dt slots, [2] stage minimum, [3..6] error norm
     double4 *Ug = nullptr;   // ghost states (reference ghost order), allocated on first use
     double *psi = nullptr;   // [8 * face entries] frozen limiter factors, allocated on first use
+    double4 *Uall = nullptr; // [cells + ghosts] state for the reference's second order
     int *fz = nullptr;       // [0] freezing state, [1] steps since the best Error[0]
     double *fz_best = nullptr;
     int cur = 0;
     bool dt_valid = false;
     bool has_state = false;
     ~cfd_solver_s() {
-        void *ps[] = {Un, Ua, Ub, acc, R, dt, grad, hbuf, err_partial, dt_partial, scal, Ug, psi, fz, fz_best};
+        void *ps[] = {Un, Ua, Ub, acc, R, dt, grad, hbuf, err_partial, dt_partial, scal, Ug, psi, fz, fz_best, Uall};
         for (void *p : ps)
             if (p) (void)hipFree(p);
     }
@@ -369,11 +375,59 @@ static StageArgs resid_args(cfd_solver_s *s, bool want_min, const double4 *uexit
     return st;
 }
 
+static int ensure_ref_cells(cfd_mesh_s *m) {
+    if (m->d_rc_nf) return 0;
+    const RefMesh &R = m->ref;
+    const Layout &L = m->lay;
+    const int nc = R.nc;
+    std::vector<int> nf(nc), nbr(4 * (size_t)nc, -1);
+    std::vector<double> dist(4 * (size_t)nc), nrm(8 * (size_t)nc), len(4 * (size_t)nc), area(nc);
+    for (int k = 0; k < nc; k++) {
+        const int c = L.perm[k];
+        nf[k] = R.nf[c];
+        for (int f = 0; f < 4; f++) {
+            const int nb = R.nbr[4 * c + f];
+            nbr[4 * k + f] = (nb < 0) ? -1 : (nb < nc ? L.iperm[nb] : nb);
+            dist[4 * k + f] = R.dist[4 * c + f];
+            len[4 * k + f] = R.len[4 * c + f];
+            nrm[8 * k + 2 * f] = R.nrm[8 * c + 2 * f];
+            nrm[8 * k + 2 * f + 1] = R.nrm[8 * c + 2 * f + 1];
+        }
+        area[k] = R.area[c];
+    }
+    HIP_TRY(upload(&m->d_rc_nf, nf));
+    HIP_TRY(upload(&m->d_rc_nbr, nbr));
+    HIP_TRY(upload(&m->d_rc_dist, dist));
+    HIP_TRY(upload(&m->d_rc_nrm, nrm));
+    HIP_TRY(upload(&m->d_rc_len, len));
+    HIP_TRY(upload(&m->d_rc_area, area));
+    return 0;
+}
+
+// The reference's Evaluate_Cell_Net_Flux_2O (Roe on limited differences): physical state and
+// ghosts (exit ghosts from uexit when set) gathered into Uall, then the cell-by-cell kernel.
+static int ref_roe2o_pass(cfd_solver_s *s, const double4 *in, double *dtmin, const double4 *uexit, hipStream_t st) {
+    cfd_mesh_s *m = s->m;
+    const int nc = m->ref.nc, ng = m->ref.ng;
+    int rc = ensure_ref_cells(m);
+    if (rc) return rc;
+    if (!s->Uall) HIP_TRY(hipMalloc((void **)&s->Uall, (size_t)(nc + ng) * sizeof(double4)));
+    HIP_TRY(hipMemcpyAsync(s->Uall, in, (size_t)nc * sizeof(double4), hipMemcpyDeviceToDevice, st));
+    HIP_TRY(launch_ghosts(ng, m->d_gcell, reinterpret_cast<const double2 *>(m->d_gnrm), m->d_gkind, s->dcfg, in,
+                          uexit, s->Uall + nc, st));
+    const RefCells cells{m->d_rc_nf, m->d_rc_nbr, m->d_rc_dist, m->d_rc_nrm, m->d_rc_len, m->d_rc_area};
+    HIP_TRY(launch_roe2o(nc, cells, s->Uall, s->cfg.zeta, s->cfg.zeta1, s->cfg.cfl, s->R, s->dt,
+                         dtmin ? s->dt_partial : nullptr, st));
+    if (dtmin) HIP_TRY(launch_min_reduce(s->dt_partial, (nc + 255) / 256, dtmin, st));
+    return 0;
+}
+
 // residual of state `in` into s->R, s->dt (+ minimum dt into *dtmin when set)
 static int residual_pass(cfd_solver_s *s, const double4 *in, double *dtmin, const double4 *uexit,
                          hipStream_t st, int order_override = 0) {
     const cfd_config_t &c = s->cfg;
     const int order = order_override ? order_override : c.order;
+    if (c.ref_mode && order == 2 && c.flux == FLUX_ROE) return ref_roe2o_pass(s, in, dtmin, uexit, st);
     if (order == 2) HIP_TRY(launch_grad(s->m->dev, s->dcfg, c.gradient, in, s->grad, c.block, st));
     StageArgs a = resid_args(s, dtmin != nullptr, uexit);
     HIP_TRY(launch_flux(s->m->dev, s->dcfg, c.flux, order, c.limiter, MODE_RESID, in, s->grad, a, c.block, st));

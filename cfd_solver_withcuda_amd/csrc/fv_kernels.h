@@ -52,6 +52,19 @@ hipError_t launch_batch_list(int ninst, int ntiles, const int *inst_tile, const 
 // fv_kernels.hip, freeze_update_kernel).
 hipError_t launch_freeze_update(const double *err4, int *freeze, double *best, int *since,
                                 double freeze_tol, int freeze_stall, hipStream_t s);
+// The reference's per-cell arrays (renumbered cells; neighbours >= nc are ghosts nc + k).
+struct RefCells {
+    const int *nf;        // [nc] faces per cell
+    const int *nbr;       // [4 nc]
+    const double *dist;   // [4 nc] centre-to-neighbour-centre distances
+    const double *nrm;    // [8 nc] unit normals
+    const double *len;    // [4 nc] face lengths
+    const double *area;   // [nc]
+};
+// Evaluate_Cell_Net_Flux_2O with ROE_2O: R, dt (and per-block dt minima into dt_partial,
+// ceil(nc / 256) of them, when not null) from U_all = physical cells then ghosts.
+hipError_t launch_roe2o(int nc, const RefCells &rc, const double4 *U_all, double zeta, double zeta1,
+                        double cfl, double4 *R, double *dt, double *dt_partial, hipStream_t s);
 hipError_t launch_min_reduce(const double *partial, int n, double *out, hipStream_t s);
 hipError_t launch_ref_update(int nc, int kind, int lts, const double *inv_area, const double4 *Un,
                              const double4 *Us, const double4 *R, const double *dt_cell,

@@ -630,6 +630,129 @@ __global__ void __launch_bounds__(256) ghost_kernel(int n, const int *__restrict
     Ug[k] = make_double4(g.U[0], g.U[1], g.U[2], g.U[3]);
 }
 
+// ---- the reference's second order: Roe dissipation on limited differences ------------------
+// Evaluate_Cell_Net_Flux_2O with ROE_2O (src/Net_Flux.cpp:97-128, src/Roe_Scheme.cpp:257-447)
+// and Second_Order_Limiter with the 2-argument MinMod (src/Limiters.cpp:58-63,97-200), cell by
+// cell on the reference's per-cell arrays (renumbered). U_all holds the physical cells then the
+// ghosts. Each face is evaluated from both sides, as the reference does.
+__device__ __forceinline__ double sign_of(double a) { return (double)((a > 0.0) - (a < 0.0)); }
+
+__global__ void __launch_bounds__(256) roe2o_kernel(int nc, const RefCells rc, const double4 *__restrict__ U_all,
+                                                    double zeta, double zeta1, double cfl,
+                                                    double4 *__restrict__ R, double *__restrict__ dt,
+                                                    double *__restrict__ dt_partial) {
+    __shared__ double red[256];
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    double dtc = 1.7976931348623157e308;
+    if (c < nc) {
+        FState L;
+        {
+            const double4 u = U_all[c];
+            state_from_U(u.x, u.y, u.z, u.w, L);
+        }
+        const int nf = rc.nf[c];
+        double net[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int f = 0; f < nf; f++) {
+            const int j = rc.nbr[4 * c + f];
+            const double nx = rc.nrm[8 * c + 2 * f], ny = rc.nrm[8 * c + 2 * f + 1], dl = rc.len[4 * c + f];
+            FState Rs;
+            {
+                const double4 u = U_all[j];
+                state_from_U(u.x, u.y, u.z, u.w, Rs);
+            }
+            double A[4];
+            avg_flux(L, Rs, nx, ny, dl, A);
+            const double Rho_L = L.W[0], u_L = L.W[1], v_L = L.W[2], C_L = L.W[4];
+            const double Vmag_L = 0.5 * (u_L * u_L + v_L * v_L);
+            const double H_L = ((C_L * C_L) / (kGamma - 1.0)) + Vmag_L;
+            const double Rho_R = Rs.W[0], u_R = Rs.W[1], v_R = Rs.W[2], C_R = Rs.W[4];
+            const double Vmag_R = 0.5 * (u_R * u_R + v_R * v_R);
+            const double H_R = ((C_R * C_R) / (kGamma - 1.0)) + Vmag_R;
+            const double sqrt_Rho_R = sqrt(Rho_R);
+            const double sqrt_Rho_L = sqrt(Rho_L);
+            const double Term1 = 1.0 / (sqrt_Rho_L + sqrt_Rho_R);
+            const double Roe_Rho = sqrt(Rho_L * Rho_R);
+            const double Roe_u = Term1 * (u_R * sqrt_Rho_R + u_L * sqrt_Rho_L);
+            const double Roe_v = Term1 * (v_R * sqrt_Rho_R + v_L * sqrt_Rho_L);
+            const double Roe_H = Term1 * (H_R * sqrt_Rho_R + H_L * sqrt_Rho_L);
+            const double Roe_Vmag = 0.5 * (Roe_u * Roe_u + Roe_v * Roe_v);
+            const double Un = nx * Roe_u + ny * Roe_v;
+            const double Ut = -ny * Roe_u + nx * Roe_v;
+            const double Roe_a = sqrt((kGamma - 1.0) * (Roe_H - Roe_Vmag));
+            // limited differences d_U (Second_Order_Limiter): quads take the next cell across
+            // the opposite face, other cells fall back to first order
+            int N1, N2;
+            double d1, d2;
+            N1 = j;
+            d1 = rc.dist[4 * c + f];
+            if (nf != 4) {
+                if (d1 <= 0.0) d1 = 1.0;
+                N2 = N1;
+                d2 = d1;
+            } else if (f < 2) {
+                N2 = (N1 >= nc) ? N1 : rc.nbr[4 * N1 + f];
+                d2 = (N1 >= nc) ? d1 : rc.dist[4 * N1 + f];
+            } else {
+                N2 = rc.nbr[4 * c + f - 2];
+                d2 = rc.dist[4 * c + f - 2];
+            }
+            const double4 uc4 = U_all[c], u14 = U_all[N1], u24 = U_all[N2];
+            const double uc[4] = {uc4.x, uc4.y, uc4.z, uc4.w}, u1[4] = {u14.x, u14.y, u14.z, u14.w},
+                         u2[4] = {u24.x, u24.y, u24.z, u24.w};
+            double d_U[4];
+            for (int k = 0; k < 4; k++) {
+                const double Slope1 = zeta * (uc[k] - u1[k]) / d1;
+                const double Slope2 = zeta1 * (u1[k] - u2[k]) / d2;
+                const double a1 = fabs(Slope1), b1 = fabs(Slope2);
+                const double phi = 0.5 * (sign_of(Slope1) + sign_of(Slope2)) * (b1 < a1 ? b1 : a1);
+                const double dR = u1[k] + 0.5 * phi * d1;
+                const double dL = uc[k] - 0.5 * phi * d1;
+                d_U[k] = dR - dL;
+            }
+            const double du = d_U[1] / Roe_Rho - d_U[0] * Roe_u / (Roe_Rho * Roe_Rho);
+            const double dv = d_U[2] / Roe_Rho - d_U[0] * Roe_v / (Roe_Rho * Roe_Rho);
+            const double dE = d_U[3] - 0.5 * Roe_Rho * (2.0 * (Roe_u * du + Roe_v * dv) + du * du + dv * dv) -
+                              0.5 * d_U[0] * Roe_Vmag;
+            const double dP = (kGamma - 1.0) * dE;
+            const double dRho = d_U[0];
+            const double dUn = nx * du + ny * dv;
+            const double dUt = -ny * du + nx * dv;
+            double L1 = fabs(Un - Roe_a), L2 = fabs(Un), L3 = fabs(Un), L4 = fabs(Un + Roe_a);
+            const double ef = 0.1 * Roe_a;
+            if (L1 < ef) L1 = 0.5 * (L1 * L1 / ef + ef);
+            if (L4 < ef) L4 = 0.5 * (L4 * L4 / ef + ef);
+            const double R12 = Roe_u - nx * Roe_a, R13 = Roe_v - ny * Roe_a, R14 = Roe_H - Roe_a * Un;
+            const double R22 = Roe_u, R23 = Roe_v, R24 = Roe_Vmag;
+            const double R32 = -ny, R33 = nx, R34 = Ut;
+            const double R42 = Roe_u + nx * Roe_a, R43 = Roe_v + ny * Roe_a, R44 = Roe_H + Roe_a * Un;
+            const double Term2 = 1.0 / (Roe_a * Roe_a);
+            const double a1 = 0.5 * Term2 * (dP - Roe_Rho * Roe_a * dUn);
+            const double a2 = dRho - Term2 * dP;
+            const double a3 = Roe_Rho * dUt;
+            const double a4 = 0.5 * Term2 * (dP + Roe_Rho * Roe_a * dUn);
+            double D[4];
+            D[0] = 0.5 * (L1 * a1 * 1.0 + L2 * a2 * 1.0 + L3 * a3 * 0.0 + L4 * a4 * 1.0) * dl;
+            D[1] = 0.5 * (L1 * a1 * R12 + L2 * a2 * R22 + L3 * a3 * R32 + L4 * a4 * R42) * dl;
+            D[2] = 0.5 * (L1 * a1 * R13 + L2 * a2 * R23 + L3 * a3 * R33 + L4 * a4 * R43) * dl;
+            D[3] = 0.5 * (L1 * a1 * R14 + L2 * a2 * R24 + L3 * a3 * R34 + L4 * a4 * R44) * dl;
+            for (int k = 0; k < 4; k++) net[k] += A[k] - D[k];
+        }
+        R[c] = make_double4(net[0], net[1], net[2], net[3]);
+        // Inviscid_Time_Step (src/Time_Step.cpp:293-329) of the cell's own state
+        double denom = 0.0;
+        for (int f = 0; f < nf; f++) {
+            const double vn = L.W[1] * rc.nrm[8 * c + 2 * f] + L.W[2] * rc.nrm[8 * c + 2 * f + 1];
+            denom += (fabs(vn) + L.W[4]) * rc.len[4 * c + f];
+        }
+        dtc = (denom <= 0.0 || !isfinite(denom)) ? 0.0 : cfl * rc.area[c] / denom;
+        dt[c] = dtc;
+    }
+    if (dt_partial) {
+        const double b = block_min(red, dtc);
+        if (threadIdx.x == 0) dt_partial[blockIdx.x] = b;
+    }
+}
+
 // ---- batched meshes ------------------------------------------------------------------------
 // After a step of a batch of independent instances: each marching instance (status 1) sums its
 // tiles' error partials [inst_tile[i], inst_tile[i+1]) in a fixed order (Estimate_Error of that
@@ -932,6 +1055,13 @@ hipError_t launch_batch_list(int ninst, int ntiles, const int *inst_tile, const 
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || ntiles <= 0) return e;
     hipLaunchKernelGGL(batch_list_fill_kernel, dim3((ntiles + 255) / 256), dim3(256), 0, s, ntiles, tile_inst, inst_tile, status, inst_off, tile_list);
+    return hipGetLastError();
+}
+
+hipError_t launch_roe2o(int nc, const RefCells &rc, const double4 *U_all, double zeta, double zeta1,
+                        double cfl, double4 *R, double *dt, double *dt_partial, hipStream_t s) {
+    if (nc <= 0) return hipSuccess;
+    hipLaunchKernelGGL(roe2o_kernel, dim3((nc + 255) / 256), dim3(256), 0, s, nc, rc, U_all, zeta, zeta1, cfl, R, dt, dt_partial);
     return hipGetLastError();
 }
 

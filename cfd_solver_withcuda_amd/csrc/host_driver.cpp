@@ -18,6 +18,7 @@ double CFL = 0.5;
 int Dissipation_Type = 3;
 bool Is_Second_Order = false, Local_Time_Stepping = false, Is_Inlet_SubSonic = false, Is_Exit_SubSonic = false;
 int Limiter_Type = 2, Gradient_Type = 1;
+double Limiter_Zeta = 0.5, Limiter_Zeta1 = 0.5;
 InletCondition inletCond;
 
 namespace {
@@ -68,6 +69,8 @@ cfd_config_t config(int order, int rk, int ref_mode) {
     c.block = 128;
     c.xcd_remap = 1;
     c.grad_lds = 0;
+    c.zeta = Limiter_Zeta;
+    c.zeta1 = Limiter_Zeta1;
     return c;
 }
 
@@ -109,7 +112,8 @@ void one_iteration(int rk, const char *where) {
 
 void residual(int order, const char *where) {
     need_mesh(where);
-    configure(order, 3, 0);
+    // the reference's second order is Roe on limited differences (ROE_2O); other fluxes use MUSCL
+    configure(order, 3, (order == 2 && Dissipation_Type == 3) ? 1 : 0);
     sync_state();
     const int n = No_Physical_Cells;
     std::vector<double> R(4 * (size_t)n), dt(n);

@@ -29,7 +29,7 @@ REF_BCS = ("inlet_ref", "exit", "wall", "wall", "wall")
 
 def make_config(flux="roe", order=1, limiter="vanleer", gradient="gg", roe_bdiss=0, cfl=0.5,
                 bc=REF_BCS, fs=(1.0, 0.0, 0.0, 1.0), rk=3, lts=0, ref_mode=0, block=256,
-                xcd_remap=1, grad_lds=1, freeze_tol=0.0, freeze_stall=0) -> Config:
+                xcd_remap=1, grad_lds=1, freeze_tol=0.0, freeze_stall=0, zeta=0.5, zeta1=0.5) -> Config:
     c = Config()
     c.flux = FLUX[flux]
     c.order = int(order)
@@ -52,6 +52,8 @@ def make_config(flux="roe", order=1, limiter="vanleer", gradient="gg", roe_bdiss
     c.grad_lds = int(grad_lds)
     c.freeze_tol = float(freeze_tol)
     c.freeze_stall = int(freeze_stall)
+    c.zeta = float(zeta)
+    c.zeta1 = float(zeta1)
     return c
 
 

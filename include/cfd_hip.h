@@ -59,6 +59,10 @@ typedef struct {
      * reuse it (slope = psi * grad W . d). 0 and 0 disable it. */
     double freeze_tol;
     int freeze_stall;
+    /* Reference iteration (ref_mode = 1) at order 2 with the Roe flux: the reference's
+     * Evaluate_Cell_Net_Flux_2O (Roe dissipation on MinMod-limited differences,
+     * src/Roe_Scheme.cpp:257-447, src/Limiters.cpp:97-200) with Limiter_Zeta / Limiter_Zeta1. */
+    double zeta, zeta1;
 } cfd_config_t;
 
 const char *cfd_last_error(void);

@@ -16,8 +16,9 @@
 //     (ghost states, stages, error norm, update); Estimate_Error() then reports that iteration's
 //     Error[0..3] and Update() moves the device's new state into U_Cells. Cells_DelU holds the
 //     iteration's increment U^{n+1} - U^n (as the reference leaves it).
-// Second order: Evaluate_Cell_Net_Flux_2O is the device path's MUSCL (limiter
-// Limiter_Type, gradients Gradient_Type), not the reference's Roe-2O limited differences.
+// Second order: with Dissipation_Type 3 (ROE), Evaluate_Cell_Net_Flux_2O and the iterations are
+// the reference's Roe on MinMod-limited differences (Limiter_Zeta, Limiter_Zeta1); other fluxes
+// use the device path's MUSCL (limiter Limiter_Type, gradients Gradient_Type).
 // Fatal conditions throw std::runtime_error (the reference calls exit()).
 #pragma once
 
@@ -41,6 +42,7 @@ extern int Dissipation_Type;  // 1 LLF, 3 ROE (reference); 101 HLLC, 102 AUSM (t
 extern bool Is_Second_Order, Local_Time_Stepping, Is_Inlet_SubSonic, Is_Exit_SubSonic;
 extern int Limiter_Type;      // second order: 1 minmod, 2 Van Leer, 3 superbee
 extern int Gradient_Type;     // second order: 0 Green-Gauss, 1 weighted least squares
+extern double Limiter_Zeta, Limiter_Zeta1;  // the reference's ROE_2O limiter coefficients
 
 // Inlet state used by the inlet ghosts (the reference's InletCondition inletCond, Globals.h:167-190)
 struct InletCondition {

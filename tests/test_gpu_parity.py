@@ -237,3 +237,19 @@ def test_limiter_freezing_vs_oracle(orc, bump, flux, lim, grad):
         for _ in range(n):
             om.step(ocfg, U, rk=3, lts=1)
         assert np.array_equal(s.get_state(), U), mode
+
+
+def test_reference_second_order_roe_vs_golden():
+    """ref_mode at order 2 with Roe: the reference's Evaluate_Cell_Net_Flux_2O (Roe dissipation on
+    MinMod-limited differences, zeta = zeta1 = 0.5) against the reference's own residual of the
+    jittered mixed bump (triangles fall back to first order), bit for bit; dt as the reference's."""
+    g = np.load(os.path.join(GOLD, "bump_mixed.npz"), allow_pickle=False)
+    m = meshgen.bump_mesh(40, 20, 0.1, seed=7)
+    s = Solver(Mesh(m.points, m.cells, tile=64),
+               make_config(flux="roe", order=2, cfl=0.5, fs=(1.4, 0.0, 0.0, 1.0), rk=3, ref_mode=1))
+    s.set_state(g["U0"])
+    R, dt, _ = s.residual()
+    assert np.array_equal(R, g["R_roe2o"]), rel(R, g["R_roe2o"])
+    assert np.array_equal(dt, g["dt"])
+    s.step(3)
+    assert np.all(np.isfinite(s.get_state()))
