@@ -8,8 +8,12 @@
  *   Boundary_Conditions_Cuda_Kernels.cu:30-324, Cuda_Kernels.h:4-33,
  * with the corrections documented in the HIP file: limiter slope ratio from the opposite face,
  * neighbour limiter values taken from the face pointing back to the cell.
- * Parity pins: closed forms of the limiter functions at r in {-1,0,0.5,1,2,3}; the Roe/HLLC/LLF
- * fluxes equal the native face fluxes up to the documented differences (tests/test_oracle_math.py).
+ * Parity pins (tests/test_oracle_ref_layout.py): HLLC and LLF equal the native face fluxes; the
+ * Roe kernel text halves the dissipation (0.5 (F_L + F_R - D) with D = 0.5 sum |lambda| alpha R,
+ * Roe_Flux_Cuda_Kernels.cu:197-238) and equals the mean of the native Roe and central fluxes;
+ * limiter functions at r in {-1,0,0.5,1,2,3}; limiters, MUSCL and Barth-Jespersen on linear
+ * data; Venkatakrishnan's limits; boundary-state invariants; RK stage orders 3 and 4; the
+ * second-order Roe flux's first-order and uniform-state limits.
  */
 #include <math.h>
 #include <stddef.h>
