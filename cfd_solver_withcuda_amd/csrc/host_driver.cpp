@@ -2,6 +2,7 @@
 #include "../../include/cfd_host.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <fstream>
 #include <stdexcept>
@@ -20,6 +21,7 @@ bool Is_Second_Order = false, Local_Time_Stepping = false, Is_Inlet_SubSonic = f
 int Limiter_Type = 2, Gradient_Type = 1;
 double Limiter_Zeta = 0.5, Limiter_Zeta1 = 0.5;
 InletCondition inletCond;
+int nx_c = 0, ny_c = 0, nz_c = 0, iterations = 0;
 
 namespace {
 
@@ -222,4 +224,35 @@ void Release_Solver() {
     if (g_mesh) cfd_mesh_destroy(g_mesh);
     g_solver = nullptr;
     g_mesh = nullptr;
+}
+
+void Write_Solution(const std::string &file, const int &type) {
+    std::ofstream out(file.c_str(), std::ios::out);
+    if (!out.is_open()) throw std::runtime_error("Write_Solution: cannot open " + file);
+    if (type != 1) throw std::runtime_error("Write_Solution: only type 1 (cell state) is written");
+    out << nx_c << "\t" << ny_c << "\t" << nz_c << std::endl;
+    out << No_Physical_Cells << std::endl;
+    out << iterations << std::endl;
+    const double gamma = 1.4, gm1 = gamma - 1.0, R_gc = 287.0;
+    for (int c = 0; c < No_Physical_Cells; c++) {
+        const double Rho = U_Cells[c][0], inv = 1.0 / Rho;
+        double v1 = U_Cells[c][1] * inv, v2 = U_Cells[c][2] * inv;
+        if (std::fabs(v1) <= 1e-10) v1 = 0.0;
+        if (std::fabs(v2) <= 1e-10) v2 = 0.0;
+        const double vmag = v1 * v1 + v2 * v2;
+        const double P = gm1 * (U_Cells[c][3] - 0.5 * Rho * vmag);
+        const double T = P * inv / R_gc;
+        const double C = std::sqrt(gamma * P * inv);
+        const double M = std::sqrt(vmag) / C;
+        const double P0 = P * std::pow(1.0 + 0.5 * gm1 * M * M, gamma / gm1);
+        out << c << "\t\t" << Cells_del_t[c] << "\t" << Rho << "\t" << P << "\t" << T << "\t\t\t\t" << v1
+            << "\t\t\t\t" << v2 << "\t\t\t\t" << M << "\t\t\t\t" << P0 << std::endl;
+    }
+}
+
+void Write_Error_File(const std::string &file) {
+    std::ofstream out(file.c_str(), std::ios::out | std::ios::app);
+    if (!out.is_open()) throw std::runtime_error("Write_Error_File: cannot open " + file);
+    for (int i = 0; i < 4; i++) out << Error[i] << "\t";
+    out << std::endl;
 }

@@ -65,5 +65,13 @@ void Runge_Kutta_Method();
 void Estimate_Error();
 void Update();
 
+// Output in the reference's formats (src/output_files.cpp:492-575): Write_Solution type 1
+// ("id dt rho P T u v M P0" per cell after the header nx_c ny_c nz_c, No_Physical_Cells,
+// iterations) and one appended row of Error[0..3]. The primitives are formed from U_Cells as
+// Calculate_Primitive_Variables does (src/Basic_Functions.cpp:85-142).
+extern int nx_c, ny_c, nz_c, iterations;
+void Write_Solution(const std::string &file, const int &type);
+void Write_Error_File(const std::string &file);
+
 // Releases the device mesh and solver.
 void Release_Solver();

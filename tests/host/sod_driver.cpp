@@ -2,7 +2,7 @@
 // host driver (include/cfd_host.hpp): Apply_Boundary_Conditions -> Runge_Kutta_Method ->
 // Estimate_Error -> Update, Roe first order with Harten's fix, CFL 0.5, supersonic inlet.
 //
-//   sod_driver <grid.vtk> <U0.bin> <nsteps> <out.bin>
+//   sod_driver <grid.vtk> <U0.bin> <nsteps> <out.bin> [solution.txt error.txt]
 // U0.bin: No_Physical_Cells x 4 doubles. out.bin: final U (N x 4), then per step Error[0..3]
 // and Min_dt (nsteps x 5 doubles), then the first call's Cells_Net_Flux (N x 4) and
 // Cells_del_t (N) from Evaluate_Cell_Net_Flux_1O.
@@ -14,8 +14,8 @@
 #include "cfd_host.hpp"
 
 int main(int argc, char **argv) {
-    if (argc != 5) {
-        std::cerr << "usage: sod_driver grid.vtk U0.bin nsteps out.bin\n";
+    if (argc != 5 && argc != 7) {
+        std::cerr << "usage: sod_driver grid.vtk U0.bin nsteps out.bin [solution.txt error.txt]\n";
         return 2;
     }
     if (!Read_VTK_Grid(argv[1])) {
@@ -42,12 +42,20 @@ int main(int argc, char **argv) {
     Evaluate_Cell_Net_Flux_1O();
     std::vector<V_D> R0 = Cells_Net_Flux;
     V_D dt0 = Cells_del_t;
+    if (argc == 7) {  // the reference's files of the initial state
+        nx_c = 101;
+        ny_c = 11;
+        nz_c = 1;
+        iterations = 0;
+        Write_Solution(argv[5], 1);
+    }
     std::vector<double> hist;
     for (int it = 0; it < nsteps; it++) {
         Apply_Boundary_Conditions();
         Runge_Kutta_Method();
         Estimate_Error();
         Update();
+        if (argc == 7 && it < 3) Write_Error_File(argv[6]);
         for (int k = 0; k < 4; k++) hist.push_back(Error[k]);
         hist.push_back(Min_dt);
     }

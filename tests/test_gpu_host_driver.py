@@ -25,7 +25,9 @@ def test_host_driver_sod_vs_reference_golden(tmp_path):
     u0 = tmp_path / "U0.bin"
     np.ascontiguousarray(g["U0"], np.float64).tofile(u0)
     out = tmp_path / "out.bin"
-    r = subprocess.run([EXE, str(vtk), str(u0), "25", str(out)], capture_output=True, text=True, timeout=120)
+    sol, errf = tmp_path / "sol.txt", tmp_path / "err.txt"
+    r = subprocess.run([EXE, str(vtk), str(u0), "25", str(out), str(sol), str(errf)], capture_output=True,
+                       text=True, timeout=120)
     print(r.stdout, r.stderr, flush=True)
     assert r.returncode == 0
     n = len(m.cells)
@@ -40,3 +42,6 @@ def test_host_driver_sod_vs_reference_golden(tmp_path):
     # Evaluate_Cell_Net_Flux_1O of the initial state against the reference's net flux and del_t
     assert np.allclose(R0, g["R_roe"], rtol=0, atol=1e-13 * np.abs(g["R_roe"]).max())
     assert np.allclose(dt0, g["dt_roe"], rtol=1e-14, atol=0)
+    # Write_Solution / Write_Error_File byte for byte as the reference writes them
+    assert sol.read_text() == open(os.path.join(ROOT, "tests", "golden", "sod_solution_ref.txt")).read()
+    assert errf.read_text() == open(os.path.join(ROOT, "tests", "golden", "sod_error_ref.txt")).read()
