@@ -162,7 +162,7 @@ def batch_bench(args):
     Error[0] <= 1e-8 per instance, in one batch with an autotuned launch shape."""
     from cfd_solver_withcuda_amd import cases
     from cfd_solver_withcuda_amd.batch import Batch
-    from cfd_solver_withcuda_amd.solver import make_config
+    from cfd_solver_withcuda_amd.solver import cons_from_prim, make_config
 
     t = time.perf_counter()
     cs = cases.ramp_batch(args.instances, min_cells=args.min_cells, max_cells=args.max_cells)
@@ -172,9 +172,8 @@ def batch_bench(args):
     info = b.info()
     setup = time.perf_counter() - t
     ms = b.time_steps(args.steps)
-    b.set_state([np.tile(np.array([c.fs[0], c.fs[0] * c.fs[1], c.fs[0] * c.fs[2],
-                                   c.fs[3] / 0.4 + 0.5 * c.fs[0] * (c.fs[1] ** 2 + c.fs[2] ** 2)]),
-                         (len(c.mesh.cells), 1)) for c in cs])
+    b.set_state([np.tile(cons_from_prim(np.array([c.fs[0]]), c.fs[1], c.fs[2], c.fs[3]), (len(c.mesh.cells), 1))
+                 for c in cs])
     r = b.solve(tol=1e-8, max_steps=args.ws_max_steps, check_every=100)
     out = {
         "metric": "time to residual 1e-8, batched plain FP64 solves",

@@ -400,7 +400,7 @@ int batch_time_steps(cfd_batch_s *b, int n, double *ms) {
 
 int set_freestream(cfd_batch_s *b, const double *fs4) {
     std::vector<double4> U(b->nc);
-    const double gm1 = 0.4;
+    const double gm1 = 1.4 - 1.0;  // formed as the reference forms gamma - 1 (include/definitions.h:62-64)
     for (int i = 0; i < b->ninst; i++) {
         const double r = fs4[4 * i], u = fs4[4 * i + 1], v = fs4[4 * i + 2], p = fs4[4 * i + 3];
         const double4 q = make_double4(r, r * u, r * v, p / gm1 + 0.5 * r * (u * u + v * v));

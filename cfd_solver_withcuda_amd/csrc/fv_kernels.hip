@@ -756,15 +756,24 @@ __global__ void __launch_bounds__(256) roe2o_kernel(int nc, const RefCells rc, c
 // ---- batched meshes ------------------------------------------------------------------------
 // After a step of a batch of independent instances: each marching instance (status 1) sums its
 // tiles' error partials [inst_tile[i], inst_tile[i+1]) in a fixed order (Estimate_Error of that
-// instance), counts the step and stops when Error[0] <= tol (status 0) or the norm is not finite
+// instance), counts the step and stops when Error[0] <= tol (status 0) or Error[0] is not finite
 // (status -1). One workgroup of 256 threads per instance.
 __device__ __forceinline__ void freeze_step(double e0, int &fz, double &best, int &since,
                                             double freeze_tol, int freeze_stall) {
     // Limiter freezing: a step that recorded psi (1) switches to frozen (2); otherwise freezing
     // starts (record on the next step) once Error[0] <= freeze_tol, or once Error[0] has not
     // dropped below 0.9 x its best value for freeze_stall steps (a limiter-switching stall).
+    // A frozen march whose Error[0] exceeds 10 x the best value seen before freezing unfreezes.
     if (fz == 1) {
         fz = 2;
+    } else if (fz == 2) {
+        // the frozen factors no longer fit the state (it moved away from where they were
+        // recorded): return to the evaluated limiter and start the search again
+        if (!(e0 <= 10.0 * best)) {
+            fz = 0;
+            best = e0;
+            since = 0;
+        }
     } else if (fz == 0 && (freeze_tol > 0.0 || freeze_stall > 0)) {
         if (e0 < 0.9 * best) {
             best = e0;
@@ -814,7 +823,9 @@ __global__ void __launch_bounds__(256) batch_status_kernel(const int *__restrict
             err4[4 * i + q] = e[q];
         }
         steps[i] += 1;
-        if (!isfinite(e[0]) || !isfinite(e[1]) || !isfinite(e[2]) || !isfinite(e[3])) status[i] = -1;
+        // Error[1..3] are infinite wherever a momentum component is exactly zero (the reference's
+        // relative norm, src/Error_Estimate_Update.cpp:10-30); only Error[0] judges the march
+        if (!isfinite(e[0])) status[i] = -1;
         else if (e[0] <= tol) status[i] = 0;
         if (freeze) {
             int f = freeze[i], sn = since[i];

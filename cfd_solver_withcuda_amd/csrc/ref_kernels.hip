@@ -127,6 +127,57 @@ __global__ void ref_roe_flux_kernel(const double *U, const double *P, const int 
     }
 }
 
+// HLLC as the kernel text writes it (HLLC_LLF_Flux_Cuda_Kernels.cu:25-208): sqrt(rho)-weighted
+// wave-speed estimate, the star state through rho (S_K - u_n) / (S_K - S*) and E / rho. The
+// native path's hllc_flux is the same flux with shared reciprocals; this keeps the text's
+// rounding for the reference-layout entry point.
+__device__ __forceinline__ void ref_hllc_text(const FState &L, const FState &R, double nx, double ny,
+                                              double dl, double *F) {
+    double rho_L = L.W[0], u_L = L.W[1], v_L = L.W[2], P_L = L.W[3], a_L = L.W[4];
+    double rho_R = R.W[0], u_R = R.W[1], v_R = R.W[2], P_R = R.W[3], a_R = R.W[4];
+    double un_L = nx * u_L + ny * v_L;
+    double un_R = nx * u_R + ny * v_R;
+    double sqrt_rho_L = sqrt(rho_L);
+    double sqrt_rho_R = sqrt(rho_R);
+    double a_roe = (a_L * sqrt_rho_L + a_R * sqrt_rho_R) / (sqrt_rho_L + sqrt_rho_R);
+    double u_roe = (u_L * sqrt_rho_L + u_R * sqrt_rho_R) / (sqrt_rho_L + sqrt_rho_R);
+    double v_roe = (v_L * sqrt_rho_L + v_R * sqrt_rho_R) / (sqrt_rho_L + sqrt_rho_R);
+    double un_roe = nx * u_roe + ny * v_roe;
+    double S_L = fmin(un_L - a_L, un_roe - a_roe);
+    double S_R = fmax(un_R + a_R, un_roe + a_roe);
+    double numer = P_R - P_L + rho_L * un_L * (S_L - un_L) - rho_R * un_R * (S_R - un_R);
+    double denom = rho_L * (S_L - un_L) - rho_R * (S_R - un_R);
+    double S_star = (fabs(denom) > 1e-14) ? numer / denom : 0.5 * (un_L + un_R);
+    double f[4];
+    if (S_L >= 0.0) {
+        f[0] = rho_L * un_L;
+        f[1] = rho_L * u_L * un_L + P_L * nx;
+        f[2] = rho_L * v_L * un_L + P_L * ny;
+        f[3] = (L.U[3] + P_L) * un_L;
+    } else if (S_R <= 0.0) {
+        f[0] = rho_R * un_R;
+        f[1] = rho_R * u_R * un_R + P_R * nx;
+        f[2] = rho_R * v_R * un_R + P_R * ny;
+        f[3] = (R.U[3] + P_R) * un_R;
+    } else {
+        const bool left = (S_star >= 0.0);
+        const FState &K = left ? L : R;
+        double S_K = left ? S_L : S_R;
+        double rho = K.W[0], u = K.W[1], v = K.W[2], P = K.W[3], E = K.U[3];
+        double un = nx * u + ny * v;
+        double factor = rho * (S_K - un) / (S_K - S_star);
+        double Us0 = factor;
+        double Us1 = factor * (u + (S_star - un) * nx);
+        double Us2 = factor * (v + (S_star - un) * ny);
+        double Us3 = factor * (E / rho + (S_star - un) * (S_star + P / (rho * (S_K - un))));
+        f[0] = rho * un + S_K * (Us0 - K.U[0]);
+        f[1] = (rho * u * un + P * nx) + S_K * (Us1 - K.U[1]);
+        f[2] = (rho * v * un + P * ny) + S_K * (Us2 - K.U[2]);
+        f[3] = ((E + P) * un) + S_K * (Us3 - K.U[3]);
+    }
+    for (int q = 0; q < 4; q++) F[q] = f[q] * dl;
+}
+
 __global__ void ref_hllc_flux_kernel(const double *U, const double *P, const int *nb,
                                      const double *normals, const double *areas, double *out,
                                      int N, double gamma) {
@@ -147,7 +198,7 @@ __global__ void ref_hllc_flux_kernel(const double *U, const double *P, const int
             L.U[q] = U[i * 4 + q];
             R.U[q] = U[j * 4 + q];
         }
-        hllc_flux(L, R, nx, ny, area, &out[(i * 4 + f) * 4]);
+        ref_hllc_text(L, R, nx, ny, area, &out[(i * 4 + f) * 4]);
     }
 }
 

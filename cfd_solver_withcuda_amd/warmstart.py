@@ -22,7 +22,7 @@ from .solver import cons_from_prim, make_config
 from .surrogate import MLP, features, predicted_state, prim_from_cons
 
 SCHEME = dict(flux="hllc", order=2, limiter="vanleer", gradient="wlsq", cfl=0.8, rk=3, lts=1,
-              roe_bdiss=1, bc=cases.BUMP_BCS)
+              roe_bdiss=1, bc=cases.BUMP_BCS, freeze_stall=300)
 
 
 def freestream_states(cs):
@@ -53,8 +53,10 @@ def run(n_train=64, n_eval=32, train_cells=30000, eval_cells=60000, tol=1e-8, tr
     tr = cases.bump_family(n_train, seed=seed, cells=train_cells, stream=200)
     b_tr, rt = solve_batch(tr, train_tol, max_steps, check_every)
     states = b_tr.get_state()
-    ok = rt["status"] == 0
-    out["train"] = dict(cases=n_train, cells=int(sum(len(c.mesh.cells) for c in tr)), converged=int(ok.sum()),
+    # converged solutions, and the finite end states of the ones still marching at max_steps
+    ok = rt["status"] >= 0
+    out["train"] = dict(cases=n_train, cells=int(sum(len(c.mesh.cells) for c in tr)),
+                        converged=int((rt["status"] == 0).sum()), used=int(ok.sum()),
                         steps_max=int(rt["steps"].max()), wall_s=rt["wall_s"])
     log(f"training solves: {out['train']}")
     del b_tr
