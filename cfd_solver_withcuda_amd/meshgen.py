@@ -32,7 +32,7 @@ def splitmix64(x: np.ndarray) -> np.ndarray:
 def uniform01(seed: int, stream: int, idx: np.ndarray) -> np.ndarray:
     """Deterministic U[0,1) doubles for indices `idx` in stream `stream` of `seed`."""
     with np.errstate(over="ignore"):
-        base = splitmix64(np.array([seed * 0x100000001B3 + stream], dtype=np.uint64))[0]
+        base = splitmix64(np.array([(seed * 0x100000001B3 + stream) & 0xFFFFFFFFFFFFFFFF], dtype=np.uint64))[0]
         h = splitmix64(idx.astype(np.uint64) ^ base)
     return (h >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
 
