@@ -89,6 +89,25 @@ class Batch:
         return dict(converged=rc == 0, steps=steps, err=err, status=status, conv_s=conv,
                     wall_s=float(wall.value))
 
+    def set_freeze(self, mode=None):
+        """Per-instance limiter-freezing mode of later solves: 0 evaluated limiter until the
+        freezing trigger, 1 record on the first step and march frozen (None: all 0)."""
+        if mode is None:
+            check(lib().cfd_batch_set_freeze(self.h, None), "cfd_batch_set_freeze")
+            return
+        m = np.ascontiguousarray(mode, np.int32)
+        if m.shape != (self.ninst,):
+            raise ValueError(f"mode must have {self.ninst} entries")
+        check(lib().cfd_batch_set_freeze(self.h, iptr(m)), "cfd_batch_set_freeze")
+
+    def history(self) -> np.ndarray:
+        """Error[0] per check of the last solve, [checks, ninst]."""
+        n = ctypes.c_int()
+        check(lib().cfd_batch_history(self.h, ctypes.byref(n), None), "cfd_batch_history")
+        h = np.zeros((n.value, self.ninst))
+        check(lib().cfd_batch_history(self.h, ctypes.byref(n), dptr(h)), "cfd_batch_history")
+        return h
+
     def time_steps(self, nsteps: int) -> float:
         ms = ctypes.c_float()
         check(lib().cfd_batch_time_steps(self.h, nsteps, ctypes.byref(ms)), "cfd_batch_time_steps")

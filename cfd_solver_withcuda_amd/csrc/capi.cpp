@@ -57,9 +57,9 @@ struct cfd_mesh_s {
     RefMesh ref;
     Layout lay;
     bool on_device = false;
-    int *d_tile_cstart = nullptr, *d_tile_fstart = nullptr, *d_fL = nullptr, *d_fR = nullptr;
-    double *d_fnx = nullptr, *d_fny = nullptr, *d_fdl = nullptr, *d_fdx = nullptr, *d_fdy = nullptr;
-    double *d_fwL = nullptr, *d_fwR = nullptr;
+    int *d_tile_cstart = nullptr, *d_tile_fstart = nullptr;
+    double *d_fgeo = nullptr, *d_fw = nullptr, *d_fnx = nullptr, *d_fny = nullptr, *d_fdl = nullptr;
+    int *d_fLR = nullptr;
     uint16_t *d_slots = nullptr;
     double *d_area = nullptr, *d_inv_area = nullptr;
     int *d_gcell = nullptr, *d_gkind = nullptr;  // ghost k: renumbered owner cell, face kind
@@ -70,7 +70,7 @@ struct cfd_mesh_s {
     double *d_rc_dist = nullptr, *d_rc_nrm = nullptr, *d_rc_len = nullptr, *d_rc_area = nullptr;
     DevMesh dev{};
     ~cfd_mesh_s() {
-        void *ps[] = {d_tile_cstart, d_tile_fstart, d_fL, d_fR, d_fnx, d_fny, d_fdl, d_fdx, d_fdy, d_fwL, d_fwR, d_slots, d_area, d_inv_area, d_gcell, d_gkind, d_gnrm,
+        void *ps[] = {d_tile_cstart, d_tile_fstart, d_fgeo, d_fw, d_fnx, d_fny, d_fdl, d_fLR, d_slots, d_area, d_inv_area, d_gcell, d_gkind, d_gnrm,
                       d_rc_nf, d_rc_nbr, d_rc_dist, d_rc_nrm, d_rc_len, d_rc_area};
         for (void *p : ps)
             if (p) (void)hipFree(p);
@@ -151,13 +151,15 @@ int cfd_mesh_create(int npts, const double *pts3, int ncells, const int *cells4,
     *out = nullptr;
     const Layout &L = m->lay;
     hipError_t e = hipSuccess;
+    std::vector<double> geo, w;
+    std::vector<int> lr;
+    pack_face_records(L, 0, geo, w, lr);
     if ((e = upload(&m->d_tile_cstart, L.tile_cstart)) != hipSuccess ||
         (e = upload(&m->d_tile_fstart, L.tile_fstart)) != hipSuccess ||
-        (e = upload(&m->d_fL, L.fL)) != hipSuccess || (e = upload(&m->d_fR, L.fR)) != hipSuccess ||
-        (e = upload(&m->d_fnx, L.fnx)) != hipSuccess || (e = upload(&m->d_fny, L.fny)) != hipSuccess ||
-        (e = upload(&m->d_fdl, L.fdl)) != hipSuccess || (e = upload(&m->d_fdx, L.fdx)) != hipSuccess ||
-        (e = upload(&m->d_fdy, L.fdy)) != hipSuccess || (e = upload(&m->d_fwL, L.fwL)) != hipSuccess ||
-        (e = upload(&m->d_fwR, L.fwR)) != hipSuccess || (e = upload(&m->d_slots, L.slots)) != hipSuccess ||
+        (e = upload(&m->d_fgeo, geo)) != hipSuccess || (e = upload(&m->d_fw, w)) != hipSuccess ||
+        (e = upload(&m->d_fLR, lr)) != hipSuccess || (e = upload(&m->d_fnx, L.fnx)) != hipSuccess ||
+        (e = upload(&m->d_fny, L.fny)) != hipSuccess || (e = upload(&m->d_fdl, L.fdl)) != hipSuccess ||
+        (e = upload(&m->d_slots, L.slots)) != hipSuccess ||
         (e = upload(&m->d_area, L.area)) != hipSuccess || (e = upload(&m->d_inv_area, L.inv_area)) != hipSuccess) {
         delete m;
         return hip_fail(e, "cfd_mesh_create upload");
@@ -187,15 +189,12 @@ int cfd_mesh_create(int npts, const double *pts3, int ncells, const int *cells4,
     d.nfe = (int)L.fL.size();
     d.tile_cstart = m->d_tile_cstart;
     d.tile_fstart = m->d_tile_fstart;
-    d.fL = m->d_fL;
-    d.fR = m->d_fR;
+    d.fgeo = reinterpret_cast<const double2 *>(m->d_fgeo);
+    d.fw = reinterpret_cast<const double2 *>(m->d_fw);
+    d.fLR = reinterpret_cast<const int2 *>(m->d_fLR);
     d.fnx = m->d_fnx;
     d.fny = m->d_fny;
     d.fdl = m->d_fdl;
-    d.fdx = m->d_fdx;
-    d.fdy = m->d_fdy;
-    d.fwL = reinterpret_cast<const double2 *>(m->d_fwL);
-    d.fwR = reinterpret_cast<const double2 *>(m->d_fwR);
     d.slots = m->d_slots;
     d.area = m->d_area;
     d.inv_area = m->d_inv_area;

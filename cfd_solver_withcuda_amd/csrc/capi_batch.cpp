@@ -67,9 +67,10 @@ struct cfd_batch_s {
     std::vector<int> inst_tile;  // [ninst+1]
     std::vector<int> iperm;      // caller cell (global) -> renumbered cell (global)
     // device mesh
-    int *d_tile_cstart = nullptr, *d_tile_fstart = nullptr, *d_fL = nullptr, *d_fR = nullptr;
-    double *d_fnx = nullptr, *d_fny = nullptr, *d_fdl = nullptr, *d_fdx = nullptr, *d_fdy = nullptr;
-    double *d_fwL = nullptr, *d_fwR = nullptr, *d_area = nullptr, *d_inv_area = nullptr;
+    int *d_tile_cstart = nullptr, *d_tile_fstart = nullptr;
+    double *d_fgeo = nullptr, *d_fw = nullptr, *d_area = nullptr, *d_inv_area = nullptr;
+    double *d_fnx = nullptr, *d_fny = nullptr, *d_fdl = nullptr;
+    int *d_fLR = nullptr;
     uint16_t *d_slots = nullptr;
     int *d_tile_inst = nullptr, *d_inst_tile = nullptr, *d_tile_list = nullptr, *d_tile_count = nullptr;
     int *d_inst_off = nullptr, *d_status = nullptr, *d_steps = nullptr;
@@ -85,13 +86,15 @@ struct cfd_batch_s {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     double graph_tol = -1.0;
+    std::vector<int> fz_start;   // per instance freeze mode each solve starts with (empty: 0)
+    std::vector<double> hist;    // Error[0] per instance at each check of the last solve
+    int hist_checks = 0;
     double build_s = 0.0, tune_s = 0.0;
     ~cfd_batch_s() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         if (stream) (void)hipStreamDestroy(stream);
-        void *ps[] = {d_tile_cstart, d_tile_fstart, d_fL, d_fR, d_fnx, d_fny, d_fdl, d_fdx, d_fdy,
-                      d_fwL, d_fwR, d_area, d_inv_area, d_slots, d_tile_inst, d_inst_tile,
+        void *ps[] = {d_tile_cstart, d_tile_fstart, d_fgeo, d_fw, d_fnx, d_fny, d_fdl, d_fLR, d_area, d_inv_area, d_slots, d_tile_inst, d_inst_tile,
                       d_tile_list, d_tile_count, d_inst_off, d_status, d_steps, d_inst_fs, d_err4,
                       Un, Ua, Ub, acc, grad, hbuf, err_partial, d_fz, d_fz_since, d_fz_best, psi};
         for (void *p : ps)
@@ -182,15 +185,12 @@ int build_batch(const std::vector<InstanceInput> &in, const double *fs4, const c
     }
     BALLOC(b->d_tile_cstart, nt + 1);
     BALLOC(b->d_tile_fstart, nt + 1);
-    BALLOC(b->d_fL, nfe);
-    BALLOC(b->d_fR, nfe);
+    BALLOC(b->d_fgeo, 6 * nfe);
+    BALLOC(b->d_fw, 4 * nfe);
+    BALLOC(b->d_fLR, 2 * nfe);
     BALLOC(b->d_fnx, nfe);
     BALLOC(b->d_fny, nfe);
     BALLOC(b->d_fdl, nfe);
-    BALLOC(b->d_fdx, nfe);
-    BALLOC(b->d_fdy, nfe);
-    BALLOC(b->d_fwL, 2 * nfe);
-    BALLOC(b->d_fwR, 2 * nfe);
     BALLOC(b->d_slots, 4 * nc);
     BALLOC(b->d_area, nc);
     BALLOC(b->d_inv_area, nc);
@@ -222,28 +222,24 @@ int build_batch(const std::vector<InstanceInput> &in, const double *fs4, const c
         Layout &L = lays[i];
         const int cb = b->cell_off[i], tb = b->inst_tile[i], fb = fe_off[i];
         const size_t nci = L.area.size(), nti = L.ntiles, nfi = L.fL.size();
-        std::vector<int> tc(nti), tf(nti), fl(nfi), fr(nfi), ti(nti, i);
+        std::vector<int> tc(nti), tf(nti), ti(nti, i);
         for (size_t t = 0; t < nti; t++) {
             tc[t] = L.tile_cstart[t] + cb;
             tf[t] = L.tile_fstart[t] + fb;
         }
-        for (size_t j = 0; j < nfi; j++) {
-            fl[j] = L.fL[j] + cb;
-            fr[j] = L.fR[j] >= 0 ? L.fR[j] + cb : L.fR[j];
-        }
+        std::vector<double> geo, w;
+        std::vector<int> lr;
+        pack_face_records(L, cb, geo, w, lr);
         for (size_t c = 0; c < nci; c++) b->iperm[cb + c] = cb + L.iperm[c];
         if ((e = h2d(b->d_tile_cstart + tb, tc.data(), nti)) != hipSuccess ||
             (e = h2d(b->d_tile_fstart + tb, tf.data(), nti)) != hipSuccess ||
             (e = h2d(b->d_tile_inst + tb, ti.data(), nti)) != hipSuccess ||
-            (e = h2d(b->d_fL + fb, fl.data(), nfi)) != hipSuccess ||
-            (e = h2d(b->d_fR + fb, fr.data(), nfi)) != hipSuccess ||
+            (e = h2d(b->d_fgeo + 6 * (size_t)fb, geo.data(), 6 * nfi)) != hipSuccess ||
+            (e = h2d(b->d_fw + 4 * (size_t)fb, w.data(), 4 * nfi)) != hipSuccess ||
+            (e = h2d(b->d_fLR + 2 * (size_t)fb, lr.data(), 2 * nfi)) != hipSuccess ||
             (e = h2d(b->d_fnx + fb, L.fnx.data(), nfi)) != hipSuccess ||
             (e = h2d(b->d_fny + fb, L.fny.data(), nfi)) != hipSuccess ||
             (e = h2d(b->d_fdl + fb, L.fdl.data(), nfi)) != hipSuccess ||
-            (e = h2d(b->d_fdx + fb, L.fdx.data(), nfi)) != hipSuccess ||
-            (e = h2d(b->d_fdy + fb, L.fdy.data(), nfi)) != hipSuccess ||
-            (e = h2d(b->d_fwL + 2 * (size_t)fb, L.fwL.data(), 2 * nfi)) != hipSuccess ||
-            (e = h2d(b->d_fwR + 2 * (size_t)fb, L.fwR.data(), 2 * nfi)) != hipSuccess ||
             (e = h2d(b->d_slots + 4 * (size_t)cb, L.slots.data(), 4 * nci)) != hipSuccess ||
             (e = h2d(b->d_area + cb, L.area.data(), nci)) != hipSuccess ||
             (e = h2d(b->d_inv_area + cb, L.inv_area.data(), nci)) != hipSuccess) {
@@ -271,15 +267,12 @@ int build_batch(const std::vector<InstanceInput> &in, const double *fs4, const c
     d.nfe = b->nfe;
     d.tile_cstart = b->d_tile_cstart;
     d.tile_fstart = b->d_tile_fstart;
-    d.fL = b->d_fL;
-    d.fR = b->d_fR;
+    d.fgeo = reinterpret_cast<const double2 *>(b->d_fgeo);
+    d.fw = reinterpret_cast<const double2 *>(b->d_fw);
+    d.fLR = reinterpret_cast<const int2 *>(b->d_fLR);
     d.fnx = b->d_fnx;
     d.fny = b->d_fny;
     d.fdl = b->d_fdl;
-    d.fdx = b->d_fdx;
-    d.fdy = b->d_fdy;
-    d.fwL = reinterpret_cast<const double2 *>(b->d_fwL);
-    d.fwR = reinterpret_cast<const double2 *>(b->d_fwR);
     d.slots = b->d_slots;
     d.area = b->d_area;
     d.inv_area = b->d_inv_area;
@@ -309,7 +302,7 @@ int batch_reset(cfd_batch_s *b) {
     BHIP(h2d(b->d_steps, zeros.data(), zeros.size()));
     if (b->d_fz) {
         std::vector<double> big(b->ninst, 1.7976931348623157e308);
-        BHIP(h2d(b->d_fz, zeros.data(), zeros.size()));
+        BHIP(h2d(b->d_fz, b->fz_start.empty() ? zeros.data() : b->fz_start.data(), zeros.size()));
         BHIP(h2d(b->d_fz_since, zeros.data(), zeros.size()));
         BHIP(h2d(b->d_fz_best, big.data(), big.size()));
     }
@@ -545,12 +538,18 @@ int cfd_batch_solve(cfd_batch_t *b, double tol, int max_steps, int check_every, 
     const double t0 = now_s();
     int done = 0;
     bool all = false;
+    std::vector<double> e4(4 * (size_t)ni);
+    b->hist.clear();
+    b->hist_checks = 0;
     while (done < max_steps && !all) {
         const int k = std::min(check_every, max_steps - done);
         for (int j = 0; j < k; j++) BHIP(hipGraphLaunch(b->exec, b->stream));
         done += k;
         BHIP(hipMemcpyAsync(st.data(), b->d_status, ni * sizeof(int), hipMemcpyDeviceToHost, b->stream));
+        BHIP(hipMemcpyAsync(e4.data(), b->d_err4, e4.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream));
         BHIP(hipStreamSynchronize(b->stream));
+        for (int i = 0; i < ni; i++) b->hist.push_back(e4[4 * (size_t)i]);
+        b->hist_checks++;
         const double t = now_s() - t0;
         all = true;
         for (int i = 0; i < ni; i++) {
@@ -566,6 +565,26 @@ int cfd_batch_solve(cfd_batch_t *b, double tol, int max_steps, int check_every, 
     if (wall_s) *wall_s = wall;
     for (int i = 0; i < ni; i++)
         if (st[i] != 0) return CFD_E_NOTCONV;
+    return 0;
+}
+
+int cfd_batch_set_freeze(cfd_batch_t *b, const int *mode) {
+    if (!b) return capi_fail(CFD_E_ARG, "cfd_batch_set_freeze: null batch");
+    if (!b->d_fz) return capi_fail(CFD_E_ARG, "cfd_batch_set_freeze: limiter freezing is off (freeze_tol, freeze_stall)");
+    if (!mode) {
+        b->fz_start.clear();
+        return 0;
+    }
+    for (int i = 0; i < b->ninst; i++)
+        if (mode[i] != 0 && mode[i] != 1) return capi_fail(CFD_E_ARG, "cfd_batch_set_freeze: mode must be 0 or 1");
+    b->fz_start.assign(mode, mode + b->ninst);
+    return 0;
+}
+
+int cfd_batch_history(const cfd_batch_t *b, int *nchecks, double *e0) {
+    if (!b || !nchecks) return capi_fail(CFD_E_ARG, "cfd_batch_history: null argument");
+    *nchecks = b->hist_checks;
+    if (e0) std::memcpy(e0, b->hist.data(), b->hist.size() * sizeof(double));
     return 0;
 }
 

@@ -14,6 +14,7 @@ namespace cfd {
 constexpr double kGamma = 1.4;
 constexpr double kGm1 = (kGamma - 1.0);
 constexpr double kGamma1 = (kGamma / kGm1);  // include/definitions.h:65 (gamma / gamma_M_1)
+constexpr double kInvGm1 = 1.0 / kGm1;
 
 // Face state: primitives (rho, u, v, p, a) and conserved (rho, rho u, rho v, rho E).
 struct FState {
@@ -386,7 +387,12 @@ __device__ __forceinline__ void reconstruct_side(const FState &c, const FState &
     out.W[2] = Wf[2];
     out.W[3] = Wf[3];
     out.W[4] = sqrt(kGamma * Wf[3] * (1.0 / Wf[0]));
-    cons_from_prim(Wf[0], Wf[1], Wf[2], Wf[3], out.U);
+    // conserved values of the face state, with p / (gamma - 1) as a product by the reciprocal
+    const double vmag = 0.5 * (Wf[1] * Wf[1] + Wf[2] * Wf[2]);
+    out.U[0] = Wf[0];
+    out.U[1] = Wf[0] * Wf[1];
+    out.U[2] = Wf[0] * Wf[2];
+    out.U[3] = Wf[3] * kInvGm1 + Wf[0] * vmag;
 }
 
 }  // namespace cfd

@@ -114,19 +114,46 @@ struct FaceRec {
     double nx, ny, dl, dx, dy;
 };
 
+// Face entry e's record: three 16-byte loads ({L, R, nx}, {ny, dl}, {dx, dy}); the first is
+// always loaded (it holds the cells), the others when asked for.
 template <bool GEOM, bool CENTRE>
 __device__ __forceinline__ void load_face(const DevMesh &m, int e, FaceRec &f) {
-    f.L = m.fL[e];
-    f.R = m.fR[e];
+    const int4 a = reinterpret_cast<const int4 *>(m.fgeo)[3 * (size_t)e];
+    f.L = a.x;
+    f.R = a.y;
+    f.nx = __hiloint2double(a.w, a.z);
     if constexpr (GEOM) {
+        const double2 b = m.fgeo[3 * (size_t)e + 1];
+        f.ny = b.x;
+        f.dl = b.y;
+    }
+    if constexpr (CENTRE) {
+        const double2 c = m.fgeo[3 * (size_t)e + 2];
+        f.dx = c.x;
+        f.dy = c.y;
+    }
+}
+
+// The gradient pass's face entry: cells, and for Green-Gauss the normal and length.
+template <bool GG>
+__device__ __forceinline__ void load_face_grad(const DevMesh &m, int e, FaceRec &f) {
+    const int2 lr = m.fLR[e];
+    f.L = lr.x;
+    f.R = lr.y;
+    if constexpr (GG) {
         f.nx = m.fnx[e];
         f.ny = m.fny[e];
         f.dl = m.fdl[e];
     }
-    if constexpr (CENTRE) {
-        f.dx = m.fdx[e];
-        f.dy = m.fdy[e];
-    }
+}
+
+// Unit normal and length of face entry e.
+__device__ __forceinline__ void face_ndl(const DevMesh &m, int e, double &nx, double &ny, double &dl) {
+    const int4 a = reinterpret_cast<const int4 *>(m.fgeo)[3 * (size_t)e];
+    const double2 b = m.fgeo[3 * (size_t)e + 1];
+    nx = __hiloint2double(a.w, a.z);
+    ny = b.x;
+    dl = b.y;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -219,17 +246,17 @@ __global__ void __launch_bounds__(256) grad_tile_kernel(DevMesh m, DevCfg cfg,
     double2 w0l{}, w0r{}, w1l{}, w1r{};
     const bool h0 = k < nft, h1 = k + B < nft;
     if (h0) {
-        load_face<GG, false>(m, fs + k, f0);
+        load_face_grad<GG>(m, fs + k, f0);
         if constexpr (!GG) {
-            w0l = m.fwL[fs + k];
-            if (f0.R >= 0) w0r = m.fwR[fs + k];
+            w0l = m.fw[2 * (size_t)(fs + k)];
+            if (f0.R >= 0) w0r = m.fw[2 * (size_t)(fs + k) + 1];
         }
     }
     if (h1) {
-        load_face<GG, false>(m, fs + k + B, f1);
+        load_face_grad<GG>(m, fs + k + B, f1);
         if constexpr (!GG) {
-            w1l = m.fwL[fs + k + B];
-            if (f1.R >= 0) w1r = m.fwR[fs + k + B];
+            w1l = m.fw[2 * (size_t)(fs + k + B)];
+            if (f1.R >= 0) w1r = m.fw[2 * (size_t)(fs + k + B) + 1];
         }
     }
     // ---- A ------------------------------------------------------------------------------
@@ -244,11 +271,11 @@ __global__ void __launch_bounds__(256) grad_tile_kernel(DevMesh m, DevCfg cfg,
     if (h1) grad_face<GRAD>(m, cfg, Us, sW, sX, T, FT, c0, nct, fs + k + B, k + B, f1, w1l, w1r, fsv);
     for (int j = k + 2 * B; j < nft; j += B) {
         FaceRec f;
-        load_face<GG, false>(m, fs + j, f);
+        load_face_grad<GG>(m, fs + j, f);
         double2 wl{}, wr{};
         if constexpr (!GG) {
-            wl = m.fwL[fs + j];
-            if (f.R >= 0) wr = m.fwR[fs + j];
+            wl = m.fw[2 * (size_t)(fs + j)];
+            if (f.R >= 0) wr = m.fw[2 * (size_t)(fs + j) + 1];
         }
         grad_face<GRAD>(m, cfg, Us, sW, sX, T, FT, c0, nct, fs + j, j, f, wl, wr, fsv);
     }
@@ -483,7 +510,9 @@ __global__ void __launch_bounds__(256) CFD_FLUX_ATTR flux_tile_kernel(DevMesh m,
                 const uint16_t sl = sls[f];
                 if (sl == 0xFFFF) break;
                 const int e = fs + (sl >> 1);
-                denom += (fabs(w1 * m.fnx[e] + w2 * m.fny[e]) + w4) * m.fdl[e];
+                double nx, ny, dl;
+                face_ndl(m, e, nx, ny, dl);
+                denom += (fabs(w1 * nx + w2 * ny) + w4) * dl;
             }
         }
         // Inviscid_Time_Step (src/Time_Step.cpp:293-329)
@@ -542,7 +571,9 @@ __global__ void __launch_bounds__(256) CFD_FLUX_ATTR flux_tile_kernel(DevMesh m,
                         const uint16_t sl = sls[f];
                         if (sl == 0xFFFF) break;
                         const int e = fs + (sl >> 1);
-                        den += (fabs(s.W[1] * m.fnx[e] + s.W[2] * m.fny[e]) + s.W[4]) * m.fdl[e];
+                        double nx, ny, dl;
+                        face_ndl(m, e, nx, ny, dl);
+                        den += (fabs(s.W[1] * nx + s.W[2] * ny) + s.W[4]) * dl;
                     }
                     const double dtn = (den <= 0.0 || !isfinite(den)) ? 0.0 : cfg.cfl * ar / den;
                     dtmin = fmin(dtmin, dtn);
@@ -763,9 +794,11 @@ __device__ __forceinline__ void freeze_step(double e0, int &fz, double &best, in
     // Limiter freezing: a step that recorded psi (1) switches to frozen (2); otherwise freezing
     // starts (record on the next step) once Error[0] <= freeze_tol, or once Error[0] has not
     // dropped below 0.9 x its best value for freeze_stall steps (a limiter-switching stall).
-    // A frozen march whose Error[0] exceeds 10 x the best value seen before freezing unfreezes.
+    // A frozen march whose Error[0] exceeds 10 x the best value seen up to the recording step
+    // unfreezes.
     if (fz == 1) {
         fz = 2;
+        best = fmin(best, e0);
     } else if (fz == 2) {
         // the frozen factors no longer fit the state (it moved away from where they were
         // recorded): return to the evaluated limiter and start the search again

@@ -351,46 +351,90 @@ void build_layout(const RefMesh &m, int tile, int renumber, Layout &L, int face_
     L.ntiles = (int)L.tile_cstart.size() - 1;
     L.tile_fstart.assign(L.ntiles + 1, 0);
     L.slots.assign(4 * (size_t)nc, 0xFFFF);
+    std::vector<int> ids, grp(L.nfaces, 0);
     for (int t = 0; t < L.ntiles; t++) {
         L.tile_fstart[t] = (int)L.fL.size();
         const int k0 = L.tile_cstart[t], k1 = L.tile_cstart[t + 1];
-        int nloc = 0;
+        // the tile's faces in first-touch order, then grouped: faces between two cells of the
+        // tile, faces to a cell of another tile (halo), boundary faces. Lanes of one wave then
+        // mostly take the same branches in the flux and gradient passes.
+        ids.clear();
         for (int k = k0; k < k1; k++) {
             const int c = L.perm[k];
             for (int f = 0; f < m.nf[c]; f++) {
                 const int id = face_of[4 * (size_t)c + f];
-                if (stamp[id] != t) {
-                    stamp[id] = t;
-                    local[id] = nloc++;
-                    const int a = canon_cell[id], s = canon_slot[id];
-                    const int j = m.nbr[4 * (size_t)a + s];
-                    L.fL.push_back(L.iperm[a]);
-                    L.fR.push_back(j < nc ? L.iperm[j] : -m.kind[4 * (size_t)a + s]);
-                    L.fnx.push_back(m.nrm[8 * (size_t)a + 2 * s]);
-                    L.fny.push_back(m.nrm[8 * (size_t)a + 2 * s + 1]);
-                    L.fdl.push_back(m.len[4 * (size_t)a + s]);
-                    L.fdx.push_back(m.ctr[3 * (size_t)j] - m.ctr[3 * (size_t)a]);
-                    L.fdy.push_back(m.ctr[3 * (size_t)j + 1] - m.ctr[3 * (size_t)a + 1]);
-                    L.fwL.push_back(CX[4 * (size_t)a + s]);
-                    L.fwL.push_back(CY[4 * (size_t)a + s]);
-                    double rx = 0.0, ry = 0.0;
-                    if (j < nc)
-                        for (int q = 0; q < m.nf[j]; q++)
-                            if (m.nbr[4 * (size_t)j + q] == a) {
-                                rx = CX[4 * (size_t)j + q];
-                                ry = CY[4 * (size_t)j + q];
-                            }
-                    L.fwR.push_back(rx);
-                    L.fwR.push_back(ry);
+                if (stamp[id] == t) continue;
+                stamp[id] = t;
+                const int a = canon_cell[id], j = m.nbr[4 * (size_t)a + canon_slot[id]];
+                if (j >= nc) {
+                    grp[id] = 2;
+                } else {
+                    const int kl = L.iperm[a], kr = L.iperm[j];
+                    grp[id] = (kl >= k0 && kl < k1 && kr >= k0 && kr < k1) ? 0 : 1;
                 }
+                ids.push_back(id);
+            }
+        }
+        std::stable_sort(ids.begin(), ids.end(), [&](int p, int q) { return grp[p] < grp[q]; });
+        const int nloc = (int)ids.size();
+        for (int i = 0; i < nloc; i++) {
+            const int id = ids[i];
+            local[id] = i;
+            const int a = canon_cell[id], s = canon_slot[id];
+            const int j = m.nbr[4 * (size_t)a + s];
+            L.fL.push_back(L.iperm[a]);
+            L.fR.push_back(j < nc ? L.iperm[j] : -m.kind[4 * (size_t)a + s]);
+            L.fnx.push_back(m.nrm[8 * (size_t)a + 2 * s]);
+            L.fny.push_back(m.nrm[8 * (size_t)a + 2 * s + 1]);
+            L.fdl.push_back(m.len[4 * (size_t)a + s]);
+            L.fdx.push_back(m.ctr[3 * (size_t)j] - m.ctr[3 * (size_t)a]);
+            L.fdy.push_back(m.ctr[3 * (size_t)j + 1] - m.ctr[3 * (size_t)a + 1]);
+            L.fwL.push_back(CX[4 * (size_t)a + s]);
+            L.fwL.push_back(CY[4 * (size_t)a + s]);
+            double rx = 0.0, ry = 0.0;
+            if (j < nc)
+                for (int q = 0; q < m.nf[j]; q++)
+                    if (m.nbr[4 * (size_t)j + q] == a) {
+                        rx = CX[4 * (size_t)j + q];
+                        ry = CY[4 * (size_t)j + q];
+                    }
+            L.fwR.push_back(rx);
+            L.fwR.push_back(ry);
+        }
+        if (nloc > 32767) throw std::runtime_error("too many faces in one tile");
+        for (int k = k0; k < k1; k++) {
+            const int c = L.perm[k];
+            for (int f = 0; f < m.nf[c]; f++) {
+                const int id = face_of[4 * (size_t)c + f];
                 const bool isR = canon_cell[id] != c;
-                if (local[id] >= 32767) throw std::runtime_error("too many faces in one tile");
                 L.slots[4 * (size_t)k + f] = (uint16_t)((local[id] << 1) | (isR ? 1 : 0));
             }
         }
         L.max_tile_faces = std::max(L.max_tile_faces, nloc);
     }
     L.tile_fstart[L.ntiles] = (int)L.fL.size();
+}
+
+void pack_face_records(const Layout &L, int cell_off, std::vector<double> &geo, std::vector<double> &w,
+                       std::vector<int> &lr) {
+    const size_t n = L.fL.size();
+    geo.resize(6 * n);
+    w.resize(4 * n);
+    lr.resize(2 * n);
+    for (size_t e = 0; e < n; e++) {
+        lr[2 * e] = L.fL[e] + cell_off;
+        lr[2 * e + 1] = L.fR[e] >= 0 ? L.fR[e] + cell_off : L.fR[e];
+        std::memcpy(&geo[6 * e], &lr[2 * e], 2 * sizeof(int));
+        geo[6 * e + 1] = L.fnx[e];
+        geo[6 * e + 2] = L.fny[e];
+        geo[6 * e + 3] = L.fdl[e];
+        geo[6 * e + 4] = L.fdx[e];
+        geo[6 * e + 5] = L.fdy[e];
+        w[4 * e] = L.fwL[2 * e];
+        w[4 * e + 1] = L.fwL[2 * e + 1];
+        w[4 * e + 2] = L.fwR[2 * e];
+        w[4 * e + 3] = L.fwR[2 * e + 1];
+    }
 }
 
 }  // namespace cfd

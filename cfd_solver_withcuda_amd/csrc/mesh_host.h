@@ -46,4 +46,10 @@ struct Layout {
 // renumber: 0 = keep the input order, 1 = Hilbert order of the cell centres.
 void build_layout(const RefMesh &m, int tile, int renumber, Layout &L, int face_budget = 0);
 
+// The device records of the layout's face entries, with cell ids shifted by `cell_off` (boundary
+// markers kept): geo gets 6 doubles per entry ({L, R} as two int32 in the first, then nx, ny, dl,
+// dx, dy), w gets 4 (the least-squares vectors of L, then of R), lr gets {L, R}.
+void pack_face_records(const Layout &L, int cell_off, std::vector<double> &geo, std::vector<double> &w,
+                       std::vector<int> &lr);
+
 }  // namespace cfd

@@ -30,8 +30,15 @@ def freestream_states(cs):
             for c in cs]
 
 
-def solve_batch(cs, tol, max_steps, check_every, block=128, start=None):
-    b = Batch([c.mesh for c in cs], np.array([c.fs for c in cs]), make_config(block=block, **SCHEME))
+def scheme(cfl=None):
+    kw = dict(SCHEME)
+    if cfl is not None:
+        kw["cfl"] = cfl
+    return kw
+
+
+def solve_batch(cs, tol, max_steps, check_every, block=128, start=None, cfl=None):
+    b = Batch([c.mesh for c in cs], np.array([c.fs for c in cs]), make_config(block=block, **scheme(cfl)))
     if start is not None:
         b.set_state(start)
     r = b.solve(tol=tol, max_steps=max_steps, check_every=check_every)
@@ -47,11 +54,12 @@ def training_set(cs, states):
 
 
 def run(n_train=64, n_eval=32, train_cells=30000, eval_cells=60000, tol=1e-8, train_tol=1e-8,
-        max_steps=60000, check_every=100, mlp_steps=3000, mlp_batch=16384, seed=5, log=print):
+        max_steps=60000, check_every=100, mlp_steps=3000, mlp_batch=16384, seed=5, cfl=None,
+        freeze_warm=True, history=False, log=print):
     out = {}
     t0 = time.perf_counter()
     tr = cases.bump_family(n_train, seed=seed, cells=train_cells, stream=200)
-    b_tr, rt = solve_batch(tr, train_tol, max_steps, check_every)
+    b_tr, rt = solve_batch(tr, train_tol, max_steps, check_every, cfl=cfl)
     states = b_tr.get_state()
     # converged solutions, and the finite end states of the ones still marching at max_steps
     ok = rt["status"] >= 0
@@ -74,10 +82,12 @@ def run(n_train=64, n_eval=32, train_cells=30000, eval_cells=60000, tol=1e-8, tr
     ev = cases.bump_family(n_eval, seed=seed + 100, cells=eval_cells, stream=300)
     fs_states = freestream_states(ev)
     # plain: freestream start
-    b = Batch([c.mesh for c in ev], np.array([c.fs for c in ev]), make_config(block=128, **SCHEME))
+    b = Batch([c.mesh for c in ev], np.array([c.fs for c in ev]), make_config(block=128, **scheme(cfl)))
     b.set_state(fs_states)
     rp = b.solve(tol=tol, max_steps=max_steps, check_every=check_every)
     plain_states = b.get_state()
+    if history:
+        out["plain_history"] = b.history().tolist()
     out["plain"] = dict(wall_s=rp["wall_s"], steps_max=int(rp["steps"].max()),
                         steps_mean=float(rp["steps"].mean()), converged=int((rp["status"] == 0).sum()))
     log(f"plain: {out['plain']}")
@@ -92,26 +102,50 @@ def run(n_train=64, n_eval=32, train_cells=30000, eval_cells=60000, tol=1e-8, tr
     use = np.isfinite(e_w) & (e_w <= e_f)
     start = [w if u else f for w, f, u in zip(warm, fs_states, use)]
     b.set_state(start)
+    if freeze_warm:
+        # a start close to the solution already has its shocks in place: record the limiter
+        # factors on the first step and march frozen (a frozen march whose error grows tenfold
+        # returns to the evaluated limiter)
+        b.set_freeze(use.astype(np.int32))
     t_check = time.perf_counter() - tw - t_inf
     rw = b.solve(tol=tol, max_steps=max_steps, check_every=check_every)
     warm_wall = time.perf_counter() - tw
     warm_states = b.get_state()
+    b.set_freeze(None)
+    if history:
+        out["warm_history"] = b.history().tolist()
     out["warm"] = dict(wall_s=warm_wall, march_s=rw["wall_s"], inference_s=t_inf, fallback_check_s=t_check,
                        steps_max=int(rw["steps"].max()), steps_mean=float(rw["steps"].mean()),
                        converged=int((rw["status"] == 0).sum()), fallbacks=int((~use).sum()))
     log(f"warm: {out['warm']}")
-    # surrogate error against the converged field, and agreement of the two converged fields
+    # surrogate error against the converged field, and agreement of the two converged fields,
+    # over the instances both marches converged
+    both = (rp["status"] == 0) & (rw["status"] == 0)
     s_err, d_conv = [], []
-    for Uw0, Up, Uw in zip(warm, plain_states, warm_states):
+    for k, (Uw0, Up, Uw) in enumerate(zip(warm, plain_states, warm_states)):
+        if not both[k]:
+            continue
         Wp = prim_from_cons(Up)
         s_err.append(np.linalg.norm(prim_from_cons(Uw0) - Wp) / np.linalg.norm(Wp))
         d_conv.append(np.abs(Uw - Up).max() / np.abs(Up).max())
-    out["surrogate_rel_l2"] = dict(mean=float(np.mean(s_err)), max=float(np.max(s_err)))
-    out["converged_fields_max_rel_diff"] = float(np.max(d_conv))
+    out["surrogate_rel_l2"] = dict(mean=float(np.mean(s_err)) if s_err else None,
+                                   max=float(np.max(s_err)) if s_err else None)
+    out["converged_fields_max_rel_diff"] = float(np.max(d_conv)) if d_conv else None
     out["speedup"] = out["plain"]["wall_s"] / out["warm"]["wall_s"]
+    # the instances both marches converged: time until the last of them converged, and steps
+    if both.any():
+        out["common"] = dict(instances=int(both.sum()),
+                             plain_s=float(rp["conv_s"][both].max()),
+                             warm_s=float(rw["conv_s"][both].max() + t_inf + t_check),
+                             plain_steps=int(rp["steps"][both].sum()), warm_steps=int(rw["steps"][both].sum()))
+        out["common"]["speedup"] = out["common"]["plain_s"] / out["common"]["warm_s"]
+        out["common"]["steps_ratio"] = out["common"]["plain_steps"] / max(1, out["common"]["warm_steps"])
+    out["instances"] = [dict(mach=c.mach, thickness=c.param, plain_status=int(a), plain_steps=int(b2),
+                             warm_status=int(w), warm_steps=int(ws))
+                        for c, a, b2, w, ws in zip(ev, rp["status"], rp["steps"], rw["status"], rw["steps"])]
     out["total_s"] = time.perf_counter() - t0
-    log(f"surrogate rel L2 {out['surrogate_rel_l2']}, converged diff {out['converged_fields_max_rel_diff']:.3e}, "
-        f"speedup {out['speedup']:.2f}")
+    log(f"surrogate rel L2 {out['surrogate_rel_l2']}, converged diff {out['converged_fields_max_rel_diff']}, "
+        f"speedup {out['speedup']:.2f}, common {out.get('common')}")
     return out
 
 

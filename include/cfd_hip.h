@@ -166,6 +166,15 @@ int cfd_batch_get_state(cfd_batch_t *b, double *U);
  * NULL. Returns 0 when every instance converged, CFD_E_NOTCONV otherwise. */
 int cfd_batch_solve(cfd_batch_t *b, double tol, int max_steps, int check_every, int *steps,
                     double *err4, int *status, double *conv_s, double *wall_s, void *stream);
+/* Per-instance limiter-freezing mode that every later solve starts with (needs freeze_tol or
+ * freeze_stall in cfg): 0 = evaluated limiter until the freezing trigger (the default), 1 = record
+ * the limiter factors on the first step and march frozen from the second (for starts already
+ * close to the solution). mode = NULL restores 0 for all. No counterpart in the reference. */
+int cfd_batch_set_freeze(cfd_batch_t *b, const int *mode);
+/* Error[0] of every instance at each check of the last solve: *nchecks checks, e0[check*ninst + i]
+ * (e0 may be NULL to query the count). The batched counterpart of the reference's per-iteration
+ * Error rows (src/output_files.cpp:558-575). */
+int cfd_batch_history(const cfd_batch_t *b, int *nchecks, double *e0);
 /* Times nsteps graph-launched steps of all instances (advances the state). */
 int cfd_batch_time_steps(cfd_batch_t *b, int nsteps, float *ms);
 

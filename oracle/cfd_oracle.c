@@ -961,7 +961,12 @@ static void reconstruct_side(const ONatCfg *cfg, const FState *cellS, const FSta
     out->W[2] = Wf[2];
     out->W[3] = Wf[3];
     out->W[4] = sqrt(GAMMA * Wf[3] * (1.0 / Wf[0]));
-    cons_from_prim(Wf[0], Wf[1], Wf[2], Wf[3], out->U);
+    /* conserved values of the face state, with p / (gamma - 1) as a product by the reciprocal */
+    const double vmag = 0.5 * (Wf[1] * Wf[1] + Wf[2] * Wf[2]);
+    out->U[0] = Wf[0];
+    out->U[1] = Wf[0] * Wf[1];
+    out->U[2] = Wf[0] * Wf[2];
+    out->U[3] = Wf[3] * (1.0 / GM1) + Wf[0] * vmag;
 }
 
 void on_residual(const OMesh *m, const ONatCfg *cfg, const double *U, const double *U_exit_src,
