@@ -75,6 +75,7 @@ def lib() -> ctypes.CDLL:
     _sig(L, "cfd_batch_solve", _i, [_vp, _d, _i, _i, _ip, _dp, _ip, _dp, _dp, _vp])
     _sig(L, "cfd_batch_time_steps", _i, [_vp, _i, ctypes.POINTER(ctypes.c_float)])
     _sig(L, "cfd_batch_set_freeze", _i, [_vp, _ip])
+    _sig(L, "cfd_batch_set_cfl", _i, [_vp, _d])
     _sig(L, "cfd_batch_history", _i, [_vp, _ip, _dp])
     _fp = ctypes.POINTER(ctypes.c_float)
     _sig(L, "cfd_mlp_create", _i, [_i, _i, _i, _i, _i, ctypes.c_ulonglong, ctypes.POINTER(_vp)])

@@ -100,6 +100,10 @@ class Batch:
             raise ValueError(f"mode must have {self.ninst} entries")
         check(lib().cfd_batch_set_freeze(self.h, iptr(m)), "cfd_batch_set_freeze")
 
+    def set_cfl(self, cfl: float):
+        """Courant number of later solves."""
+        check(lib().cfd_batch_set_cfl(self.h, float(cfl)), "cfd_batch_set_cfl")
+
     def history(self) -> np.ndarray:
         """Error[0] per check of the last solve, [checks, ninst]."""
         n = ctypes.c_int()

@@ -581,6 +581,14 @@ int cfd_batch_set_freeze(cfd_batch_t *b, const int *mode) {
     return 0;
 }
 
+int cfd_batch_set_cfl(cfd_batch_t *b, double cfl) {
+    if (!b || !(cfl > 0.0) || !std::isfinite(cfl)) return capi_fail(CFD_E_ARG, "cfd_batch_set_cfl: cfl must be positive");
+    b->cfg.cfl = cfl;
+    b->dcfg.cfl = cfl;
+    b->graph_tol = std::nan("");  // the captured step holds the old value: capture again
+    return 0;
+}
+
 int cfd_batch_history(const cfd_batch_t *b, int *nchecks, double *e0) {
     if (!b || !nchecks) return capi_fail(CFD_E_ARG, "cfd_batch_history: null argument");
     *nchecks = b->hist_checks;

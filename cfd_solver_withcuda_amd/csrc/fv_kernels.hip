@@ -147,15 +147,6 @@ __device__ __forceinline__ void load_face_grad(const DevMesh &m, int e, FaceRec 
     }
 }
 
-// Unit normal and length of face entry e.
-__device__ __forceinline__ void face_ndl(const DevMesh &m, int e, double &nx, double &ny, double &dl) {
-    const int4 a = reinterpret_cast<const int4 *>(m.fgeo)[3 * (size_t)e];
-    const double2 b = m.fgeo[3 * (size_t)e + 1];
-    nx = __hiloint2double(a.w, a.z);
-    ny = b.x;
-    dl = b.y;
-}
-
 // ------------------------------------------------------------------------------------------
 // Gradient pass: Green-Gauss (reference semantics, ghost faces use the cell value) or weighted
 // least squares (ghost neighbours at the reflected centre with the boundary ghost state).
@@ -432,6 +423,17 @@ __global__ void __launch_bounds__(256) CFD_FLUX_ATTR flux_tile_kernel(DevMesh m,
     double *sU = sW + 5 * T;                    // [4][T] conserved (first order only)
     double *sF = sU + (ORDER == 1 ? 4 * T : 0); // [4][FT] face fluxes
     double *sG = sF + 4 * FT;                   // [8][T] (second order, LDS-staged gradients)
+    double *sN = sG + (glds ? 8 * T : 0);       // [3][FT] face normal and length (time steps)
+    // phase C forms time steps from the face geometry: staged here from the face records of
+    // phase B instead of read again from HBM
+    const bool needN = (MODE == MODE_RESID) || (st.lts && st.stage == 0) || (st.last && st.dt_partial);
+    auto stageN = [&](int j, const FaceRec &f) {
+        if (needN) {
+            sN[j] = f.nx;
+            sN[FT + j] = f.ny;
+            sN[2 * FT + j] = f.dl;
+        }
+    };
     const int k = threadIdx.x;
     const bool hasCell = k < nct;
     const int c = c0 + k;
@@ -475,11 +477,18 @@ __global__ void __launch_bounds__(256) CFD_FLUX_ATTR flux_tile_kernel(DevMesh m,
     __syncthreads();
 
     // ---- B: one flux per face ----------------------------------------------------------
-    if (h0) flux_face<FLUX, ORDER, LIM>(m, cfg, Us, grad, st.Uexit, sW, sU, sG, sF, glds, T, FT, c0, nct, k, f0, fz, st.psi, fs + k, fsv);
-    if (h1) flux_face<FLUX, ORDER, LIM>(m, cfg, Us, grad, st.Uexit, sW, sU, sG, sF, glds, T, FT, c0, nct, k + B, f1, fz, st.psi, fs + k + B, fsv);
+    if (h0) {
+        stageN(k, f0);
+        flux_face<FLUX, ORDER, LIM>(m, cfg, Us, grad, st.Uexit, sW, sU, sG, sF, glds, T, FT, c0, nct, k, f0, fz, st.psi, fs + k, fsv);
+    }
+    if (h1) {
+        stageN(k + B, f1);
+        flux_face<FLUX, ORDER, LIM>(m, cfg, Us, grad, st.Uexit, sW, sU, sG, sF, glds, T, FT, c0, nct, k + B, f1, fz, st.psi, fs + k + B, fsv);
+    }
     for (int j = k + 2 * B; j < nft; j += B) {
         FaceRec f;
         load_face<true, ORDER == 2>(m, fs + j, f);
+        stageN(j, f);
         flux_face<FLUX, ORDER, LIM>(m, cfg, Us, grad, st.Uexit, sW, sU, sG, sF, glds, T, FT, c0, nct, j, f, fz, st.psi, fs + j, fsv);
     }
     __syncthreads();
@@ -509,10 +518,8 @@ __global__ void __launch_bounds__(256) CFD_FLUX_ATTR flux_tile_kernel(DevMesh m,
             for (int f = 0; f < 4; f++) {
                 const uint16_t sl = sls[f];
                 if (sl == 0xFFFF) break;
-                const int e = fs + (sl >> 1);
-                double nx, ny, dl;
-                face_ndl(m, e, nx, ny, dl);
-                denom += (fabs(w1 * nx + w2 * ny) + w4) * dl;
+                const int j = sl >> 1;
+                denom += (fabs(w1 * sN[j] + w2 * sN[FT + j]) + w4) * sN[2 * FT + j];
             }
         }
         // Inviscid_Time_Step (src/Time_Step.cpp:293-329)
@@ -570,10 +577,8 @@ __global__ void __launch_bounds__(256) CFD_FLUX_ATTR flux_tile_kernel(DevMesh m,
                     for (int f = 0; f < 4; f++) {
                         const uint16_t sl = sls[f];
                         if (sl == 0xFFFF) break;
-                        const int e = fs + (sl >> 1);
-                        double nx, ny, dl;
-                        face_ndl(m, e, nx, ny, dl);
-                        den += (fabs(s.W[1] * nx + s.W[2] * ny) + s.W[4]) * dl;
+                        const int j = sl >> 1;
+                        den += (fabs(s.W[1] * sN[j] + s.W[2] * sN[FT + j]) + s.W[4]) * sN[2 * FT + j];
                     }
                     const double dtn = (den <= 0.0 || !isfinite(den)) ? 0.0 : cfg.cfl * ar / den;
                     dtmin = fmin(dtmin, dtn);
@@ -998,7 +1003,7 @@ size_t grad_smem_bytes(const DevMesh &m) {
 }
 
 size_t flux_smem_bytes(const DevMesh &m, int order, int grad_lds) {
-    size_t d = 5 * (size_t)m.tile + (order == 1 ? 4 * (size_t)m.tile : 0) + 4 * (size_t)m.max_tile_faces;
+    size_t d = 5 * (size_t)m.tile + (order == 1 ? 4 * (size_t)m.tile : 0) + 7 * (size_t)m.max_tile_faces;
     if (order == 2 && grad_lds) d += 8 * (size_t)m.tile;
     // the block reductions of phase C need 4 doubles per thread (blockDim <= 256)
     if (d < 4 * 256) d = 4 * 256;

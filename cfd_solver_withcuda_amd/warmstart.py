@@ -21,7 +21,7 @@ from .batch import Batch
 from .solver import cons_from_prim, make_config
 from .surrogate import MLP, features, predicted_state, prim_from_cons
 
-SCHEME = dict(flux="hllc", order=2, limiter="vanleer", gradient="wlsq", cfl=0.8, rk=3, lts=1,
+SCHEME = dict(flux="hllc", order=2, limiter="vanleer", gradient="wlsq", cfl=1.2, rk=3, lts=1,
               roe_bdiss=1, bc=cases.BUMP_BCS, freeze_stall=300)
 
 
@@ -55,7 +55,7 @@ def training_set(cs, states):
 
 def run(n_train=64, n_eval=32, train_cells=30000, eval_cells=60000, tol=1e-8, train_tol=1e-8,
         max_steps=60000, check_every=100, mlp_steps=3000, mlp_batch=16384, seed=5, cfl=None,
-        freeze_warm=True, history=False, log=print):
+        freeze_warm=True, warm_cfl=None, history=False, log=print):
     out = {}
     t0 = time.perf_counter()
     tr = cases.bump_family(n_train, seed=seed, cells=train_cells, stream=200)
@@ -102,6 +102,8 @@ def run(n_train=64, n_eval=32, train_cells=30000, eval_cells=60000, tol=1e-8, tr
     use = np.isfinite(e_w) & (e_w <= e_f)
     start = [w if u else f for w, f, u in zip(warm, fs_states, use)]
     b.set_state(start)
+    if warm_cfl is not None:
+        b.set_cfl(warm_cfl)
     if freeze_warm:
         # a start close to the solution already has its shocks in place: record the limiter
         # factors on the first step and march frozen (a frozen march whose error grows tenfold

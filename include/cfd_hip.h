@@ -171,6 +171,8 @@ int cfd_batch_solve(cfd_batch_t *b, double tol, int max_steps, int check_every, 
  * the limiter factors on the first step and march frozen from the second (for starts already
  * close to the solution). mode = NULL restores 0 for all. No counterpart in the reference. */
 int cfd_batch_set_freeze(cfd_batch_t *b, const int *mode);
+/* Courant number of later solves (the local time steps scale with it). */
+int cfd_batch_set_cfl(cfd_batch_t *b, double cfl);
 /* Error[0] of every instance at each check of the last solve: *nchecks checks, e0[check*ninst + i]
  * (e0 may be NULL to query the count). The batched counterpart of the reference's per-iteration
  * Error rows (src/output_files.cpp:558-575). */

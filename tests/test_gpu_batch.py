@@ -113,3 +113,46 @@ def test_batch_freezing_reaches_tolerance():
           r1["steps"].tolist(), flush=True)
     assert r1["converged"], r1
     assert np.all(r1["err"][:, 0] <= 1e-8)
+
+
+def test_batch_history_cfl_and_first_step_freezing():
+    """cfd_batch_history returns Error[0] per check (its last row is the solve's Error[0]);
+    cfd_batch_set_cfl gives the same march as a batch created with that CFL; freezing mode 1
+    (record on the first step, then frozen) matches a single solver put in the same mode, and
+    leaves the other instances' marches unchanged."""
+    meshes, fs = instances()
+    meshes, fs = meshes[:3], fs[:3]
+    cfg = dict(flux="hllc", order=2, limiter="vanleer", gradient="wlsq", rk=3, lts=1, bc=RAMP_BCS,
+               freeze_stall=50)
+    print("case batch history / cfl / freeze mode 1, 3 ramps", flush=True)
+    b = Batch(meshes, fs, make_config(block=128, cfl=0.8, **cfg))
+    r = b.solve(tol=-1.0, max_steps=30, check_every=10)
+    h = b.history()
+    assert h.shape == (3, 3)
+    assert np.array_equal(h[-1], r["err"][:, 0])
+    assert np.all(np.isfinite(h)) and np.all(h > 0)
+    # CFL override vs a batch built with it
+    b.set_cfl(0.5)
+    b.set_state([freestream_state(len(m.cells), f) for m, f in zip(meshes, fs)])
+    b.solve(tol=-1.0, max_steps=20, check_every=20)
+    b5 = Batch(meshes, fs, make_config(block=128, cfl=0.5, **cfg))
+    b5.solve(tol=-1.0, max_steps=20, check_every=20)
+    for u, v in zip(b.get_state(), b5.get_state()):
+        assert np.array_equal(u, v)
+    # first-step freezing on instance 0 only
+    b.set_cfl(0.8)
+    b.set_state([freestream_state(len(m.cells), f) for m, f in zip(meshes, fs)])
+    b.set_freeze([1, 0, 0])
+    b.solve(tol=-1.0, max_steps=12, check_every=12)
+    Ub = b.get_state()
+    b.set_freeze(None)
+    for i, m in enumerate(meshes):
+        s = Solver(Mesh(m.points, m.cells, tile=128, face_budget=256),
+                   make_config(block=128, cfl=0.8, fs=fs[i], **cfg))
+        s.set_state(freestream_state(len(m.cells), fs[i]))
+        if i == 0:
+            s.set_freeze(1)
+        s.step(12)
+        assert np.array_equal(Ub[i], s.get_state()), i
+    with pytest.raises(Exception):
+        b.set_freeze([2, 0, 0])
