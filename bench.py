@@ -62,6 +62,7 @@ def parse():
     p.add_argument("--ws-mlp-steps", type=int, default=3000)
     p.add_argument("--ws-max-steps", type=int, default=30000)
     p.add_argument("--instances", type=int, default=32, help="batch workload: ramp instances")
+    p.add_argument("--batch-max-steps", type=int, default=60000, help="batch workload: step limit")
     p.add_argument("--min-cells", type=int, default=20000)
     p.add_argument("--max-cells", type=int, default=100000)
     p.add_argument("--tile", type=int, default=128, help="max cells per tile")
@@ -155,6 +156,7 @@ def warmstart_bench(args):
             "instances": args.ws_eval, "cells_per_instance": args.ws_eval_cells,
             "fallbacks": r["warm"]["fallbacks"], "surrogate_rel_l2": r["surrogate_rel_l2"],
             "converged_fields_max_rel_diff": r["converged_fields_max_rel_diff"],
+            "converged_fields_rel_l2_diff": r["converged_fields_rel_l2_diff"],
             "training": {"cases": r["train"]["cases"], "solve_s": r["train"]["wall_s"],
                          "mlp_steps": r["mlp"]["steps"], "mlp_s": r["mlp"]["train_s"],
                          "loss_first": r["mlp"]["loss_first"], "loss_last": r["mlp"]["loss_last"]},
@@ -180,7 +182,10 @@ def batch_bench(args):
     ms = b.time_steps(args.steps)
     b.set_state([np.tile(cons_from_prim(np.array([c.fs[0]]), c.fs[1], c.fs[2], c.fs[3]), (len(c.mesh.cells), 1))
                  for c in cs])
-    r = b.solve(tol=1e-8, max_steps=args.ws_max_steps, check_every=100)
+    r = b.solve(tol=1e-8, max_steps=args.batch_max_steps, check_every=100,
+                progress=lambda done, marching: print(f"batch: {done} steps, {marching} instances marching",
+                                                      file=sys.stderr, flush=True),
+                progress_every=20)
     out = {
         "metric": "time to residual 1e-8, batched plain FP64 solves",
         "value": r["wall_s"], "unit": "s", "n_gpus": 1, "steps": int(r["steps"].max()), "warmup": 0,
@@ -190,6 +195,7 @@ def batch_bench(args):
                    "cells": info["cells"], "tiles": info["tiles"], "autotuned": {k: info[k] for k in ("tile", "face_budget", "block")}},
         "cell_updates_per_s": info["cells"] / (ms / args.steps / 1e3),
         "converged": int((r["status"] == 0).sum()), "instances": args.instances,
+        "last_converged_s": float(r["conv_s"][r["status"] == 0].max()) if (r["status"] == 0).any() else None,
         "steps_per_instance": {"min": int(r["steps"].min()), "max": int(r["steps"].max())},
         "setup_s": setup, "autotune_s": info["autotune_s"],
     }

@@ -33,6 +33,10 @@ class Config(ctypes.Structure):
                 ("freeze_tol", _d), ("freeze_stall", _i), ("zeta", _d), ("zeta1", _d)]
 
 
+# cfd_progress_fn: void (*)(int steps_done, int marching, void *user)
+PROGRESS_FN = ctypes.CFUNCTYPE(None, ctypes.c_int, ctypes.c_int, ctypes.c_void_p)
+
+
 def _sig(lib, name, res, args):
     fn = getattr(lib, name)
     fn.restype = res
@@ -76,6 +80,7 @@ def lib() -> ctypes.CDLL:
     _sig(L, "cfd_batch_time_steps", _i, [_vp, _i, ctypes.POINTER(ctypes.c_float)])
     _sig(L, "cfd_batch_set_freeze", _i, [_vp, _ip])
     _sig(L, "cfd_batch_set_cfl", _i, [_vp, _d])
+    _sig(L, "cfd_batch_set_progress", _i, [_vp, PROGRESS_FN, _vp, _i])
     _sig(L, "cfd_batch_history", _i, [_vp, _ip, _dp])
     _fp = ctypes.POINTER(ctypes.c_float)
     _sig(L, "cfd_mlp_create", _i, [_i, _i, _i, _i, _i, ctypes.c_ulonglong, ctypes.POINTER(_vp)])

@@ -13,7 +13,7 @@ import os
 
 import numpy as np
 
-from ._lib import Config, check, dptr, iptr, lib
+from ._lib import PROGRESS_FN, Config, check, dptr, iptr, lib
 
 
 class Batch:
@@ -75,15 +75,24 @@ class Batch:
         return [U[self.cell_off[i]:self.cell_off[i + 1]] for i in range(self.ninst)]
 
     def solve(self, tol: float = 1e-8, max_steps: int = 100000, check_every: int = 100,
-              stream: int = 0) -> dict:
+              stream: int = 0, progress=None, progress_every: int = 10) -> dict:
+        """progress(steps_done, marching) is called every `progress_every` status checks."""
         n = self.ninst
+        cb = None
+        if progress is not None:
+            cb = PROGRESS_FN(lambda done, marching, user: progress(int(done), int(marching)))
+            check(lib().cfd_batch_set_progress(self.h, cb, None, int(progress_every)), "cfd_batch_set_progress")
         steps = np.zeros(n, np.int32)
         err = np.zeros((n, 4))
         status = np.zeros(n, np.int32)
         conv = np.zeros(n)
         wall = ctypes.c_double()
-        rc = lib().cfd_batch_solve(self.h, tol, max_steps, check_every, iptr(steps), dptr(err),
-                                   iptr(status), dptr(conv), ctypes.byref(wall), ctypes.c_void_p(stream))
+        try:
+            rc = lib().cfd_batch_solve(self.h, tol, max_steps, check_every, iptr(steps), dptr(err),
+                                       iptr(status), dptr(conv), ctypes.byref(wall), ctypes.c_void_p(stream))
+        finally:
+            if cb is not None:
+                lib().cfd_batch_set_progress(self.h, PROGRESS_FN(), None, 0)
         if rc not in (0, -4):
             check(rc, "cfd_batch_solve")
         return dict(converged=rc == 0, steps=steps, err=err, status=status, conv_s=conv,

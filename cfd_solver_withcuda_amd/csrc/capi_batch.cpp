@@ -89,6 +89,9 @@ struct cfd_batch_s {
     std::vector<int> fz_start;   // per instance freeze mode each solve starts with (empty: 0)
     std::vector<double> hist;    // Error[0] per instance at each check of the last solve
     int hist_checks = 0;
+    cfd_progress_fn progress = nullptr;  // called every progress_every checks of a solve
+    void *progress_user = nullptr;
+    int progress_every = 0;
     double build_s = 0.0, tune_s = 0.0;
     ~cfd_batch_s() {
         if (exec) (void)hipGraphExecDestroy(exec);
@@ -552,10 +555,16 @@ int cfd_batch_solve(cfd_batch_t *b, double tol, int max_steps, int check_every, 
         b->hist_checks++;
         const double t = now_s() - t0;
         all = true;
+        int marching = 0;
         for (int i = 0; i < ni; i++) {
             if (st[i] != 1 && conv[i] < 0.0) conv[i] = t;
-            if (st[i] == 1) all = false;
+            if (st[i] == 1) {
+                all = false;
+                marching++;
+            }
         }
+        if (b->progress && b->progress_every > 0 && b->hist_checks % b->progress_every == 0)
+            b->progress(done, marching, b->progress_user);
     }
     const double wall = now_s() - t0;
     if (steps) BHIP(hipMemcpy(steps, b->d_steps, ni * sizeof(int), hipMemcpyDeviceToHost));
@@ -578,6 +587,14 @@ int cfd_batch_set_freeze(cfd_batch_t *b, const int *mode) {
     for (int i = 0; i < b->ninst; i++)
         if (mode[i] != 0 && mode[i] != 1) return capi_fail(CFD_E_ARG, "cfd_batch_set_freeze: mode must be 0 or 1");
     b->fz_start.assign(mode, mode + b->ninst);
+    return 0;
+}
+
+int cfd_batch_set_progress(cfd_batch_t *b, cfd_progress_fn fn, void *user, int every_checks) {
+    if (!b || every_checks < 0) return capi_fail(CFD_E_ARG, "cfd_batch_set_progress: bad arguments");
+    b->progress = fn;
+    b->progress_user = user;
+    b->progress_every = every_checks;
     return 0;
 }
 

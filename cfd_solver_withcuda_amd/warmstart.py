@@ -123,16 +123,18 @@ def run(n_train=64, n_eval=32, train_cells=30000, eval_cells=60000, tol=1e-8, tr
     # surrogate error against the converged field, and agreement of the two converged fields,
     # over the instances both marches converged
     both = (rp["status"] == 0) & (rw["status"] == 0)
-    s_err, d_conv = [], []
+    s_err, d_conv, d_l2 = [], [], []
     for k, (Uw0, Up, Uw) in enumerate(zip(warm, plain_states, warm_states)):
         if not both[k]:
             continue
         Wp = prim_from_cons(Up)
         s_err.append(np.linalg.norm(prim_from_cons(Uw0) - Wp) / np.linalg.norm(Wp))
         d_conv.append(np.abs(Uw - Up).max() / np.abs(Up).max())
+        d_l2.append(np.linalg.norm(Uw - Up) / np.linalg.norm(Up))
     out["surrogate_rel_l2"] = dict(mean=float(np.mean(s_err)) if s_err else None,
                                    max=float(np.max(s_err)) if s_err else None)
     out["converged_fields_max_rel_diff"] = float(np.max(d_conv)) if d_conv else None
+    out["converged_fields_rel_l2_diff"] = dict(mean=float(np.mean(d_l2)), max=float(np.max(d_l2))) if d_l2 else None
     out["speedup"] = out["plain"]["wall_s"] / out["warm"]["wall_s"]
     # the instances both marches converged: time until the last of them converged, and steps
     if both.any():

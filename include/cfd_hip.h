@@ -171,6 +171,11 @@ int cfd_batch_solve(cfd_batch_t *b, double tol, int max_steps, int check_every, 
  * the limiter factors on the first step and march frozen from the second (for starts already
  * close to the solution). mode = NULL restores 0 for all. No counterpart in the reference. */
 int cfd_batch_set_freeze(cfd_batch_t *b, const int *mode);
+/* Progress report of long solves: fn(steps done, instances still marching, user) after every
+ * `every_checks` status checks of cfd_batch_solve (fn = NULL or every_checks = 0: none). It runs
+ * on the calling thread, between graph launches. */
+typedef void (*cfd_progress_fn)(int steps_done, int marching, void *user);
+int cfd_batch_set_progress(cfd_batch_t *b, cfd_progress_fn fn, void *user, int every_checks);
 /* Courant number of later solves (the local time steps scale with it). */
 int cfd_batch_set_cfl(cfd_batch_t *b, double cfl);
 /* Error[0] of every instance at each check of the last solve: *nchecks checks, e0[check*ninst + i]
