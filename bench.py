@@ -5,7 +5,7 @@ Workload (default): steady supersonic flow over a 10%-thick circular-arc bump, c
 (~1.27M cells); HLLC flux, MUSCL with the Van Leer limiter, weighted-least-squares gradients,
 local time stepping, TVD-RK3, FP64. One step = one RK3 step (3 gradient + 3 flux passes).
 
-    python bench.py [--steps K] [--warmup W] [--workload bump|ffs|batch] [--no-cpu] [--warmstart 1]
+    python bench.py [--steps K] [--warmup W] [--workload bump|ffs|batch|sod] [--no-cpu] [--warmstart 0|1]
 
 Prints one JSON line. `value` is FP64 face-flux evaluations per second (unique faces times
 residual evaluations per second) with the state resident in HBM. The roofline entry is the
@@ -51,7 +51,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=40)
     p.add_argument("--warmup", type=int, default=5)
-    p.add_argument("--workload", default="bump", choices=["bump", "ffs", "batch"])
+    p.add_argument("--workload", default="bump", choices=["bump", "ffs", "batch", "sod"])
     p.add_argument("--warmstart", type=int, default=1,
                    help="1 (default): also measure time to Error[0] <= 1e-8 of batched bump-family solves, "
                         "surrogate-warm vs freestream starts (sub-object time_to_residual); 0: skip")
@@ -165,6 +165,81 @@ def warmstart_bench(args):
                     "both_converged: the instances both marches brought to 1e-8, time until the last of them"}
 
 
+def sod_bench(args):
+    """Sod shock tube, 400x40 uniform quads on [0,1]x[0,0.1]: the reference's iteration (Roe
+    first order with the Harten fix, TVD-RK3 in delta form, global dt, CFL 0.5) from the
+    discontinuous start to t >= 0.2 (the last step is not clipped, as in the reference). The same
+    steps run on the GPU, in the oracle and, where it is built, in the reference itself."""
+    from cfd_solver_withcuda_amd import meshgen
+    from cfd_solver_withcuda_amd.solver import Mesh, Solver, cons_from_prim, make_config
+    from cfd_solver_withcuda_amd.surrogate import cell_centres
+
+    m = meshgen.shock_tube_mesh(400, 40)
+    xc = cell_centres(m.points, m.cells)[:, 0]
+    left = xc < 0.5
+    U0 = np.ascontiguousarray(cons_from_prim(np.where(left, 1.0, 0.125), 0 * xc, 0 * xc, np.where(left, 1.0, 0.1)))
+    mesh = Mesh(m.points, m.cells, tile=128)
+    s = Solver(mesh, make_config(flux="roe", order=1, cfl=0.5, fs=(1.0, 0.0, 0.0, 1.0), rk=3, ref_mode=1))
+    # the number of steps to t = 0.2 (the dt sequence is the reference's, so it is fixed)
+    s.set_state(U0)
+    t, n = 0.0, 0
+    while t < 0.2:
+        _, dts = s.step(50, history=True)
+        for d in dts:
+            if t < 0.2:
+                t += d
+                n += 1
+    s.set_state(U0)
+    s.get_state()
+    t0 = time.perf_counter()
+    s.step(n)
+    U = s.get_state()
+    gpu_s = time.perf_counter() - t0
+    out = {
+        "metric": "time to t=0.2, Sod shock tube (reference iteration)",
+        "value": gpu_s, "unit": "s", "n_gpus": 1, "steps": n, "warmup": 0,
+        "ms_per_step": 1e3 * gpu_s / n, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (Sod initial state)",
+        "config": {"workload": "sod_400x40_quads_roe1o_harten_tvdrk3_cfl0.5_global_dt", "cells": int(len(m.cells)),
+                   "end_time": t},
+        "cell_updates_per_s": len(m.cells) * n / gpu_s,
+    }
+    if not args.no_cpu:
+        from oracle.oracle import Oracle, Reference, make_cfg
+
+        om = Oracle().mesh(m.points, m.cells)
+        Uo = np.ascontiguousarray(U0.copy())
+        ocfg = make_cfg(flux="roe", order=1, cfl=0.5, fs=(1.0, 0, 0, 1.0))
+        t0 = time.perf_counter()
+        for _ in range(n):
+            om.step(ocfg, Uo, rk=3, lts=0, ref=True)
+        port_s = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": port_s, "unit": "s", "cores": 1, "kind": "port",
+                               "sample": f"the same {n} steps in the oracle (C restatement), 1 thread"}
+        out["gpu_vs_oracle_max_abs_diff"] = float(np.abs(U - Uo).max())
+        if Reference.available():
+            import tempfile
+
+            ref = Reference()
+            with tempfile.TemporaryDirectory() as d:
+                path = os.path.join(d, "sod.vtk")
+                meshgen.write_vtk(path, m)
+                ref.load_vtk(path)
+            ref.configure(dissipation=3, second_order=0, time_accurate=1, lts=0, cfl=0.5, rho_in=1.0, p_in=1.0)
+            ref.init_state(U0)
+            t0 = time.perf_counter()
+            for _ in range(n):
+                ref.step()
+            ref_s = time.perf_counter() - t0
+            Ur = ref.state()[0][:len(m.cells)]
+            out["cpu_baseline"] = {"value": ref_s, "unit": "s", "cores": 1, "kind": "reference",
+                                   "sample": f"the same {n} steps in the reference's own CPU solver (built from its "
+                                             f"sources), 1 thread; the oracle took {port_s:.2f} s"}
+            out["gpu_vs_reference_max_rel_diff"] = float(np.abs(U - Ur).max() / np.abs(Ur).max())
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def batch_bench(args):
     """Batched steady compression-ramp solves (AUSM + MUSCL/Van Leer + WLSQ, LTS TVD-RK3) to
     Error[0] <= 1e-8 per instance, in one batch with an autotuned launch shape."""
@@ -225,6 +300,8 @@ def main():
 
     if args.workload == "batch":
         return batch_bench(args)
+    if args.workload == "sod":
+        return sod_bench(args)
     m, cfg, desc = workload(args.workload)
     t_setup = time.perf_counter()
     mesh = Mesh(m.points, m.cells, tile=args.tile, renumber=1, face_budget=args.face_budget)
