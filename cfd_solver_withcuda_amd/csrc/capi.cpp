@@ -283,7 +283,7 @@ static int check_config(const cfd_mesh_s *m, const cfd_config_t *c) {
         if (c->bc[k] < 1 || c->bc[k] > 7) return fail(CFD_E_ARG, "config: bc types must be 1..7");
     if (m->dev.tile > c->block) return fail(CFD_E_ARG, "config: the mesh tile (cells per tile) must not exceed the block size");
     const size_t sm = flux_smem_bytes(m->dev, c->order, c->grad_lds);
-    if (sm > 65536 || grad_smem_bytes(m->dev) > 65536)
+    if (sm > 65536 || grad_smem_bytes(m->dev, c->gradient) > 65536)
         return fail(CFD_E_ARG, "config: tile too large for 64 KiB of LDS per workgroup; use a smaller tile");
     return 0;
 }

@@ -290,7 +290,7 @@ int build_batch(const std::vector<InstanceInput> &in, const double *fs4, const c
     b->dcfg.xcd_remap = cfg.xcd_remap;
     b->dcfg.grad_lds = cfg.grad_lds;
     const size_t sm = flux_smem_bytes(d, cfg.order, cfg.grad_lds);
-    if (sm > 65536 || grad_smem_bytes(d) > 65536) {
+    if (sm > 65536 || grad_smem_bytes(d, cfg.gradient) > 65536) {
         delete b;
         return capi_fail(CFD_E_ARG, "cfd_batch_create: tile too large for 64 KiB of LDS per workgroup");
     }

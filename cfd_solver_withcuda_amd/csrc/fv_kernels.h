@@ -30,7 +30,7 @@ struct StageArgs {
     double *psi;
 };
 
-size_t grad_smem_bytes(const DevMesh &m);
+size_t grad_smem_bytes(const DevMesh &m, int grad_type);
 size_t flux_smem_bytes(const DevMesh &m, int order, int grad_lds);
 hipError_t launch_grad(const DevMesh &m, const DevCfg &c, int grad_type, const double4 *Us,
                        double *grad, int block, hipStream_t s);

@@ -156,8 +156,7 @@ template <int GRAD>
 __device__ __forceinline__ void grad_face(const DevMesh &m, const DevCfg &cfg,
                                           const double4 *__restrict__ Us, const double *sW,
                                           double *sX, int T, int FT, int c0, int nct, int e,
-                                          int j, const FaceRec &f, double2 wl, double2 wr,
-                                          const double *fsv) {
+                                          int j, const FaceRec &f, const double *fsv) {
     const int lL = f.L - c0;
     double WL[5], WR[4];
     if ((unsigned)lL < (unsigned)nct) {
@@ -187,18 +186,17 @@ __device__ __forceinline__ void grad_face(const DevMesh &m, const DevCfg &cfg,
             for (int q = 0; q < 4; q++) WR[q] = s.W[q];
         }
     }
+    // per face: Green-Gauss, the face value times the scaled normal (8 values); WLSQ, the
+    // difference of each variable (4 values; the gather phase reads each cell face's
+    // least-squares vector itself)
     if constexpr (GRAD == GRAD_GG) {
         for (int q = 0; q < 4; q++) {
-            double phi_face = 0.5 * (WL[q] + WR[q]);
+            const double phi_face = 0.5 * (WL[q] + WR[q]);
             sX[(2 * q) * FT + j] = phi_face * f.nx * f.dl;
             sX[(2 * q + 1) * FT + j] = phi_face * f.ny * f.dl;
         }
     } else {
         for (int q = 0; q < 4; q++) sX[q * FT + j] = WR[q] - WL[q];
-        sX[4 * FT + j] = wl.x;
-        sX[5 * FT + j] = wl.y;
-        sX[6 * FT + j] = wr.x;
-        sX[7 * FT + j] = wr.y;
     }
 }
 
@@ -218,7 +216,7 @@ __global__ void __launch_bounds__(256) grad_tile_kernel(DevMesh m, DevCfg cfg,
     const int nct = m.tile_cstart[tile + 1] - c0;
     const int fs = m.tile_fstart[tile], nft = m.tile_fstart[tile + 1] - fs;
     double *sW = smem;              // [5][T]
-    double *sX = smem + 5 * T;      // [8][FT]
+    double *sX = smem + 5 * T;      // [8][FT] Green-Gauss, [4][FT] WLSQ
     constexpr bool GG = (GRAD == GRAD_GG);
     const int k = threadIdx.x;
     const bool hasCell = k < nct;
@@ -234,22 +232,9 @@ __global__ void __launch_bounds__(256) grad_tile_kernel(DevMesh m, DevCfg cfg,
         if constexpr (GG) ia = m.inv_area[c];
     }
     FaceRec f0{}, f1{};
-    double2 w0l{}, w0r{}, w1l{}, w1r{};
     const bool h0 = k < nft, h1 = k + B < nft;
-    if (h0) {
-        load_face_grad<GG>(m, fs + k, f0);
-        if constexpr (!GG) {
-            w0l = m.fw[2 * (size_t)(fs + k)];
-            if (f0.R >= 0) w0r = m.fw[2 * (size_t)(fs + k) + 1];
-        }
-    }
-    if (h1) {
-        load_face_grad<GG>(m, fs + k + B, f1);
-        if constexpr (!GG) {
-            w1l = m.fw[2 * (size_t)(fs + k + B)];
-            if (f1.R >= 0) w1r = m.fw[2 * (size_t)(fs + k + B) + 1];
-        }
-    }
+    if (h0) load_face_grad<GG>(m, fs + k, f0);
+    if (h1) load_face_grad<GG>(m, fs + k + B, f1);
     // ---- A ------------------------------------------------------------------------------
     if (hasCell) {
         FState s;
@@ -258,23 +243,30 @@ __global__ void __launch_bounds__(256) grad_tile_kernel(DevMesh m, DevCfg cfg,
     }
     __syncthreads();
     // ---- B ------------------------------------------------------------------------------
-    if (h0) grad_face<GRAD>(m, cfg, Us, sW, sX, T, FT, c0, nct, fs + k, k, f0, w0l, w0r, fsv);
-    if (h1) grad_face<GRAD>(m, cfg, Us, sW, sX, T, FT, c0, nct, fs + k + B, k + B, f1, w1l, w1r, fsv);
+    if (h0) grad_face<GRAD>(m, cfg, Us, sW, sX, T, FT, c0, nct, fs + k, k, f0, fsv);
+    if (h1) grad_face<GRAD>(m, cfg, Us, sW, sX, T, FT, c0, nct, fs + k + B, k + B, f1, fsv);
     for (int j = k + 2 * B; j < nft; j += B) {
         FaceRec f;
         load_face_grad<GG>(m, fs + j, f);
-        double2 wl{}, wr{};
-        if constexpr (!GG) {
-            wl = m.fw[2 * (size_t)(fs + j)];
-            if (f.R >= 0) wr = m.fw[2 * (size_t)(fs + j) + 1];
-        }
-        grad_face<GRAD>(m, cfg, Us, sW, sX, T, FT, c0, nct, fs + j, j, f, wl, wr, fsv);
+        grad_face<GRAD>(m, cfg, Us, sW, sX, T, FT, c0, nct, fs + j, j, f, fsv);
     }
     __syncthreads();
     // ---- C ------------------------------------------------------------------------------
     if (hasCell) {
         double g[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         const uint16_t sls[4] = {sl4.x, sl4.y, sl4.z, sl4.w};
+        // WLSQ: the least-squares vector of each of the cell's faces, all loads issued before
+        // the sums
+        double ga[4] = {0, 0, 0, 0}, gb[4] = {0, 0, 0, 0};
+        if constexpr (!GG) {
+            for (int f = 0; f < 4; f++) {
+                const uint16_t sl = sls[f];
+                if (sl == 0xFFFF) break;
+                const double2 w = m.fw[2 * (size_t)(fs + (sl >> 1)) + (sl & 1)];
+                ga[f] = w.x;
+                gb[f] = w.y;
+            }
+        }
         for (int f = 0; f < 4; f++) {
             const uint16_t sl = sls[f];
             if (sl == 0xFFFF) break;
@@ -286,13 +278,11 @@ __global__ void __launch_bounds__(256) grad_tile_kernel(DevMesh m, DevCfg cfg,
                     g[q] += isR ? -v : v;
                 }
             } else {
-                const double cx = isR ? sX[6 * FT + j] : sX[4 * FT + j];
-                const double cy = isR ? sX[7 * FT + j] : sX[5 * FT + j];
                 for (int q = 0; q < 4; q++) {
                     double d = sX[q * FT + j];
                     double dW = isR ? -d : d;
-                    g[2 * q] += cx * dW;
-                    g[2 * q + 1] += cy * dW;
+                    g[2 * q] += ga[f] * dW;
+                    g[2 * q + 1] += gb[f] * dW;
                 }
             }
         }
@@ -998,8 +988,8 @@ __global__ void __launch_bounds__(256) ref_update_kernel(int nc, int kind, int l
 // ------------------------------------------------------------------------------------------
 // Host-side launchers
 // ------------------------------------------------------------------------------------------
-size_t grad_smem_bytes(const DevMesh &m) {
-    return sizeof(double) * (5 * (size_t)m.tile + 8 * (size_t)m.max_tile_faces);
+size_t grad_smem_bytes(const DevMesh &m, int grad_type) {
+    return sizeof(double) * (5 * (size_t)m.tile + (grad_type == GRAD_GG ? 8 : 4) * (size_t)m.max_tile_faces);
 }
 
 size_t flux_smem_bytes(const DevMesh &m, int order, int grad_lds) {
@@ -1016,7 +1006,7 @@ static int grid_of(const DevMesh &m, const DevCfg &c) {
 
 hipError_t launch_grad(const DevMesh &m, const DevCfg &c, int grad_type, const double4 *Us,
                        double *grad, int block, hipStream_t s) {
-    const size_t sm = grad_smem_bytes(m);
+    const size_t sm = grad_smem_bytes(m, grad_type);
     if (grad_type == GRAD_GG)
         hipLaunchKernelGGL(grad_tile_kernel<GRAD_GG>, dim3(grid_of(m, c)), dim3(block), sm, s, m, c, Us, grad);
     else
