@@ -32,7 +32,7 @@ struct DevMesh {
     int nfe;       // face entries (all tiles)
     const int *tile_cstart;   // [ntiles+1] first cell of each tile
     const int *tile_fstart;   // [ntiles+1] first face entry of each tile
-    // Face entry e is one 48-byte record (three 16-byte loads), see FaceGeo:
+    // Face entry e is one 48-byte record (three 16-byte loads):
     //   fgeo[3e]   = {L, R} (two int32) and nx;  fgeo[3e+1] = {ny, dl};  fgeo[3e+2] = {dx, dy}
     // L = left cell (renumbered), R = right cell or -(face kind) on a boundary face, (nx, ny) the
     // unit normal from L, dl the length, (dx, dy) the centre offset x_R - x_L (ghost: reflected
@@ -40,7 +40,7 @@ struct DevMesh {
     const double2 *fgeo;      // [3*nfe] (flux pass)
     // The gradient pass reads only what it needs, from compact arrays:
     const int2 *fLR;          // [nfe] {L, R}
-    const double *fnx, *fny, *fdl;  // [nfe] unit normal and length (Green-Gauss)
+    const double *fnx, *fny, *fdl;  // [nfe] unit normal and length (Green-Gauss, WLSQ ghosts)
     const double2 *fw;        // [2*nfe] least-squares coefficient vectors of L (2e) and R (2e+1)
     const uint16_t *slots;    // [4*nc] (local face << 1) | (cell is R); 0xFFFF = none
     const double *area, *inv_area;  // [nc]
@@ -50,13 +50,6 @@ struct DevMesh {
     const int *tile_list;     // [ntiles] tiles of the instances still marching
     const int *tile_count;    // [1] entries of tile_list in use
 };
-
-// Host-side packing of one face entry into the fgeo record.
-struct FaceGeo {
-    int32_t L, R;
-    double nx, ny, dl, dx, dy;
-};
-static_assert(sizeof(FaceGeo) == 48, "FaceGeo is three 16-byte words");
 
 // Per-launch scheme settings.
 struct DevCfg {
