@@ -116,7 +116,8 @@ def test_batch_freezing_reaches_tolerance():
 
 
 def test_batch_history_cfl_and_first_step_freezing():
-    """cfd_batch_history returns Error[0] per check (its last row is the solve's Error[0]);
+    """cfd_batch_history returns Error[0] per check (its last row is the solve's Error[0]) and the
+    progress callback runs once per check;
     cfd_batch_set_cfl gives the same march as a batch created with that CFL; freezing mode 1
     (record on the first step, then frozen) matches a single solver put in the same mode, and
     leaves the other instances' marches unchanged."""
@@ -126,7 +127,10 @@ def test_batch_history_cfl_and_first_step_freezing():
                freeze_stall=50)
     print("case batch history / cfl / freeze mode 1, 3 ramps", flush=True)
     b = Batch(meshes, fs, make_config(block=128, cfl=0.8, **cfg))
-    r = b.solve(tol=-1.0, max_steps=30, check_every=10)
+    calls = []
+    r = b.solve(tol=-1.0, max_steps=30, check_every=10, progress=lambda s, n: calls.append((s, n)),
+                progress_every=1)
+    assert calls == [(10, 3), (20, 3), (30, 3)]
     h = b.history()
     assert h.shape == (3, 3)
     assert np.array_equal(h[-1], r["err"][:, 0])
