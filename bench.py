@@ -69,6 +69,8 @@ def parse():
     p.add_argument("--face-budget", type=int, default=256, help="max faces per tile")
     p.add_argument("--block", type=int, default=128)
     p.add_argument("--grad-lds", type=int, default=0)
+    p.add_argument("--gradient", default="", choices=["", "gg", "wlsq"],
+                   help="override the workload's gradient (bump: wlsq, ffs: gg)")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--kernel-reps", type=int, default=20)
     p.add_argument("--copy-calibration", action="store_true",
@@ -103,6 +105,15 @@ def flux_pass_bytes(C, F, cfg):
     length (24) + centre offset (16)."""
     per_cell = 32 + 64 + 16 + 8 + 32 + (8 if cfg["lts"] else 0) + (32 if cfg["rk"] == 4 else 0)
     return per_cell * C + 48 * F
+
+
+def residual_contract(C, F, gradient, ms):
+    """One MUSCL residual evaluation (gradient pass + flux pass, each timed alone) against the
+    two-pass contract bytes B = 224 C + 80 F (Green-Gauss) or 224 C + 96 F (WLSQ), with C the
+    physical plus ghost cells and F all faces."""
+    b = 224 * C + (96 if gradient == "wlsq" else 80) * F
+    gbs = b / (ms / 1e3) / 1e9
+    return {"contract_bytes": b, "ms": ms, "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS}
 
 
 def grad_pass_bytes(C, F, gradient):
@@ -303,6 +314,9 @@ def main():
     if args.workload == "sod":
         return sod_bench(args)
     m, cfg, desc = workload(args.workload)
+    if args.gradient and args.gradient != cfg["gradient"]:
+        desc = desc.replace("_" + cfg["gradient"] + "_", "_" + args.gradient + "_")
+        cfg["gradient"] = args.gradient
     t_setup = time.perf_counter()
     mesh = Mesh(m.points, m.cells, tile=args.tile, renumber=1, face_budget=args.face_budget)
     s = Solver(mesh, make_config(roe_bdiss=1, block=args.block, xcd_remap=1, grad_lds=args.grad_lds, **cfg))
@@ -328,7 +342,7 @@ def main():
 
     t_step = ms_steps / 1e3 / args.steps
     launch_cfg = (f"workload={args.workload} tile={args.tile} face_budget={args.face_budget} "
-                  f"block={args.block} grad_lds={args.grad_lds}")
+                  f"block={args.block} grad_lds={args.grad_lds}" + (f" gradient={args.gradient}" if args.gradient else ""))
     value = F * nres / t_step
     bflux = flux_pass_bytes(C, F, cfg)
     achieved = bflux / (ms_flux / 1e3) / 1e9
@@ -362,6 +376,7 @@ def main():
         "grad_pass": {"avg_launch_ms": ms_grad,
                       "algorithmic_bytes_per_launch": grad_pass_bytes(C, F, cfg["gradient"]),
                       "achieved_GBs": grad_pass_bytes(C, F, cfg["gradient"]) / (ms_grad / 1e3) / 1e9},
+        "residual_contract": residual_contract(mesh.nphys + mesh.nghost, F, cfg["gradient"], ms_grad + ms_flux),
         "setup_s": t_setup,
     }
     if not args.no_cpu:
