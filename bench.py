@@ -5,7 +5,7 @@ Workload (default): steady supersonic flow over a 10%-thick circular-arc bump, c
 (~1.27M cells); HLLC flux, MUSCL with the Van Leer limiter, weighted-least-squares gradients,
 local time stepping, TVD-RK3, FP64. One step = one RK3 step (3 gradient + 3 flux passes).
 
-    python bench.py [--steps K] [--warmup W] [--workload bump|ffs|batch|sod] [--no-cpu] [--warmstart 0|1]
+    python bench.py [--steps K] [--warmup W] [--workload bump|ffs|batch|sod|steady] [--no-cpu] [--warmstart 0|1]
 
 Prints one JSON line. `value` is FP64 face-flux evaluations per second (unique faces times
 residual evaluations per second) with the state resident in HBM. The roofline entry is the
@@ -51,7 +51,10 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=40)
     p.add_argument("--warmup", type=int, default=5)
-    p.add_argument("--workload", default="bump", choices=["bump", "ffs", "batch", "sod"])
+    p.add_argument("--workload", default="bump", choices=["bump", "ffs", "batch", "sod", "steady"])
+    p.add_argument("--thickness", type=float, default=0.04, help="steady workload: bump thickness")
+    p.add_argument("--cfl", type=float, default=1.2, help="steady workload: Courant number")
+    p.add_argument("--steady-max-steps", type=int, default=200000, help="steady workload: step limit")
     p.add_argument("--warmstart", type=int, default=1,
                    help="1 (default): also measure time to Error[0] <= 1e-8 of batched bump-family solves, "
                         "surrogate-warm vs freestream starts (sub-object time_to_residual); 0: skip")
@@ -68,6 +71,9 @@ def parse():
     p.add_argument("--tile", type=int, default=128, help="max cells per tile")
     p.add_argument("--face-budget", type=int, default=256, help="max faces per tile")
     p.add_argument("--block", type=int, default=128)
+    p.add_argument("--develop", type=int, default=-1,
+                   help="steps marched from freestream before the warmup, so the timed steps see a "
+                        "developed flow (-1: 2000 for bump, 300 for ffs)")
     p.add_argument("--grad-lds", type=int, default=0)
     p.add_argument("--gradient", default="", choices=["", "gg", "wlsq"],
                    help="override the workload's gradient (bump: wlsq, ffs: gg)")
@@ -251,6 +257,43 @@ def sod_bench(args):
     return 0
 
 
+def steady_bench(args):
+    """One steady solve of the bump configuration at full size (1400x700 mixed, Mach 1.65, HLLC +
+    MUSCL/Van Leer + WLSQ, LTS TVD-RK3, limiter freezing on a 300-step stall) from freestream to
+    Error[0] <= 1e-8; --thickness picks the bump (0.10 does not converge with this scheme)."""
+    from cfd_solver_withcuda_amd import meshgen
+    from cfd_solver_withcuda_amd.solver import Mesh, Solver, cons_from_prim, freestream, make_config
+
+    m = meshgen.bump_mesh(1400, 700, args.thickness, seed=1)
+    fs = freestream(1.65)
+    cfg = dict(flux="hllc", order=2, limiter="vanleer", gradient="wlsq", cfl=args.cfl, rk=3, lts=1,
+               bc=("freestream", "exit", "wall", "wall", "wall"), fs=fs, freeze_stall=300)
+    mesh = Mesh(m.points, m.cells, tile=128, renumber=1, face_budget=256)
+    s = Solver(mesh, make_config(roe_bdiss=1, block=128, xcd_remap=1, **cfg))
+    s.set_state(np.tile(cons_from_prim(np.array([fs[0]]), fs[1], fs[2], fs[3]), (mesh.nphys, 1)))
+    s.get_state()
+    done, ok, err = 0, False, np.full(4, np.nan)
+    t0 = time.perf_counter()
+    while done < args.steady_max_steps:
+        ok, n, err = s.run_steady(1e-8, min(2000, args.steady_max_steps - done), 100)
+        done += n
+        print(f"steady: {done} steps, Error[0] {err[0]:.3e}, {time.perf_counter() - t0:.1f} s",
+              file=sys.stderr, flush=True)
+        if ok or not np.isfinite(err[0]):
+            break
+    wall = time.perf_counter() - t0
+    out = {
+        "metric": "time to residual 1e-8, one steady solve", "value": wall, "unit": "s", "n_gpus": 1,
+        "steps": done, "warmup": 0, "ms_per_step": 1e3 * wall / max(done, 1), "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (freestream initial state)",
+        "config": {"workload": f"bump_1400x700_mixed_t{args.thickness:g}_M1.65_hllc_muscl_vanleer_wlsq_lts_rk3"
+                               f"_cfl{args.cfl:g}", "cells": mesh.nphys},
+        "converged": bool(ok), "error": [float(x) for x in err],
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def batch_bench(args):
     """Batched steady compression-ramp solves (AUSM + MUSCL/Van Leer + WLSQ, LTS TVD-RK3) to
     Error[0] <= 1e-8 per instance, in one batch with an autotuned launch shape."""
@@ -282,6 +325,9 @@ def batch_bench(args):
         "cell_updates_per_s": info["cells"] / (ms / args.steps / 1e3),
         "converged": int((r["status"] == 0).sum()), "instances": args.instances,
         "last_converged_s": float(r["conv_s"][r["status"] == 0].max()) if (r["status"] == 0).any() else None,
+        "not_converged": [{"mach": round(c.mach, 4), "angle_deg": round(c.param, 3), "status": int(st),
+                           "error0": float(e[0])}
+                          for c, st, e in zip(cs, r["status"], r["err"]) if st != 0],
         "steps_per_instance": {"min": int(r["steps"].min()), "max": int(r["steps"].max())},
         "setup_s": setup, "autotune_s": info["autotune_s"],
     }
@@ -313,6 +359,8 @@ def main():
         return batch_bench(args)
     if args.workload == "sod":
         return sod_bench(args)
+    if args.workload == "steady":
+        return steady_bench(args)
     m, cfg, desc = workload(args.workload)
     if args.gradient and args.gradient != cfg["gradient"]:
         desc = desc.replace("_" + cfg["gradient"] + "_", "_" + args.gradient + "_")
@@ -328,6 +376,15 @@ def main():
     nres = 4 if cfg["rk"] == 4 else 3
 
     stream = torch.cuda.current_stream().cuda_stream
+    # The timed steps run on a developed flow, not on the uniform start: on a uniform state every
+    # limiter sees zero differences and skips its division, and every face takes the same flux
+    # branch, which would flatter the kernels.
+    develop = args.develop if args.develop >= 0 else {"bump": 2000, "ffs": 300}[args.workload]
+    t_dev = time.perf_counter()
+    if develop:
+        s.step(develop, stream=stream)
+    U_dev = s.get_state()
+    t_dev = time.perf_counter() - t_dev
     s.step(args.warmup, stream=stream)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -358,7 +415,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded generated mesh, freestream initial state)",
+        "data": f"synthetic (seeded generated mesh; the state marched {develop} steps from freestream, untimed, "
+                f"before the warmup)",
         "value_is": "FP64 face-flux evals/s of the bump workload (unique faces x residual evaluations per s); "
                     "the time-to-residual part is the time_to_residual object",
         "config": {"workload": desc, "cells": C, "faces": F, "tile": args.tile,
@@ -377,10 +435,10 @@ def main():
                       "algorithmic_bytes_per_launch": grad_pass_bytes(C, F, cfg["gradient"]),
                       "achieved_GBs": grad_pass_bytes(C, F, cfg["gradient"]) / (ms_grad / 1e3) / 1e9},
         "residual_contract": residual_contract(mesh.nphys + mesh.nghost, F, cfg["gradient"], ms_grad + ms_flux),
-        "setup_s": t_setup,
+        "setup_s": t_setup, "develop_steps": develop, "develop_s": t_dev,
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(m, cfg, U0)
+        out["cpu_baseline"] = cpu_baseline(m, cfg, U_dev)
     if args.warmstart:
         del s, mesh
         out["time_to_residual"] = warmstart_bench(args)

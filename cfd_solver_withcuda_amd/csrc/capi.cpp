@@ -85,7 +85,7 @@ struct cfd_solver_s {
     double *dt = nullptr, *grad = nullptr, *hbuf = nullptr, *err_partial = nullptr, *dt_partial = nullptr;
     double *scal = nullptr;  // [0],[1] global dt slots, [2] stage minimum, [3..6] error norm
     double4 *Ug = nullptr;   // ghost states (reference ghost order), allocated on first use
-    double *psi = nullptr;   // [8 * face entries] frozen limiter factors, allocated on first use
+    float *psi = nullptr;    // [8 * face entries] frozen limiter factors, allocated on first use
     double4 *Uall = nullptr; // [cells + ghosts] state for the reference's second order
     int *fz = nullptr;       // [0] freezing state, [1] steps since the best Error[0]
     double *fz_best = nullptr;
@@ -485,7 +485,7 @@ static bool auto_freeze(const cfd_solver_s *s) {
 // Allocates the limiter-freezing buffers and sets the state to `mode` (best Error[0] reset).
 static int freeze_init(cfd_solver_s *s, int mode) {
     if (!s->psi) {
-        HIP_TRY(hipMalloc((void **)&s->psi, 8 * (size_t)s->m->dev.nfe * sizeof(double)));
+        HIP_TRY(hipMalloc((void **)&s->psi, 8 * (size_t)s->m->dev.nfe * sizeof(float)));
         HIP_TRY(hipMalloc((void **)&s->fz, 2 * sizeof(int)));
         HIP_TRY(hipMalloc((void **)&s->fz_best, sizeof(double)));
     }

@@ -75,7 +75,8 @@ struct cfd_batch_s {
     int *d_tile_inst = nullptr, *d_inst_tile = nullptr, *d_tile_list = nullptr, *d_tile_count = nullptr;
     int *d_inst_off = nullptr, *d_status = nullptr, *d_steps = nullptr;
     int *d_fz = nullptr, *d_fz_since = nullptr;  // per instance limiter-freezing state
-    double *d_fz_best = nullptr, *psi = nullptr;
+    double *d_fz_best = nullptr;
+    float *psi = nullptr;  // [8 * face entries] frozen limiter factors
     double4 *d_inst_fs = nullptr;
     double *d_err4 = nullptr;
     // solver state

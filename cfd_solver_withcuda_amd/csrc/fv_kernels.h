@@ -25,9 +25,9 @@ struct StageArgs {
     double *err_partial; // [4*ntiles] per-tile sums of the reference error terms
     const double4 *Uexit;  // exit ghosts copy this state when set (reference RK stages)
     // Limiter freezing (order 2): freeze[instance] = 0 off, 1 record psi, 2 use recorded psi;
-    // psi [8][nfe] = slope / (grad W . d) per face entry, side (L, R) and variable.
+    // psi [nfe][8] = slope / (grad W . d) per face entry, side (L, R) and variable (FP32).
     const int *freeze;
-    double *psi;
+    float *psi;
 };
 
 size_t grad_smem_bytes(const DevMesh &m, int grad_type);

@@ -315,7 +315,7 @@ __device__ __forceinline__ void flux_face(const DevMesh &m, const DevCfg &cfg,
                                           const double *__restrict__ grad, const double4 *Uexit,
                                           const double *sW, const double *sU, const double *sG,
                                           double *sF, bool glds, int T, int FT, int c0, int nct,
-                                          int j, const FaceRec &f, int fz, double *psi, int e,
+                                          int j, const FaceRec &f, int fz, float *psi, int e,
                                           const double *fsv) {
     const double nx = f.nx, ny = f.ny, dl = f.dl;
     const int lL = f.L - c0;
@@ -365,8 +365,16 @@ __device__ __forceinline__ void flux_face(const DevMesh &m, const DevCfg &cfg,
             load_grad(grad, f.L, gL);
         }
         FState fL, fR;
-        double *pL = fz ? psi + e : nullptr;
-        reconstruct_side<LIM>(SL, SR, gL, f.dx, f.dy, 1.0, fz, pL, m.nfe, fL);
+        // frozen limiter factors of this face entry: side L in pv[0..3], side R in pv[4..7]
+        // (8 contiguous floats per entry, two 16-byte loads)
+        float pv[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        float4 *p4 = fz ? reinterpret_cast<float4 *>(psi) + 2 * (size_t)e : nullptr;
+        if (fz == 2) {
+            const float4 a = p4[0], b = p4[1];
+            pv[0] = a.x; pv[1] = a.y; pv[2] = a.z; pv[3] = a.w;
+            pv[4] = b.x; pv[5] = b.y; pv[6] = b.z; pv[7] = b.w;
+        }
+        reconstruct_side<LIM>(SL, SR, gL, f.dx, f.dy, 1.0, fz, pv, fL);
         if (boundary) {
             if (bc == BC_EXIT && Uexit) {
                 FState src;
@@ -382,7 +390,11 @@ __device__ __forceinline__ void flux_face(const DevMesh &m, const DevCfg &cfg,
             } else {
                 load_grad(grad, f.R, gR);
             }
-            reconstruct_side<LIM>(SR, SL, gR, f.dx, f.dy, -1.0, fz, fz ? pL + 4 * (size_t)m.nfe : nullptr, m.nfe, fR);
+            reconstruct_side<LIM>(SR, SL, gR, f.dx, f.dy, -1.0, fz, pv + 4, fR);
+        }
+        if (fz == 1) {
+            p4[0] = make_float4(pv[0], pv[1], pv[2], pv[3]);
+            p4[1] = make_float4(pv[4], pv[5], pv[6], pv[7]);
         }
         face_flux<FLUX>(boundary, cfg.roe_bdiss, fL, fR, nx, ny, dl, F);
     } else {

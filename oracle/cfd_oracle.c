@@ -934,7 +934,7 @@ void on_gradients(const OMesh *m, const ONatCfg *cfg, const double *W5, double *
  * which makes the iteration near a converged state differentiable (limiter switching at shocks
  * otherwise stalls steady convergence). */
 static void reconstruct_side(const ONatCfg *cfg, const FState *cellS, const FState *otherS,
-                             const double *g, double dx, double dy, double sgn, double *phi,
+                             const double *g, double dx, double dy, double sgn, float *phi,
                              FState *out) {
     double Wf[4];
     for (int k = 0; k < 4; k++) {
@@ -942,11 +942,11 @@ static void reconstruct_side(const ONatCfg *cfg, const FState *cellS, const FSta
         double c = g[2 * k] * dx + g[2 * k + 1] * dy;
         double slope;
         if (cfg->freeze == 2) {
-            slope = phi[k] * c;
+            slope = (double)phi[k] * c;
         } else {
             double dminus = 2.0 * c - dplus;
             slope = or_limiter_slope(cfg->limiter, dminus, dplus);
-            if (cfg->freeze == 1) phi[k] = (c != 0.0) ? slope / c : 0.0;
+            if (cfg->freeze == 1) phi[k] = (float)((c != 0.0) ? slope / c : 0.0);
         }
         Wf[k] = cellS->W[k] + sgn * 0.5 * slope;
     }
@@ -996,7 +996,7 @@ void on_residual(const OMesh *m, const ONatCfg *cfg, const double *U, const doub
             double dx = m->ctr[3 * Rc] - m->ctr[3 * L];
             double dy = m->ctr[3 * Rc + 1] - m->ctr[3 * L + 1];
             FState fL, fR;
-            double *phiL = cfg->phi ? &cfg->phi[8 * (size_t)fi] : NULL;
+            float *phiL = cfg->phi ? &cfg->phi[8 * (size_t)fi] : NULL;
             reconstruct_side(cfg, &SL, &SR, &grad[8 * L], dx, dy, 1.0, phiL, &fL);
             if (boundary) {
                 FState src = fL;

@@ -64,9 +64,9 @@ typedef struct {
     double cfl;
     int bc[6];       /* bc[kind] for face kinds 1..5 (LEFT, RIGHT, TOP, BOTTOM, WALL) */
     double fs[4];    /* prescribed primitive state (rho, u, v, p) for inlet/freestream/far field */
-    int freeze;      /* limiter freezing (order 2): 0 off, 1 record phi = slope / d+ into phi,
-                        2 use the recorded phi (slope = phi * d+) */
-    double *phi;     /* [8 * faces]: per face, side L (4 variables) then side R */
+    int freeze;      /* limiter freezing (order 2): 0 off, 1 record phi = slope / (grad W . d),
+                        rounded to FP32, into phi; 2 use the recorded phi (slope = phi * grad W . d) */
+    float *phi;      /* [8 * faces]: per face, side L (4 variables) then side R */
 } ONatCfg;
 
 /* ---- primitive / ghost helpers --------------------------------------------------------- */

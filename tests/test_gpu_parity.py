@@ -228,7 +228,7 @@ def test_limiter_freezing_vs_oracle(orc, bump, flux, lim, grad):
     s.set_state(U0)
     U = np.ascontiguousarray(U0.copy())
     ocfg = make_cfg(**kw)
-    phi = np.zeros(8 * om.nfaces)
+    phi = np.zeros(8 * om.nfaces, np.float32)
     ocfg.phi = phi.ctypes.data
     for mode, n in ((0, 3), (1, 1), (2, 4)):
         s.set_freeze(mode)

@@ -481,17 +481,19 @@ def test_limiter_freezing_record_step_and_bounds(orc):
     U0 = np.ascontiguousarray(np.tile(cons_from_prim(*fs), (om.nphys, 1)))
     for _ in range(40):
         om.step(make_cfg(flux="ausm", order=2, limiter="vanleer", gradient="wlsq", cfl=0.8, bc=bc, fs=fs), U0, rk=3, lts=1)
-    phi = np.zeros(8 * om.nfaces)
+    phi = np.zeros(8 * om.nfaces, np.float32)
     cfg = make_cfg(flux="ausm", order=2, limiter="vanleer", gradient="wlsq", cfl=0.8, bc=bc, fs=fs)
     R0 = om.residual(cfg, U0)[0]
     cfg.phi = phi.ctypes.data
     cfg.freeze = 1
     R1 = om.residual(cfg, U0)[0]
     assert np.array_equal(R0, R1)
-    assert phi.min() >= 0.0 and phi.max() <= 1.0 + 1e-15 and (phi > 0).any()
+    assert phi.min() >= 0.0 and phi.max() <= 1.0 and (phi > 0).any()
     cfg.freeze = 2
     R2 = om.residual(cfg, U0)[0]
-    assert np.allclose(R2, R0, rtol=0, atol=1e-13 * np.abs(R0).max())
+    # the factors are stored in FP32 (relative rounding 6e-8), so the frozen residual matches the
+    # evaluated one to that level
+    assert np.allclose(R2, R0, rtol=0, atol=1e-6 * np.abs(R0).max())
 
 
 def test_limiter_freezing_converges_where_switching_stalls(orc):
@@ -505,7 +507,7 @@ def test_limiter_freezing_converges_where_switching_stalls(orc):
     U = np.ascontiguousarray(np.tile(cons_from_prim(*fs), (om.nphys, 1)))
     e = [om.step(cfg, U, rk=3, lts=1)[0][0] for _ in range(700)]
     assert min(e[-200:]) > 1e-6  # stalled
-    phi = np.zeros(8 * om.nfaces)
+    phi = np.zeros(8 * om.nfaces, np.float32)
     cfg.phi = phi.ctypes.data
     cfg.freeze = 1
     om.step(cfg, U, rk=3, lts=1)
