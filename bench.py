@@ -53,7 +53,8 @@ def parse():
     p.add_argument("--warmup", type=int, default=5)
     p.add_argument("--workload", default="bump", choices=["bump", "ffs", "batch", "sod", "steady"])
     p.add_argument("--thickness", type=float, default=0.04, help="steady workload: bump thickness")
-    p.add_argument("--cfl", type=float, default=1.2, help="steady workload: Courant number")
+    p.add_argument("--cfl", type=float, default=None,
+                   help="steady and batch workloads: Courant number (default 1.2)")
     p.add_argument("--steady-max-steps", type=int, default=200000, help="steady workload: step limit")
     p.add_argument("--warmstart", type=int, default=1,
                    help="1 (default): also measure time to Error[0] <= 1e-8 of batched bump-family solves, "
@@ -266,7 +267,8 @@ def steady_bench(args):
 
     m = meshgen.bump_mesh(1400, 700, args.thickness, seed=1)
     fs = freestream(1.65)
-    cfg = dict(flux="hllc", order=2, limiter="vanleer", gradient="wlsq", cfl=args.cfl, rk=3, lts=1,
+    cfl = args.cfl if args.cfl is not None else 1.2
+    cfg = dict(flux="hllc", order=2, limiter="vanleer", gradient="wlsq", cfl=cfl, rk=3, lts=1,
                bc=("freestream", "exit", "wall", "wall", "wall"), fs=fs, freeze_stall=300)
     mesh = Mesh(m.points, m.cells, tile=128, renumber=1, face_budget=256)
     s = Solver(mesh, make_config(roe_bdiss=1, block=128, xcd_remap=1, **cfg))
@@ -287,7 +289,7 @@ def steady_bench(args):
         "steps": done, "warmup": 0, "ms_per_step": 1e3 * wall / max(done, 1), "higher_is_better": False,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (freestream initial state)",
         "config": {"workload": f"bump_1400x700_mixed_t{args.thickness:g}_M1.65_hllc_muscl_vanleer_wlsq_lts_rk3"
-                               f"_cfl{args.cfl:g}", "cells": mesh.nphys},
+                               f"_cfl{cfl:g}", "cells": mesh.nphys},
         "converged": bool(ok), "error": [float(x) for x in err],
     }
     print(json.dumps(out), flush=True)
@@ -303,7 +305,8 @@ def batch_bench(args):
 
     t = time.perf_counter()
     cs = cases.ramp_batch(args.instances, min_cells=args.min_cells, max_cells=args.max_cells)
-    cfg = make_config(flux="ausm", order=2, limiter="vanleer", gradient="wlsq", cfl=0.8, rk=3, lts=1,
+    cfl = args.cfl if args.cfl is not None else 1.2
+    cfg = make_config(flux="ausm", order=2, limiter="vanleer", gradient="wlsq", cfl=cfl, rk=3, lts=1,
                       roe_bdiss=1, bc=cases.RAMP_BCS, block=0, freeze_stall=300)
     b = Batch([c.mesh for c in cs], np.array([c.fs for c in cs]), cfg)
     info = b.info()
@@ -320,7 +323,7 @@ def batch_bench(args):
         "value": r["wall_s"], "unit": "s", "n_gpus": 1, "steps": int(r["steps"].max()), "warmup": 0,
         "ms_per_step": ms / args.steps, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic (seeded ramp family, freestream initial states)",
-        "config": {"workload": f"ramp_batch_{args.instances}x[{args.min_cells},{args.max_cells}]_ausm_muscl_vanleer_wlsq_lts_rk3",
+        "config": {"workload": f"ramp_batch_{args.instances}x[{args.min_cells},{args.max_cells}]_ausm_muscl_vanleer_wlsq_lts_rk3_cfl{cfl:g}",
                    "cells": info["cells"], "tiles": info["tiles"], "autotuned": {k: info[k] for k in ("tile", "face_budget", "block")}},
         "cell_updates_per_s": info["cells"] / (ms / args.steps / 1e3),
         "converged": int((r["status"] == 0).sum()), "instances": args.instances,
