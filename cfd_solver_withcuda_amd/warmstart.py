@@ -8,7 +8,9 @@
      surrogate's field (warm, with per-instance fallback to freestream when the warm start's
      first-step residual exceeds the freestream start's). Both marches run to Error[0] <= tol.
 
-The plain FP64 march stays the judge; the surrogate only moves its starting point.
+Warm-started instances record their limiter factors on the first step and march frozen (their
+shocks are already in place); a frozen march whose error grows tenfold returns to the evaluated
+limiter. The plain FP64 march stays the judge; the surrogate only moves its starting point.
 """
 from __future__ import annotations
 
