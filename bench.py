@@ -177,7 +177,12 @@ def warmstart_bench(args):
             "converged_fields_rel_l2_diff": r["converged_fields_rel_l2_diff"],
             "training": {"cases": r["train"]["cases"], "solve_s": r["train"]["wall_s"],
                          "mlp_steps": r["mlp"]["steps"], "mlp_s": r["mlp"]["train_s"],
-                         "loss_first": r["mlp"]["loss_first"], "loss_last": r["mlp"]["loss_last"]},
+                         "loss_first": r["mlp"]["loss_first"], "loss_last": r["mlp"]["loss_last"],
+                         # forward + backward of the 331,268-parameter MLP: 6 flops per parameter
+                         # and row, against the dense BF16 MFMA peak of 2.5 PFLOP/s
+                         "mlp_tflops": 6 * 331268 * r["mlp"]["batch"] * r["mlp"]["steps"] / r["mlp"]["train_s"] / 1e12,
+                         "mlp_frac_of_bf16_peak": 6 * 331268 * r["mlp"]["batch"] * r["mlp"]["steps"]
+                                                  / r["mlp"]["train_s"] / 2.5e15},
             "scheme": "HLLC + MUSCL/Van Leer + WLSQ, LTS TVD-RK3 at CFL 1.2, limiter freezing on stall",
             "note": "plain_s / warm_s: wall time until every instance converged or reached the step limit; "
                     "both_converged: the instances both marches brought to 1e-8, time until the last of them"}
