@@ -864,7 +864,7 @@ __global__ void __launch_bounds__(256) batch_status_kernel(const int *__restrict
         }
         steps[i] += 1;
         // Error[1..3] are infinite wherever a momentum component is exactly zero (the reference's
-        // relative norm, src/Error_Estimate_Update.cpp:10-30); only Error[0] judges the march
+        // relative norm, src/Error_Estimate_Update.cpp:10-30); only Error[0] decides the march
         if (!isfinite(e[0])) status[i] = -1;
         else if (e[0] <= tol) status[i] = 0;
         if (freeze) {

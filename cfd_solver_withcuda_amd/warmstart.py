@@ -10,7 +10,7 @@
 
 Warm-started instances record their limiter factors on the first step and march frozen (their
 shocks are already in place); a frozen march whose error grows tenfold returns to the evaluated
-limiter. The plain FP64 march stays the judge; the surrogate only moves its starting point.
+limiter. The plain FP64 march decides the answer; the surrogate only moves its starting point.
 """
 from __future__ import annotations
 
