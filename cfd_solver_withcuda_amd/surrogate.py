@@ -2,7 +2,7 @@
 (rho, u, v, p) trained on the solver's own steady solutions, on libcfdhip.so's BF16-MFMA kernels
 (cfd_mlp_*). The reference has no counterpart. The plain march stays the arbiter: the predicted
 field only initialises it, and a start whose first-step residual exceeds the freestream start's
-falls back to freestream (see `warm_states`).
+falls back to freestream (the fallback test is in warmstart.run).
 """
 from __future__ import annotations
 
