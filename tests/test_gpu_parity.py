@@ -72,6 +72,23 @@ def test_residual_bit_exact_vs_oracle(orc, bump, flux, order, lim, grad):
     assert np.array_equal(R, Ro), rel(R, Ro)
 
 
+@pytest.mark.parametrize("tri_frac,jitter", [(0.0, 0.0), (0.0, 0.15), (1.0, 0.1), (0.5, 0.1)])
+@pytest.mark.parametrize("flux,grad", [("hllc", "wlsq"), ("roe", "gg"), ("ausm", "wlsq"), ("llf", "gg")])
+def test_residual_on_quad_triangle_and_mixed_meshes(orc, tri_frac, jitter, flux, grad):
+    """Uniform and jittered quads, all triangles, and half split: the second-order residual,
+    local dt and gradients equal the oracle's bit for bit whatever the cell shapes."""
+    print(f"case tri_frac={tri_frac} jitter={jitter} flux={flux} grad={grad}", flush=True)
+    m = meshgen.shock_tube_mesh(36, 18, length=2.0, height=1.0, seed=13, jitter=jitter, tri_frac=tri_frac)
+    om = orc.mesh(m.points, m.cells)
+    U = smooth_state(om.arrays().ctr, om.nphys)
+    kw = dict(flux=flux, order=2, limiter="vanleer", gradient=grad, cfl=0.7, bc=BCS_NATIVE, fs=FS)
+    Ro, dto, go = om.residual(make_cfg(roe_bdiss=1, **kw), U)
+    s = Solver(Mesh(m.points, m.cells, tile=64, renumber=1), make_config(roe_bdiss=1, **kw))
+    s.set_state(U)
+    R, dt, g = s.residual()
+    assert np.array_equal(dt, dto) and np.array_equal(g, go) and np.array_equal(R, Ro), rel(R, Ro)
+
+
 @pytest.mark.parametrize("tile,renumber,block,remap", [(32, 0, 64, 0), (128, 1, 128, 1), (256, 1, 256, 1)])
 def test_residual_independent_of_tiling(orc, bump, tile, renumber, block, remap):
     om = orc.mesh(bump.points, bump.cells)
