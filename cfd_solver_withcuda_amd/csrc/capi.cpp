@@ -51,6 +51,27 @@ namespace cfd {
 // Error reporting shared with the other C-ABI translation units (one message per thread).
 int capi_fail(int code, const std::string &msg) { return fail(code, msg); }
 int capi_hip_fail(hipError_t e, const char *where) { return hip_fail(e, where); }
+
+// Viscous settings shared by single and batched solvers: validation and the device copy.
+int capi_check_viscous(const cfd_config_t *c, const char *who) {
+    if (c->viscous < 0 || c->viscous > 2) return fail(CFD_E_ARG, std::string(who) + ": viscous must be 0, 1 or 2");
+    if (!c->viscous) return 0;
+    if (c->order != 2) return fail(CFD_E_ARG, std::string(who) + ": viscous terms need order 2 (cell gradients)");
+    if (c->ref_mode) return fail(CFD_E_ARG, std::string(who) + ": viscous terms exist in the native iteration only");
+    if (!(c->prandtl > 0.0) || !std::isfinite(c->prandtl)) return fail(CFD_E_ARG, std::string(who) + ": prandtl must be positive");
+    if (!(c->r_gas > 0.0) || !std::isfinite(c->r_gas)) return fail(CFD_E_ARG, std::string(who) + ": r_gas must be positive");
+    if (c->viscous == 1 && (!(c->mu >= 0.0) || !std::isfinite(c->mu)))
+        return fail(CFD_E_ARG, std::string(who) + ": mu must be finite and >= 0");
+    return 0;
+}
+
+void capi_fill_viscous(DevCfg &d, const cfd_config_t &c) {
+    d.viscous = c.viscous;
+    d.mu = c.viscous ? c.mu : 0.0;
+    d.r_gas = c.viscous ? c.r_gas : 1.0;
+    d.kfac = c.viscous ? 1.4 * c.r_gas / ((1.4 - 1.0) * c.prandtl) : 0.0;
+    d.inv_pr = c.viscous ? 1.0 / c.prandtl : 0.0;
+}
 }  // namespace cfd
 
 struct cfd_mesh_s {
@@ -280,7 +301,8 @@ static int check_config(const cfd_mesh_s *m, const cfd_config_t *c) {
     if (c->ref_mode && c->rk == 4) return fail(CFD_E_ARG, "config: the reference iteration has no RK4");
     if (c->block != 64 && c->block != 128 && c->block != 256) return fail(CFD_E_ARG, "config: block must be 64, 128 or 256");
     for (int k = 1; k <= 5; k++)
-        if (c->bc[k] < 1 || c->bc[k] > 7) return fail(CFD_E_ARG, "config: bc types must be 1..7");
+        if (c->bc[k] < 1 || c->bc[k] > 8) return fail(CFD_E_ARG, "config: bc types must be 1..8");
+    if (int rc = capi_check_viscous(c, "config")) return rc;
     if (m->dev.tile > c->block) return fail(CFD_E_ARG, "config: the mesh tile (cells per tile) must not exceed the block size");
     const size_t sm = flux_smem_bytes(m->dev, c->order, c->grad_lds);
     if (sm > 65536 || grad_smem_bytes(m->dev, c->gradient) > 65536)
@@ -299,6 +321,7 @@ int cfd_solver_configure(cfd_solver_t *s, const cfd_config_t *cfg) {
     s->dcfg.roe_bdiss = cfg->roe_bdiss;
     s->dcfg.xcd_remap = cfg->xcd_remap;
     s->dcfg.grad_lds = cfg->grad_lds;
+    capi_fill_viscous(s->dcfg, *cfg);
     s->dt_valid = false;
     return 0;
 }
@@ -421,6 +444,10 @@ static int ref_roe2o_pass(cfd_solver_s *s, const double4 *in, double *dtmin, con
     return 0;
 }
 
+// Order of the pass that forms the first global time step: the inviscid time step does not
+// depend on the order, so a first-order pass suffices; the viscous term exists at order 2 only.
+static int dt_pass_order(const cfd_config_t &c) { return c.viscous ? 2 : 1; }
+
 // residual of state `in` into s->R, s->dt (+ minimum dt into *dtmin when set)
 static int residual_pass(cfd_solver_s *s, const double4 *in, double *dtmin, const double4 *uexit,
                          hipStream_t st, int order_override = 0) {
@@ -514,7 +541,7 @@ static int native_step(cfd_solver_s *s, bool want_err, hipStream_t st) {
     }
     const DevMesh &dm = s->m->dev;
     if (!c.lts && !s->dt_valid) {
-        int rc = residual_pass(s, s->Un, s->scal + s->cur, nullptr, st, 1);
+        int rc = residual_pass(s, s->Un, s->scal + s->cur, nullptr, st, dt_pass_order(c));
         if (rc) return rc;
         s->dt_valid = true;
     }
@@ -606,7 +633,7 @@ int cfd_time_pass(cfd_solver_t *s, int which, int reps, float *ms, void *stream)
     const cfd_config_t &c = s->cfg;
     const DevMesh &dm = s->m->dev;
     if (!c.lts && !s->dt_valid) {
-        int rc = residual_pass(s, s->Un, s->scal + s->cur, nullptr, st, 1);
+        int rc = residual_pass(s, s->Un, s->scal + s->cur, nullptr, st, dt_pass_order(c));
         if (rc) return rc;
         s->dt_valid = true;
     }

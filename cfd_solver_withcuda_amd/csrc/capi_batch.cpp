@@ -28,6 +28,8 @@ using namespace cfd;
 namespace cfd {
 int capi_fail(int code, const std::string &msg);
 int capi_hip_fail(hipError_t e, const char *where);
+int capi_check_viscous(const cfd_config_t *c, const char *who);
+void capi_fill_viscous(DevCfg &d, const cfd_config_t &c);
 }  // namespace cfd
 
 namespace {
@@ -126,8 +128,8 @@ int check_batch_config(const cfd_config_t *c) {
         return capi_fail(CFD_E_ARG, "batch config: freeze_tol and freeze_stall must be >= 0");
     if (!c->lts) return capi_fail(CFD_E_ARG, "batch config: batched steady solves use local time stepping (lts = 1)");
     for (int k = 1; k <= 5; k++)
-        if (c->bc[k] < 1 || c->bc[k] > 7) return capi_fail(CFD_E_ARG, "batch config: bc types must be 1..7");
-    return 0;
+        if (c->bc[k] < 1 || c->bc[k] > 8) return capi_fail(CFD_E_ARG, "batch config: bc types must be 1..8");
+    return capi_check_viscous(c, "batch config");
 }
 
 // Builds connectivity and tiled layouts of the instances (in parallel), concatenates them on
@@ -290,6 +292,7 @@ int build_batch(const std::vector<InstanceInput> &in, const double *fs4, const c
     b->dcfg.roe_bdiss = cfg.roe_bdiss;
     b->dcfg.xcd_remap = cfg.xcd_remap;
     b->dcfg.grad_lds = cfg.grad_lds;
+    capi_fill_viscous(b->dcfg, cfg);
     const size_t sm = flux_smem_bytes(d, cfg.order, cfg.grad_lds);
     if (sm > 65536 || grad_smem_bytes(d, cfg.gradient) > 65536) {
         delete b;

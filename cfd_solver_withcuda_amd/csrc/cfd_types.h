@@ -13,7 +13,8 @@ enum BcType {
     BC_SYMMETRY = 4,            // same arithmetic as the wall
     BC_FREESTREAM = 5,          // all primitives prescribed
     BC_FARFIELD = 6,            // Riemann-invariant far field
-    BC_SUBSONIC_INLET_REF = 7   // the reference's subsonic inlet
+    BC_SUBSONIC_INLET_REF = 7,  // the reference's subsonic inlet
+    BC_WALL_NOSLIP = 8          // no-slip adiabatic wall: velocity negated, rho and p copied
 };
 // Reference face kinds (src/Read_Gmsh_File.cpp:318-323).
 enum FaceKind { KIND_INTERNAL = 0, KIND_LEFT = 1, KIND_RIGHT = 2, KIND_TOP = 3, KIND_BOTTOM = 4, KIND_WALL = 5 };
@@ -59,6 +60,12 @@ struct DevCfg {
     int roe_bdiss;
     int xcd_remap;   // map consecutive tiles onto one XCD
     int grad_lds;    // stage the tile's gradients in LDS (1) or read them through L2 (0)
+    // laminar viscous terms (second order only): 0 off, 1 constant viscosity mu, 2 Sutherland
+    int viscous;
+    double mu;       // dynamic viscosity (viscous == 1)
+    double r_gas;    // gas constant: T = p / (rho r_gas)
+    double kfac;     // conductivity / viscosity = cp / Pr = gamma r_gas / ((gamma - 1) Pr)
+    double inv_pr;   // 1 / Pr
 };
 
 }  // namespace cfd

@@ -88,6 +88,15 @@ __device__ __forceinline__ void ghost_state(int bc, const FState &in, double n1,
         Ug[3] = ((in.W[3] / kGm1) + in.W[0] * vmag);
         break;
     }
+    case BC_WALL_NOSLIP: {  // no-slip adiabatic wall: velocity negated, rho and p copied
+        double u = in.W[1], v = in.W[2];
+        double vmag = 0.5 * (u * u + v * v);
+        Ug[0] = in.W[0];
+        Ug[1] = in.W[0] * (-u);
+        Ug[2] = in.W[0] * (-v);
+        Ug[3] = ((in.W[3] / kGm1) + in.W[0] * vmag);
+        break;
+    }
     case BC_FREESTREAM:
         cons_from_prim(fs[0], fs[1], fs[2], fs[3], Ug);
         break;
@@ -394,6 +403,79 @@ __device__ __forceinline__ void reconstruct_side(const FState &c, const FState &
     out.U[1] = Wf[0] * Wf[1];
     out.U[2] = Wf[0] * Wf[2];
     out.U[3] = Wf[3] * kInvGm1 + Wf[0] * vmag;
+}
+
+// ------------------------------------------------------------------------------------------
+// Laminar viscous terms (2-D Navier-Stokes, Stokes hypothesis).
+// ------------------------------------------------------------------------------------------
+// Sutherland's law constants of CUDA_KERNELS/Viscous_Flux_Cuda_Kernels.cu:6-7 (air, SI units).
+constexpr double kSuthC1 = 1.45793265452e-06;
+constexpr double kSuthS = 110.4;
+
+// Dynamic viscosity at temperature T: the constant cfg.mu, or Sutherland's law
+// C1 T^1.5 / (T + S) (calculate_viscosity, Viscous_Flux_Cuda_Kernels.cu:12-14) with T^1.5 formed
+// as T sqrt(T), which is correctly rounded on every target.
+__device__ __forceinline__ double viscosity(const DevCfg &cfg, double T) {
+    if (cfg.viscous == 2) return kSuthC1 * (T * sqrt(T)) / (T + kSuthS);
+    return cfg.mu;
+}
+
+// Face gradient of one variable: the mean of the two cells' gradients with its component along
+// the centre-to-centre direction t replaced by the difference quotient (phi_R - phi_L) / |d|, so
+// that neighbouring values are coupled directly (no odd-even decoupling on the averaged stencil).
+__device__ __forceinline__ void face_gradient(double axL, double ayL, double axR, double ayR,
+                                              double phiL, double phiR, double inv_dist, double tx,
+                                              double ty, double &gx, double &gy) {
+    const double ax = 0.5 * (axL + axR);
+    const double ay = 0.5 * (ayL + ayR);
+    const double corr = (phiR - phiL) * inv_dist - (ax * tx + ay * ty);
+    gx = ax + corr * tx;
+    gy = ay + corr * ty;
+}
+
+// Viscous flux through a face times its length, from the cell-centre primitives WL, WR (WR the
+// ghost state on a boundary face) and the cells' gradients gL, gR of (rho, u, v, p) (gR = gL on a
+// boundary face); (dx, dy) = x_R - x_L. Stress and heat flux as Viscous_Flux_on_Face
+// (src/Grid_Refine_Functions.cpp:418-475): tau_xx = 2/3 mu (2 u_x - v_y), tau_xy = mu (u_y + v_x),
+// tau_yy = 2/3 mu (2 v_y - u_x), q = k grad T, face velocity and temperature the two cells' mean;
+// F_v = (0, tau_xx nx + tau_xy ny, tau_xy nx + tau_yy ny, (tau_x. u + q_x) nx + (tau_y. u + q_y) ny) dl.
+// Temperatures are T = p / (rho r_gas), grad T = (grad p - (p / rho) grad rho) / (rho r_gas).
+__device__ __forceinline__ void viscous_flux(const DevCfg &cfg, const double *WL, const double *WR,
+                                             const double *gL, const double *gR, double nx,
+                                             double ny, double dl, double dx, double dy,
+                                             double *Fv) {
+    const double rgL = WL[0] * cfg.r_gas, rgR = WR[0] * cfg.r_gas;
+    const double TL = WL[3] / rgL, TR = WR[3] / rgR;
+    const double qL = WL[3] / WL[0], qR = WR[3] / WR[0];
+    const double TxL = (gL[6] - qL * gL[0]) / rgL, TyL = (gL[7] - qL * gL[1]) / rgL;
+    const double TxR = (gR[6] - qR * gR[0]) / rgR, TyR = (gR[7] - qR * gR[1]) / rgR;
+    const double dist = sqrt(dx * dx + dy * dy);
+    const double inv_dist = 1.0 / dist;
+    const double tx = dx * inv_dist, ty = dy * inv_dist;
+    double ux, uy, vx, vy, Tx, Ty;
+    face_gradient(gL[2], gL[3], gR[2], gR[3], WL[1], WR[1], inv_dist, tx, ty, ux, uy);
+    face_gradient(gL[4], gL[5], gR[4], gR[5], WL[2], WR[2], inv_dist, tx, ty, vx, vy);
+    face_gradient(TxL, TyL, TxR, TyR, TL, TR, inv_dist, tx, ty, Tx, Ty);
+    const double mu = viscosity(cfg, 0.5 * (TL + TR));
+    const double k = mu * cfg.kfac;
+    const double t11 = (2.0 / 3.0) * mu * (2.0 * ux - vy);
+    const double t12 = mu * (uy + vx);
+    const double t22 = (2.0 / 3.0) * mu * (2.0 * vy - ux);
+    const double uf = 0.5 * (WL[1] + WR[1]), vf = 0.5 * (WL[2] + WR[2]);
+    const double Qx = k * Tx, Qy = k * Ty;
+    Fv[0] = 0.0;
+    Fv[1] = (t11 * nx + t12 * ny) * dl;
+    Fv[2] = (t12 * nx + t22 * ny) * dl;
+    Fv[3] = ((t11 * uf + t12 * vf + Qx) * nx * dl + (t12 * uf + t22 * vf + Qy) * ny * dl);
+}
+
+// Viscous part of a cell's time-step denominator, Viscous_Time_Step_1
+// (src/Time_Step.cpp:103-109) with the cell's own density and viscosity:
+// 4 (1/A) max(4 / (3 rho), gamma / rho) (mu / Pr) sum_f dl_f^2; dt = CFL A / (inviscid + this).
+__device__ __forceinline__ double viscous_dt_term(const DevCfg &cfg, double rho, double p,
+                                                  double sdl2, double area) {
+    const double mu = viscosity(cfg, p / (rho * cfg.r_gas));
+    return 4.0 * ((fmax(4.0 / (3.0 * rho), kGamma / rho) * mu * cfg.inv_pr * sdl2) / area);
 }
 
 }  // namespace cfd

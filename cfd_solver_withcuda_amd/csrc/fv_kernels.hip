@@ -309,7 +309,7 @@ __device__ __forceinline__ void load_grad(const double *__restrict__ grad, int c
     }
 }
 
-template <int FLUX, int ORDER, int LIM>
+template <int FLUX, int ORDER, int LIM, bool VISC>
 __device__ __forceinline__ void flux_face(const DevMesh &m, const DevCfg &cfg,
                                           const double4 *__restrict__ Us,
                                           const double *__restrict__ grad, const double4 *Uexit,
@@ -364,6 +364,7 @@ __device__ __forceinline__ void flux_face(const DevMesh &m, const DevCfg &cfg,
         } else {
             load_grad(grad, f.L, gL);
         }
+        double gR[8];
         FState fL, fR;
         // frozen limiter factors of this face entry: side L in pv[0..3], side R in pv[4..7]
         // (8 contiguous floats per entry, two 16-byte loads)
@@ -383,8 +384,9 @@ __device__ __forceinline__ void flux_face(const DevMesh &m, const DevCfg &cfg,
             } else {
                 ghost_state(bc, fL, nx, ny, fsv, fR);
             }
+            if constexpr (VISC)
+                for (int q = 0; q < 8; q++) gR[q] = gL[q];
         } else {
-            double gR[8];
             if (glds && locR) {
                 for (int q = 0; q < 8; q++) gR[q] = sG[q * T + lR];
             } else {
@@ -397,13 +399,19 @@ __device__ __forceinline__ void flux_face(const DevMesh &m, const DevCfg &cfg,
             p4[1] = make_float4(pv[4], pv[5], pv[6], pv[7]);
         }
         face_flux<FLUX>(boundary, cfg.roe_bdiss, fL, fR, nx, ny, dl, F);
+        if constexpr (VISC) {
+            // viscous flux from the cell-centre states (ghost on a boundary face), subtracted
+            double Fv[4];
+            viscous_flux(cfg, SL.W, SR.W, gL, gR, nx, ny, dl, f.dx, f.dy, Fv);
+            for (int q = 0; q < 4; q++) F[q] = F[q] - Fv[q];
+        }
     } else {
         face_flux<FLUX>(boundary, cfg.roe_bdiss, SL, SR, nx, ny, dl, F);
     }
     for (int q = 0; q < 4; q++) sF[q * FT + j] = F[q];
 }
 
-template <int FLUX, int ORDER, int LIM, int MODE>
+template <int FLUX, int ORDER, int LIM, int MODE, bool VISC>
 __global__ void __launch_bounds__(256) CFD_FLUX_ATTR flux_tile_kernel(DevMesh m, DevCfg cfg,
                                                         const double4 *__restrict__ Us,
                                                         const double *__restrict__ grad,
@@ -481,17 +489,17 @@ __global__ void __launch_bounds__(256) CFD_FLUX_ATTR flux_tile_kernel(DevMesh m,
     // ---- B: one flux per face ----------------------------------------------------------
     if (h0) {
         stageN(k, f0);
-        flux_face<FLUX, ORDER, LIM>(m, cfg, Us, grad, st.Uexit, sW, sU, sG, sF, glds, T, FT, c0, nct, k, f0, fz, st.psi, fs + k, fsv);
+        flux_face<FLUX, ORDER, LIM, VISC>(m, cfg, Us, grad, st.Uexit, sW, sU, sG, sF, glds, T, FT, c0, nct, k, f0, fz, st.psi, fs + k, fsv);
     }
     if (h1) {
         stageN(k + B, f1);
-        flux_face<FLUX, ORDER, LIM>(m, cfg, Us, grad, st.Uexit, sW, sU, sG, sF, glds, T, FT, c0, nct, k + B, f1, fz, st.psi, fs + k + B, fsv);
+        flux_face<FLUX, ORDER, LIM, VISC>(m, cfg, Us, grad, st.Uexit, sW, sU, sG, sF, glds, T, FT, c0, nct, k + B, f1, fz, st.psi, fs + k + B, fsv);
     }
     for (int j = k + 2 * B; j < nft; j += B) {
         FaceRec f;
         load_face<true, ORDER == 2>(m, fs + j, f);
         stageN(j, f);
-        flux_face<FLUX, ORDER, LIM>(m, cfg, Us, grad, st.Uexit, sW, sU, sG, sF, glds, T, FT, c0, nct, j, f, fz, st.psi, fs + j, fsv);
+        flux_face<FLUX, ORDER, LIM, VISC>(m, cfg, Us, grad, st.Uexit, sW, sU, sG, sF, glds, T, FT, c0, nct, j, f, fz, st.psi, fs + j, fsv);
     }
     __syncthreads();
 
@@ -517,14 +525,17 @@ __global__ void __launch_bounds__(256) CFD_FLUX_ATTR flux_tile_kernel(DevMesh m,
         double denom = 0.0;
         if (MODE == MODE_RESID || (st.lts && st.stage == 0)) {
             const double w1 = sW[1 * T + k], w2 = sW[2 * T + k], w4 = sW[4 * T + k];
+            double sdl2 = 0.0;
             for (int f = 0; f < 4; f++) {
                 const uint16_t sl = sls[f];
                 if (sl == 0xFFFF) break;
                 const int j = sl >> 1;
                 denom += (fabs(w1 * sN[j] + w2 * sN[FT + j]) + w4) * sN[2 * FT + j];
+                if constexpr (VISC) sdl2 += sN[2 * FT + j] * sN[2 * FT + j];
             }
+            if constexpr (VISC) denom += viscous_dt_term(cfg, sW[0 * T + k], sW[3 * T + k], sdl2, ar);
         }
-        // Inviscid_Time_Step (src/Time_Step.cpp:293-329)
+        // Inviscid_Time_Step (src/Time_Step.cpp:293-329), plus the viscous term
         const double dtc = (denom <= 0.0 || !isfinite(denom)) ? 0.0 : cfg.cfl * ar / denom;
         if constexpr (MODE == MODE_RESID) {
             reinterpret_cast<double4 *>(st.R_out)[c] = make_double4(r[0], r[1], r[2], r[3]);
@@ -575,13 +586,15 @@ __global__ void __launch_bounds__(256) CFD_FLUX_ATTR flux_tile_kernel(DevMesh m,
                     // time step of the new state, for the next step's global dt
                     FState s;
                     state_from_U(Uo[0], Uo[1], Uo[2], Uo[3], s);
-                    double den = 0.0;
+                    double den = 0.0, sdl2 = 0.0;
                     for (int f = 0; f < 4; f++) {
                         const uint16_t sl = sls[f];
                         if (sl == 0xFFFF) break;
                         const int j = sl >> 1;
                         den += (fabs(s.W[1] * sN[j] + s.W[2] * sN[FT + j]) + s.W[4]) * sN[2 * FT + j];
+                        if constexpr (VISC) sdl2 += sN[2 * FT + j] * sN[2 * FT + j];
                     }
+                    if constexpr (VISC) den += viscous_dt_term(cfg, s.W[0], s.W[3], sdl2, ar);
                     const double dtn = (den <= 0.0 || !isfinite(den)) ? 0.0 : cfg.cfl * ar / den;
                     dtmin = fmin(dtmin, dtn);
                 }
@@ -1030,10 +1043,19 @@ template <int FLUX, int ORDER, int LIM>
 static void launch_flux_t(const DevMesh &m, const DevCfg &c, int mode, const double4 *Us,
                           const double *grad, const StageArgs &st, int block, size_t sm,
                           hipStream_t s) {
+    if constexpr (ORDER == 2) {
+        if (c.viscous) {
+            if (mode == MODE_RESID)
+                hipLaunchKernelGGL((flux_tile_kernel<FLUX, ORDER, LIM, MODE_RESID, true>), dim3(grid_of(m, c)), dim3(block), sm, s, m, c, Us, grad, st);
+            else
+                hipLaunchKernelGGL((flux_tile_kernel<FLUX, ORDER, LIM, MODE_STAGE, true>), dim3(grid_of(m, c)), dim3(block), sm, s, m, c, Us, grad, st);
+            return;
+        }
+    }
     if (mode == MODE_RESID)
-        hipLaunchKernelGGL((flux_tile_kernel<FLUX, ORDER, LIM, MODE_RESID>), dim3(grid_of(m, c)), dim3(block), sm, s, m, c, Us, grad, st);
+        hipLaunchKernelGGL((flux_tile_kernel<FLUX, ORDER, LIM, MODE_RESID, false>), dim3(grid_of(m, c)), dim3(block), sm, s, m, c, Us, grad, st);
     else
-        hipLaunchKernelGGL((flux_tile_kernel<FLUX, ORDER, LIM, MODE_STAGE>), dim3(grid_of(m, c)), dim3(block), sm, s, m, c, Us, grad, st);
+        hipLaunchKernelGGL((flux_tile_kernel<FLUX, ORDER, LIM, MODE_STAGE, false>), dim3(grid_of(m, c)), dim3(block), sm, s, m, c, Us, grad, st);
 }
 
 template <int FLUX>

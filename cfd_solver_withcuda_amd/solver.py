@@ -20,7 +20,9 @@ from ._lib import CfdError, Config, check, dptr, iptr, lib, uptr
 FLUX = dict(roe=0, hllc=1, llf=2, ausm=3)
 LIMITER = dict(none=0, minmod=1, vanleer=2, superbee=3)
 GRADIENT = dict(gg=0, wlsq=1)
-BC = dict(inlet_ref=1, exit=2, wall=3, symmetry=4, freestream=5, farfield=6, subsonic_inlet_ref=7)
+BC = dict(inlet_ref=1, exit=2, wall=3, symmetry=4, freestream=5, farfield=6, subsonic_inlet_ref=7,
+          noslip_wall=8)
+VISCOUS = dict(off=0, constant=1, sutherland=2)
 
 # ghost types of the reference's face kinds (LEFT->Inlet, RIGHT->Exit, rest->Wall,
 # src/Read_Gmsh_File.cpp:579-621)
@@ -29,7 +31,10 @@ REF_BCS = ("inlet_ref", "exit", "wall", "wall", "wall")
 
 def make_config(flux="roe", order=1, limiter="vanleer", gradient="gg", roe_bdiss=0, cfl=0.5,
                 bc=REF_BCS, fs=(1.0, 0.0, 0.0, 1.0), rk=3, lts=0, ref_mode=0, block=256,
-                xcd_remap=1, grad_lds=1, freeze_tol=0.0, freeze_stall=0, zeta=0.5, zeta1=0.5) -> Config:
+                xcd_remap=1, grad_lds=1, freeze_tol=0.0, freeze_stall=0, zeta=0.5, zeta1=0.5,
+                viscous="off", mu=0.0, prandtl=0.72, r_gas=287.0) -> Config:
+    """Scheme settings (cfd_config_t). viscous: "off" (Euler), "constant" (viscosity mu) or
+    "sutherland" (Sutherland's law in T = p / (rho r_gas)); laminar viscous terms need order 2."""
     c = Config()
     c.flux = FLUX[flux]
     c.order = int(order)
@@ -54,6 +59,10 @@ def make_config(flux="roe", order=1, limiter="vanleer", gradient="gg", roe_bdiss
     c.freeze_stall = int(freeze_stall)
     c.zeta = float(zeta)
     c.zeta1 = float(zeta1)
+    c.viscous = VISCOUS[viscous] if isinstance(viscous, str) else int(viscous)
+    c.mu = float(mu)
+    c.prandtl = float(prandtl)
+    c.r_gas = float(r_gas)
     return c
 
 

@@ -44,7 +44,8 @@ typedef struct {
     int bc[6];     /* ghost type for face kinds 1..5 (LEFT, RIGHT, TOP, BOTTOM, WALL):
                       1 inlet (rho,P given, interior velocity), 2 exit (copy), 3 slip wall,
                       4 symmetry, 5 freestream, 6 characteristic far field,
-                      7 the reference's subsonic inlet */
+                      7 the reference's subsonic inlet, 8 no-slip adiabatic wall (velocity
+                      negated, rho and P copied) */
     double fs[4];  /* prescribed primitive state (rho, u, v, p) */
     int rk;        /* 1 forward Euler (reference mode), 3 TVD-RK3, 4 classical RK4 */
     int lts;       /* local time stepping */
@@ -63,6 +64,13 @@ typedef struct {
      * Evaluate_Cell_Net_Flux_2O (Roe dissipation on MinMod-limited differences,
      * src/Roe_Scheme.cpp:257-447, src/Limiters.cpp:97-200) with Limiter_Zeta / Limiter_Zeta1. */
     double zeta, zeta1;
+    /* Laminar viscous terms (order 2, native mode): 0 off (Euler), 1 constant viscosity mu,
+     * 2 Sutherland's law in T = p / (rho r_gas). Stress tensor and heat flux as the reference's
+     * Viscous_Flux_on_Face (src/Grid_Refine_Functions.cpp:418-475) with dimensional mu,
+     * conductivity k = mu gamma r_gas / ((gamma - 1) prandtl); the local time step adds the
+     * viscous spectral radius of Viscous_Time_Step_1 (src/Time_Step.cpp:103-109). */
+    int viscous;
+    double mu, prandtl, r_gas;
 } cfd_config_t;
 
 const char *cfd_last_error(void);

@@ -490,6 +490,16 @@ static void ghost_fstate(int bc, const FState *in, double n1, double n2, const d
         state_from_U(Ug, g);
         return;
     }
+    case OBC_WALL_NOSLIP: { /* no-slip adiabatic wall: velocity negated, rho and p copied */
+        double u = in->W[1], v = in->W[2];
+        double vmag = 0.5 * (u * u + v * v);
+        Ug[0] = in->W[0];
+        Ug[1] = in->W[0] * (-u);
+        Ug[2] = in->W[0] * (-v);
+        Ug[3] = ((in->W[3] / GM1) + in->W[0] * vmag);
+        state_from_U(Ug, g);
+        return;
+    }
     case OBC_EXIT: /* Supersonic_Exit_Boundary_Condition (src/Exit_Boundary_Conditions.cpp:68-81) */
         *g = *in;
         return;
@@ -969,6 +979,70 @@ static void reconstruct_side(const ONatCfg *cfg, const FState *cellS, const FSta
     out->U[3] = Wf[3] * (1.0 / GM1) + Wf[0] * vmag;
 }
 
+/* ------------------------------------------------------------------------------------------
+ * Laminar viscous terms
+ * ---------------------------------------------------------------------------------------- */
+/* Dynamic viscosity at temperature T: constant mu, or Sutherland's law C1 T^1.5 / (T + S) with the
+ * constants of CUDA_KERNELS/Viscous_Flux_Cuda_Kernels.cu:6-7,12-14 and T^1.5 as T sqrt(T). */
+static double viscosity(const ONatCfg *cfg, double T) {
+    if (cfg->viscous == 2) return 1.45793265452e-06 * (T * sqrt(T)) / (T + 110.4);
+    return cfg->mu;
+}
+
+/* Face gradient of one variable: mean of the two cell gradients, its component along the
+ * centre-to-centre unit vector t replaced by (phi_R - phi_L) / |d|. */
+static void face_gradient(double axL, double ayL, double axR, double ayR, double phiL, double phiR,
+                          double inv_dist, double tx, double ty, double *gx, double *gy) {
+    double ax = 0.5 * (axL + axR);
+    double ay = 0.5 * (ayL + ayR);
+    double corr = (phiR - phiL) * inv_dist - (ax * tx + ay * ty);
+    *gx = ax + corr * tx;
+    *gy = ay + corr * ty;
+}
+
+/* Viscous face flux times the face length (Viscous_Flux_on_Face,
+ * src/Grid_Refine_Functions.cpp:418-475, dimensional mu in place of mu / Re):
+ *   tau_xx = 2/3 mu (2 u_x - v_y), tau_xy = mu (u_y + v_x), tau_yy = 2/3 mu (2 v_y - u_x),
+ *   Fv = (0, tau_xx nx + tau_xy ny, tau_xy nx + tau_yy ny,
+ *         (tau_xx u + tau_xy v + k T_x) nx + (tau_xy u + tau_yy v + k T_y) ny) dl
+ * with face velocity, temperature and (directionally corrected) gradients from the two cells;
+ * T = p / (rho R), grad T = (grad p - (p / rho) grad rho) / (rho R), k = mu gamma R / ((gamma-1) Pr). */
+static void viscous_flux(const ONatCfg *cfg, const double *WL, const double *WR, const double *gL,
+                         const double *gR, double nx, double ny, double dl, double dx, double dy,
+                         double *Fv) {
+    double kfac = GAMMA * cfg->r_gas / ((GAMMA - 1.0) * cfg->prandtl);
+    double rgL = WL[0] * cfg->r_gas, rgR = WR[0] * cfg->r_gas;
+    double TL = WL[3] / rgL, TR = WR[3] / rgR;
+    double qL = WL[3] / WL[0], qR = WR[3] / WR[0];
+    double TxL = (gL[6] - qL * gL[0]) / rgL, TyL = (gL[7] - qL * gL[1]) / rgL;
+    double TxR = (gR[6] - qR * gR[0]) / rgR, TyR = (gR[7] - qR * gR[1]) / rgR;
+    double dist = sqrt(dx * dx + dy * dy);
+    double inv_dist = 1.0 / dist;
+    double tx = dx * inv_dist, ty = dy * inv_dist;
+    double ux, uy, vx, vy, Tx, Ty;
+    face_gradient(gL[2], gL[3], gR[2], gR[3], WL[1], WR[1], inv_dist, tx, ty, &ux, &uy);
+    face_gradient(gL[4], gL[5], gR[4], gR[5], WL[2], WR[2], inv_dist, tx, ty, &vx, &vy);
+    face_gradient(TxL, TyL, TxR, TyR, TL, TR, inv_dist, tx, ty, &Tx, &Ty);
+    double mu = viscosity(cfg, 0.5 * (TL + TR));
+    double k = mu * kfac;
+    double t11 = (2.0 / 3.0) * mu * (2.0 * ux - vy);
+    double t12 = mu * (uy + vx);
+    double t22 = (2.0 / 3.0) * mu * (2.0 * vy - ux);
+    double uf = 0.5 * (WL[1] + WR[1]), vf = 0.5 * (WL[2] + WR[2]);
+    double Qx = k * Tx, Qy = k * Ty;
+    Fv[0] = 0.0;
+    Fv[1] = (t11 * nx + t12 * ny) * dl;
+    Fv[2] = (t12 * nx + t22 * ny) * dl;
+    Fv[3] = ((t11 * uf + t12 * vf + Qx) * nx * dl + (t12 * uf + t22 * vf + Qy) * ny * dl);
+}
+
+/* Viscous part of the time-step denominator, Viscous_Time_Step_1 (src/Time_Step.cpp:103-109)
+ * with the cell's own density and viscosity: 4 (1/A) max(4/(3 rho), gamma/rho) (mu/Pr) sum dl^2. */
+static double viscous_dt_term(const ONatCfg *cfg, double rho, double p, double sdl2, double area) {
+    double mu = viscosity(cfg, p / (rho * cfg->r_gas));
+    return 4.0 * ((fmax(4.0 / (3.0 * rho), GAMMA / rho) * mu * (1.0 / cfg->prandtl) * sdl2) / area);
+}
+
 void on_residual(const OMesh *m, const ONatCfg *cfg, const double *U, const double *U_exit_src,
                  double *R, double *dt, double *grad) {
     const int nc = m->nc;
@@ -1006,6 +1080,13 @@ void on_residual(const OMesh *m, const ONatCfg *cfg, const double *U, const doub
                 reconstruct_side(cfg, &SR, &SL, &grad[8 * Rc], dx, dy, -1.0, phiL ? phiL + 4 : NULL, &fR);
             }
             face_flux(cfg, boundary, &fL, &fR, nx, ny, dl, &Ff[4 * fi]);
+            if (cfg->viscous) {
+                /* cell-centre states (ghost on a boundary face, with the cell's own gradient) */
+                double Fv[4];
+                const double *gR = boundary ? &grad[8 * L] : &grad[8 * Rc];
+                viscous_flux(cfg, SL.W, SR.W, &grad[8 * L], gR, nx, ny, dl, dx, dy, Fv);
+                for (int k = 0; k < 4; k++) Ff[4 * fi + k] = Ff[4 * fi + k] - Fv[k];
+            }
         } else {
             face_flux(cfg, boundary, &SL, &SR, nx, ny, dl, &Ff[4 * fi]);
         }
@@ -1020,11 +1101,13 @@ void on_residual(const OMesh *m, const ONatCfg *cfg, const double *U, const doub
         for (int k = 0; k < 4; k++) R[4 * c + k] = r[k];
         /* Inviscid_Time_Step (src/Time_Step.cpp:293-329) */
         double u0 = S[c].W[1], v0 = S[c].W[2], a0 = S[c].W[4];
-        double denom = 0.0;
+        double denom = 0.0, sdl2 = 0.0;
         for (int f = 0; f < m->nf[c]; f++) {
             double vn = u0 * m->nrm[8 * c + 2 * f] + v0 * m->nrm[8 * c + 2 * f + 1];
             denom += (fabs(vn) + a0) * m->len[4 * c + f];
+            sdl2 += m->len[4 * c + f] * m->len[4 * c + f];
         }
+        if (cfg->order == 2 && cfg->viscous) denom += viscous_dt_term(cfg, S[c].W[0], S[c].W[3], sdl2, m->area[c]);
         dt[c] = (denom <= 0.0 || !isfinite(denom)) ? 0.0 : cfg->cfl * m->area[c] / denom;
     }
     free(Ff);

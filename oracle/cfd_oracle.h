@@ -49,7 +49,8 @@ enum {
     OBC_SYMMETRY = 4,   /* same arithmetic as the reference symmetry condition */
     OBC_FREESTREAM = 5, /* all four primitives prescribed */
     OBC_FARFIELD = 6,   /* characteristic (Riemann-invariant) far field */
-    OBC_SUBSONIC_INLET_REF = 7 /* reference subsonic inlet, incl. its energy expression */
+    OBC_SUBSONIC_INLET_REF = 7, /* reference subsonic inlet, incl. its energy expression */
+    OBC_WALL_NOSLIP = 8 /* no-slip adiabatic wall: velocity negated, rho and p copied */
 };
 enum { OFLUX_ROE = 0, OFLUX_HLLC = 1, OFLUX_LLF = 2, OFLUX_AUSM = 3 };
 enum { OLIM_NONE = 0, OLIM_MINMOD = 1, OLIM_VANLEER = 2, OLIM_SUPERBEE = 3 };
@@ -67,6 +68,10 @@ typedef struct {
     int freeze;      /* limiter freezing (order 2): 0 off, 1 record phi = slope / (grad W . d),
                         rounded to FP32, into phi; 2 use the recorded phi (slope = phi * grad W . d) */
     float *phi;      /* [8 * faces]: per face, side L (4 variables) then side R */
+    int viscous;     /* laminar viscous terms (order 2): 0 off, 1 constant mu, 2 Sutherland */
+    double mu;       /* dynamic viscosity (viscous == 1) */
+    double prandtl;  /* Prandtl number: conductivity k = mu gamma r_gas / ((gamma - 1) Pr) */
+    double r_gas;    /* gas constant: T = p / (rho r_gas) */
 } ONatCfg;
 
 /* ---- primitive / ghost helpers --------------------------------------------------------- */

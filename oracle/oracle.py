@@ -23,7 +23,9 @@ REF_SO = os.path.join(HERE, "_ref", "libcfdref.so")
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
 
-OBC = dict(inlet_ref=1, exit=2, wall=3, symmetry=4, freestream=5, farfield=6, subsonic_inlet_ref=7)
+OBC = dict(inlet_ref=1, exit=2, wall=3, symmetry=4, freestream=5, farfield=6, subsonic_inlet_ref=7,
+           noslip_wall=8)
+OVISC = dict(off=0, constant=1, sutherland=2)
 OFLUX = dict(roe=0, hllc=1, llf=2, ausm=3)
 OLIM = dict(none=0, minmod=1, vanleer=2, superbee=3)
 OGRAD = dict(gg=0, wlsq=1)
@@ -41,11 +43,13 @@ class ONatCfg(ctypes.Structure):
     _fields_ = [("flux", ctypes.c_int), ("order", ctypes.c_int), ("limiter", ctypes.c_int),
                 ("gradient", ctypes.c_int), ("roe_bdiss", ctypes.c_int), ("cfl", ctypes.c_double),
                 ("bc", ctypes.c_int * 6), ("fs", ctypes.c_double * 4), ("freeze", ctypes.c_int),
-                ("phi", ctypes.c_void_p)]
+                ("phi", ctypes.c_void_p), ("viscous", ctypes.c_int), ("mu", ctypes.c_double),
+                ("prandtl", ctypes.c_double), ("r_gas", ctypes.c_double)]
 
 
 def make_cfg(flux="roe", order=1, limiter="vanleer", gradient="gg", roe_bdiss=0, cfl=0.5,
-             bc=("inlet_ref", "exit", "wall", "wall", "wall"), fs=(1.0, 0.0, 0.0, 1.0)) -> ONatCfg:
+             bc=("inlet_ref", "exit", "wall", "wall", "wall"), fs=(1.0, 0.0, 0.0, 1.0), viscous="off",
+             mu=0.0, prandtl=0.72, r_gas=287.0) -> ONatCfg:
     c = ONatCfg()
     c.flux = OFLUX[flux]
     c.order = order
@@ -58,6 +62,10 @@ def make_cfg(flux="roe", order=1, limiter="vanleer", gradient="gg", roe_bdiss=0,
         c.bc[k + 1] = OBC[name]
     for k in range(4):
         c.fs[k] = fs[k]
+    c.viscous = OVISC[viscous] if isinstance(viscous, str) else int(viscous)
+    c.mu = mu
+    c.prandtl = prandtl
+    c.r_gas = r_gas
     return c
 
 

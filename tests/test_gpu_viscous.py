@@ -1,0 +1,156 @@
+"""GPU parity of the laminar viscous terms (viscous = constant mu or Sutherland) against the CPU
+oracle: residual, local time step and gradients bit for bit on a bump mesh with no-slip walls and
+on triangle/mixed meshes; fused RK3/RK4 steps bit for bit; batched solves bit for bit as single
+solvers; invalid viscous settings rejected; a shear wave decays at the viscous rate."""
+import numpy as np
+import pytest
+
+from cfd_solver_withcuda_amd import meshgen
+from cfd_solver_withcuda_amd.batch import Batch
+from cfd_solver_withcuda_amd.solver import CfdError, Mesh, Solver, cons_from_prim, freestream, make_config
+from oracle.oracle import Oracle, make_cfg
+
+pytestmark = pytest.mark.gpu
+
+GAMMA = 1.4
+FS = freestream(0.5)  # rho 1.4, p 1, Mach 0.5
+VBCS = ("freestream", "exit", "symmetry", "noslip_wall", "noslip_wall")
+# viscous settings: constant mu (Reynolds ~ 500 on the unit bump), and Sutherland's law with a gas
+# constant that puts the freestream at 300 K
+VISC = {"constant": dict(viscous="constant", mu=1.5e-3, prandtl=0.72, r_gas=1.0),
+        "sutherland": dict(viscous="sutherland", mu=0.0, prandtl=0.72, r_gas=1.0 / (1.4 * 300.0))}
+
+
+@pytest.fixture(scope="module")
+def orc():
+    return Oracle()
+
+
+def shear_state(ctr, n):
+    x, y = ctr[:n, 0], ctr[:n, 1]
+    rho = 1.4 * (1 + 0.05 * np.sin(3 * x) * np.cos(2 * y))
+    u = FS[1] * np.tanh(8 * y) * (1 + 0.05 * np.cos(4 * x))
+    v = 0.02 * np.sin(2 * x) * y
+    p = 1.0 + 0.05 * np.cos(5 * x + y)
+    return cons_from_prim(rho, u, v, p)
+
+
+def rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+CASES = [("hllc", "vanleer", "wlsq", "constant"), ("roe", "minmod", "gg", "constant"),
+         ("ausm", "superbee", "wlsq", "sutherland"), ("llf", "vanleer", "gg", "sutherland")]
+
+
+@pytest.mark.parametrize("flux,lim,grad,visc", CASES)
+def test_viscous_residual_bit_exact_vs_oracle(orc, flux, lim, grad, visc):
+    print(f"case bump viscous={visc} flux={flux} lim={lim} grad={grad}", flush=True)
+    m = meshgen.bump_mesh(40, 20, 0.1, seed=7)
+    om = orc.mesh(m.points, m.cells)
+    U = shear_state(om.arrays().ctr, om.nphys)
+    kw = dict(flux=flux, order=2, limiter=lim, gradient=grad, roe_bdiss=1, cfl=0.7, bc=VBCS, fs=FS, **VISC[visc])
+    Ro, dto, go = om.residual(make_cfg(**kw), U)
+    s = Solver(Mesh(m.points, m.cells, tile=64, renumber=1), make_config(**kw))
+    s.set_state(U)
+    R, dt, g = s.residual()
+    assert np.array_equal(g, go), rel(g, go)
+    assert np.array_equal(dt, dto), rel(dt, dto)
+    assert np.array_equal(R, Ro), rel(R, Ro)
+    # the viscous terms are not negligible here: the residual differs from the Euler one
+    kw0 = {k: v for k, v in kw.items() if k not in ("viscous", "mu", "prandtl", "r_gas")}
+    R0, dt0, _ = om.residual(make_cfg(**kw0), U)
+    if visc == "constant":
+        assert rel(R, R0) > 1e-4 and np.all(dt < dt0)
+
+
+@pytest.mark.parametrize("tri_frac,jitter", [(1.0, 0.1), (0.5, 0.15)])
+def test_viscous_residual_on_triangle_and_mixed_meshes(orc, tri_frac, jitter):
+    print(f"case viscous tri_frac={tri_frac} jitter={jitter}", flush=True)
+    m = meshgen.shock_tube_mesh(36, 18, length=2.0, height=1.0, seed=13, jitter=jitter, tri_frac=tri_frac)
+    om = orc.mesh(m.points, m.cells)
+    U = shear_state(om.arrays().ctr, om.nphys)
+    kw = dict(flux="hllc", order=2, limiter="vanleer", gradient="wlsq", roe_bdiss=1, cfl=0.7,
+              bc=("freestream", "exit", "noslip_wall", "noslip_wall", "noslip_wall"), fs=FS, **VISC["constant"])
+    Ro, dto, go = om.residual(make_cfg(**kw), U)
+    s = Solver(Mesh(m.points, m.cells, tile=128, face_budget=256), make_config(**kw))
+    s.set_state(U)
+    R, dt, g = s.residual()
+    assert np.array_equal(g, go) and np.array_equal(dt, dto)
+    assert np.array_equal(R, Ro), rel(R, Ro)
+
+
+@pytest.mark.parametrize("flux,rk,lts,visc", [("hllc", 3, 1, "constant"), ("roe", 4, 0, "constant"),
+                                              ("ausm", 3, 0, "sutherland")])
+def test_viscous_steps_vs_oracle(orc, flux, rk, lts, visc):
+    """Fused stages with viscous terms: state bit-exact after 6 steps, error norms to 1e-12."""
+    print(f"case viscous steps flux={flux} rk={rk} lts={lts} viscous={visc}", flush=True)
+    m = meshgen.bump_mesh(40, 20, 0.1, seed=7)
+    om = orc.mesh(m.points, m.cells)
+    U0 = shear_state(om.arrays().ctr, om.nphys)
+    kw = dict(flux=flux, order=2, limiter="vanleer", gradient="wlsq", roe_bdiss=1, cfl=0.6, bc=VBCS, fs=FS,
+              **VISC[visc])
+    ocfg = make_cfg(**kw)
+    Uo = np.ascontiguousarray(U0.copy())
+    errs = []
+    for _ in range(6):
+        e, _ = om.step(ocfg, Uo, rk=rk, lts=lts)
+        errs.append(e)
+    s = Solver(Mesh(m.points, m.cells, tile=64), make_config(rk=rk, lts=lts, **kw))
+    s.set_state(U0)
+    err, _ = s.step(6, history=True)
+    U = s.get_state()
+    assert np.array_equal(U, Uo), rel(U, Uo)
+    assert np.allclose(err, np.array(errs), rtol=1e-12, atol=0)
+
+
+def test_viscous_batch_bit_exact_vs_single_solvers():
+    print("case viscous batch vs single, 2 ramps, 5 steps", flush=True)
+    specs = [(24, 12, 8.0, 0.6, 31), (30, 16, 12.0, 0.7, 32)]
+    meshes = [meshgen.ramp_mesh(nx, ny, th, seed=sd) for nx, ny, th, _, sd in specs]
+    fs = np.array([freestream(M) for *_, M, _ in specs])
+    kw = dict(flux="hllc", order=2, limiter="vanleer", gradient="wlsq", cfl=0.8, rk=3, lts=1,
+              bc=("freestream", "exit", "symmetry", "noslip_wall", "noslip_wall"), block=128, **VISC["constant"])
+    b = Batch(meshes, fs, make_config(**kw))
+    b.solve(tol=-1.0, max_steps=5, check_every=5)
+    Ub = b.get_state()
+    for i, m in enumerate(meshes):
+        s = Solver(Mesh(m.points, m.cells, tile=128, face_budget=256), make_config(fs=fs[i], **kw))
+        s.set_state(np.tile(cons_from_prim(np.array([fs[i][0]]), fs[i][1], fs[i][2], fs[i][3]), (len(m.cells), 1)))
+        s.step(5)
+        assert np.array_equal(Ub[i], s.get_state()), i
+
+
+@pytest.mark.parametrize("bad", [dict(order=1), dict(ref_mode=1, rk=3), dict(prandtl=0.0), dict(r_gas=-1.0),
+                                 dict(mu=float("nan")), dict(viscous=3)])
+def test_invalid_viscous_settings_rejected(bad):
+    m = meshgen.shock_tube_mesh(8, 4)
+    kw = dict(flux="hllc", order=2, limiter="vanleer", gradient="wlsq", cfl=0.5, fs=FS, bc=VBCS,
+              **VISC["constant"])
+    kw.update(bad)
+    with pytest.raises(CfdError, match="viscous|prandtl|r_gas|mu must"):
+        Solver(Mesh(m.points, m.cells), make_config(**kw))
+
+
+def test_shear_wave_decay_on_the_gpu():
+    """u = U cos(pi y) between symmetry planes at Mach 0.05 on 4 x 128 quads decays as
+    exp(-nu pi^2 t) to 1e-3 (global time step, TVD-RK3, HLLC)."""
+    print("case shear wave 4x128", flush=True)
+    nx, ny = 4, 128
+    m = meshgen.shock_tube_mesh(nx, ny, length=nx / ny, height=1.0)
+    mesh = Mesh(m.points, m.cells)
+    y = m.points[m.cells][:, :, 1].mean(axis=1)  # centroids of the rectangles
+    rho, p, mu, Uw = 1.0, 1.0 / GAMMA, 0.01, 0.05
+    shape = np.cos(np.pi * y)
+    s = Solver(mesh, make_config(flux="hllc", order=2, limiter="vanleer", gradient="wlsq", cfl=0.8, rk=3, lts=0,
+                                 bc=("exit", "exit", "symmetry", "symmetry", "wall"), fs=(rho, 0.0, 0.0, p),
+                                 viscous="constant", mu=mu))
+    s.set_state(cons_from_prim(np.full_like(y, rho), Uw * shape, np.zeros_like(y), np.full_like(y, p)))
+    t = 0.0
+    while t < 0.5:
+        _, dt = s.step(200, history=True)
+        t += float(dt.sum())
+    U = s.get_state()
+    amp = float(np.sum(U[:, 1] / U[:, 0] * shape) / np.sum(shape * shape))
+    want = Uw * np.exp(-(mu / rho) * np.pi ** 2 * t)
+    assert abs(amp / want - 1) < 1e-3, (amp, want, t)
