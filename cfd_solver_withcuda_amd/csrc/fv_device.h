@@ -88,7 +88,9 @@ __device__ __forceinline__ void ghost_state(int bc, const FState &in, double n1,
         Ug[3] = ((in.W[3] / kGm1) + in.W[0] * vmag);
         break;
     }
-    case BC_WALL_NOSLIP: {  // no-slip adiabatic wall: velocity negated, rho and p copied
+    case BC_WALL_NOSLIP: {  // Viscous_Wall_Boundary_Condition (src/Wall_Boundary_Conditions.cpp:55-76):
+        // momentum negated, density and energy kept; formed from the primitives, as the face
+        // states hold them
         double u = in.W[1], v = in.W[2];
         double vmag = 0.5 * (u * u + v * v);
         Ug[0] = in.W[0];
@@ -436,7 +438,7 @@ __device__ __forceinline__ void face_gradient(double axL, double ayL, double axR
 // Viscous flux through a face times its length, from the cell-centre primitives WL, WR (WR the
 // ghost state on a boundary face) and the cells' gradients gL, gR of (rho, u, v, p) (gR = gL on a
 // boundary face); (dx, dy) = x_R - x_L. Stress and heat flux as Viscous_Flux_on_Face
-// (src/Grid_Refine_Functions.cpp:418-475): tau_xx = 2/3 mu (2 u_x - v_y), tau_xy = mu (u_y + v_x),
+// (src/Grid_Refine_Functions.cpp:418-479): tau_xx = 2/3 mu (2 u_x - v_y), tau_xy = mu (u_y + v_x),
 // tau_yy = 2/3 mu (2 v_y - u_x), q = k grad T, face velocity and temperature the two cells' mean;
 // F_v = (0, tau_xx nx + tau_xy ny, tau_xy nx + tau_yy ny, (tau_x. u + q_x) nx + (tau_y. u + q_y) ny) dl.
 // Temperatures are T = p / (rho r_gas), grad T = (grad p - (p / rho) grad rho) / (rho r_gas).

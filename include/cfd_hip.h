@@ -66,7 +66,7 @@ typedef struct {
     double zeta, zeta1;
     /* Laminar viscous terms (order 2, native mode): 0 off (Euler), 1 constant viscosity mu,
      * 2 Sutherland's law in T = p / (rho r_gas). Stress tensor and heat flux as the reference's
-     * Viscous_Flux_on_Face (src/Grid_Refine_Functions.cpp:418-475) with dimensional mu,
+     * Viscous_Flux_on_Face (src/Grid_Refine_Functions.cpp:418-479) with dimensional mu,
      * conductivity k = mu gamma r_gas / ((gamma - 1) prandtl); the local time step adds the
      * viscous spectral radius of Viscous_Time_Step_1 (src/Time_Step.cpp:103-109). */
     int viscous;

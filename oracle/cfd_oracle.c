@@ -490,7 +490,8 @@ static void ghost_fstate(int bc, const FState *in, double n1, double n2, const d
         state_from_U(Ug, g);
         return;
     }
-    case OBC_WALL_NOSLIP: { /* no-slip adiabatic wall: velocity negated, rho and p copied */
+    case OBC_WALL_NOSLIP: { /* Viscous_Wall_Boundary_Condition (src/Wall_Boundary_Conditions.cpp:55-76):
+                           * momentum negated, density and energy kept, formed from primitives */
         double u = in->W[1], v = in->W[2];
         double vmag = 0.5 * (u * u + v * v);
         Ug[0] = in->W[0];
@@ -1001,7 +1002,7 @@ static void face_gradient(double axL, double ayL, double axR, double ayR, double
 }
 
 /* Viscous face flux times the face length (Viscous_Flux_on_Face,
- * src/Grid_Refine_Functions.cpp:418-475, dimensional mu in place of mu / Re):
+ * src/Grid_Refine_Functions.cpp:418-479, dimensional mu in place of mu / Re):
  *   tau_xx = 2/3 mu (2 u_x - v_y), tau_xy = mu (u_y + v_x), tau_yy = 2/3 mu (2 v_y - u_x),
  *   Fv = (0, tau_xx nx + tau_xy ny, tau_xy nx + tau_yy ny,
  *         (tau_xx u + tau_xy v + k T_x) nx + (tau_xy u + tau_yy v + k T_y) ny) dl
