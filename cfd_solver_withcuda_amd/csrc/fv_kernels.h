@@ -37,6 +37,11 @@ hipError_t launch_grad(const DevMesh &m, const DevCfg &c, int grad_type, const d
 hipError_t launch_flux(const DevMesh &m, const DevCfg &c, int flux, int order, int lim, int mode,
                        const double4 *Us, const double *grad, const StageArgs &st, int block,
                        hipStream_t s);
+// The viscous variants of the flux pass (fv_visc.hip), launched by launch_flux when
+// c.viscous is set at order 2; sm is the dynamic LDS of the launch.
+hipError_t launch_flux_visc(const DevMesh &m, const DevCfg &c, int flux, int lim, int mode,
+                            const double4 *Us, const double *grad, const StageArgs &st, int block,
+                            size_t sm, hipStream_t s);
 hipError_t launch_fill(double *p, double v, int n, hipStream_t s);
 hipError_t launch_err_finalize(const double *partial, int n, double *err4, hipStream_t s);
 hipError_t launch_ghosts(int n, const int *gcell, const double2 *gnrm, const int *gkind,
