@@ -89,10 +89,11 @@ struct cfd_mesh_s {
     // second order)
     int *d_rc_nf = nullptr, *d_rc_nbr = nullptr;
     double *d_rc_dist = nullptr, *d_rc_nrm = nullptr, *d_rc_len = nullptr, *d_rc_area = nullptr;
+    int4 *d_fst = nullptr;  // WENO line stencils of the face entries (built on first use)
     DevMesh dev{};
     ~cfd_mesh_s() {
         void *ps[] = {d_tile_cstart, d_tile_fstart, d_fgeo, d_fw, d_fnx, d_fny, d_fdl, d_fLR, d_slots, d_area, d_inv_area, d_gcell, d_gkind, d_gnrm,
-                      d_rc_nf, d_rc_nbr, d_rc_dist, d_rc_nrm, d_rc_len, d_rc_area};
+                      d_rc_nf, d_rc_nbr, d_rc_dist, d_rc_nrm, d_rc_len, d_rc_area, d_fst};
         for (void *p : ps)
             if (p) (void)hipFree(p);
     }
@@ -291,7 +292,10 @@ int cfd_mesh_layout_export(const cfd_mesh_t *m, int *tile_cstart, int *tile_fsta
 
 static int check_config(const cfd_mesh_s *m, const cfd_config_t *c) {
     if (c->flux < 0 || c->flux > 3) return fail(CFD_E_ARG, "config: flux must be 0..3");
-    if (c->order != 1 && c->order != 2) return fail(CFD_E_ARG, "config: order must be 1 or 2");
+    if (c->order < 1 || c->order > 3) return fail(CFD_E_ARG, "config: order must be 1, 2 or 3 (WENO5)");
+    if (c->order == 3 && c->ref_mode) return fail(CFD_E_ARG, "config: WENO5 (order 3) exists in the native iteration only");
+    if (c->order == 3 && (c->freeze_tol > 0.0 || c->freeze_stall > 0))
+        return fail(CFD_E_ARG, "config: limiter freezing needs order 2");
     if (c->limiter < 0 || c->limiter > 3) return fail(CFD_E_ARG, "config: limiter must be 0..3");
     if (c->gradient < 0 || c->gradient > 1) return fail(CFD_E_ARG, "config: gradient must be 0 or 1");
     if (!(c->cfl > 0.0)) return fail(CFD_E_ARG, "config: cfl must be positive");
@@ -310,10 +314,30 @@ static int check_config(const cfd_mesh_s *m, const cfd_config_t *c) {
     return 0;
 }
 
+// WENO line stencils of the mesh's face entries on the device (first order-3 configuration).
+static int ensure_line_stencils(cfd_mesh_s *m) {
+    if (m->d_fst) return 0;
+    std::vector<int> st;
+    line_stencils(m->ref, m->lay, st);
+    int4 *d = nullptr;
+    HIP_TRY(hipMalloc((void **)&d, std::max<size_t>(st.size(), 4) * sizeof(int)));
+    if (!st.empty()) {
+        hipError_t e = hipMemcpy(d, st.data(), st.size() * sizeof(int), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            (void)hipFree(d);
+            return hip_fail(e, "ensure_line_stencils hipMemcpy");
+        }
+    }
+    m->d_fst = d;
+    m->dev.fst = d;
+    return 0;
+}
+
 int cfd_solver_configure(cfd_solver_t *s, const cfd_config_t *cfg) {
     if (!s || !cfg) return fail(CFD_E_ARG, "cfd_solver_configure: null argument");
     int rc = check_config(s->m, cfg);
     if (rc) return rc;
+    if (cfg->order == 3 && (rc = ensure_line_stencils(s->m))) return rc;
     s->cfg = *cfg;
     for (int k = 0; k < 6; k++) s->dcfg.bc[k] = cfg->bc[k];
     for (int k = 0; k < 4; k++) s->dcfg.fs[k] = cfg->fs[k];
@@ -454,7 +478,7 @@ static int residual_pass(cfd_solver_s *s, const double4 *in, double *dtmin, cons
     const cfd_config_t &c = s->cfg;
     const int order = order_override ? order_override : c.order;
     if (c.ref_mode && order == 2 && c.flux == FLUX_ROE) return ref_roe2o_pass(s, in, dtmin, uexit, st);
-    if (order == 2) HIP_TRY(launch_grad(s->m->dev, s->dcfg, c.gradient, in, s->grad, c.block, st));
+    if (order >= 2) HIP_TRY(launch_grad(s->m->dev, s->dcfg, c.gradient, in, s->grad, c.block, st));
     StageArgs a = resid_args(s, dtmin != nullptr, uexit);
     HIP_TRY(launch_flux(s->m->dev, s->dcfg, c.flux, order, c.limiter, MODE_RESID, in, s->grad, a, c.block, st));
     if (dtmin) HIP_TRY(launch_min_reduce(s->dt_partial, s->m->dev.ntiles, dtmin, st));
@@ -480,7 +504,7 @@ int cfd_residual(cfd_solver_t *s, double *R, double *dt, double *grad, void *str
         HIP_TRY(hipMemcpy(h.data(), s->dt, h.size() * sizeof(double), hipMemcpyDeviceToHost));
         for (int k = 0; k < nc; k++) dt[perm[k]] = h[k];
     }
-    if (grad && s->cfg.order == 2) {
+    if (grad && s->cfg.order >= 2) {
         std::vector<double> h(8 * (size_t)nc);
         HIP_TRY(hipMemcpy(h.data(), s->grad, h.size() * sizeof(double), hipMemcpyDeviceToHost));
         for (int k = 0; k < nc; k++) std::memcpy(&grad[8 * (size_t)perm[k]], &h[8 * (size_t)k], 8 * sizeof(double));
@@ -553,7 +577,7 @@ static int native_step(cfd_solver_s *s, bool want_err, hipStream_t st) {
     for (int k = 0; k < nst; k++) {
         const double4 *in = (nst == 4) ? ins4[k] : ins3[k];
         double4 *out = (nst == 4) ? outs4[k] : outs3[k];
-        if (c.order == 2) HIP_TRY(launch_grad(dm, s->dcfg, c.gradient, in, s->grad, c.block, st));
+        if (c.order >= 2) HIP_TRY(launch_grad(dm, s->dcfg, c.gradient, in, s->grad, c.block, st));
         StageArgs a{};
         a.rk = nst;
         a.stage = k;
@@ -627,7 +651,7 @@ int cfd_time_pass(cfd_solver_t *s, int which, int reps, float *ms, void *stream)
     if (!s || reps <= 0 || !ms) return fail(CFD_E_ARG, "cfd_time_pass: bad arguments");
     if (!s->has_state) return fail(CFD_E_STATE, "cfd_time_pass: state not set");
     if (which < 0 || which > 2) return fail(CFD_E_ARG, "cfd_time_pass: which must be 0, 1 or 2");
-    if (which == 0 && s->cfg.order != 2) return fail(CFD_E_ARG, "cfd_time_pass: no gradient pass at order 1");
+    if (which == 0 && s->cfg.order == 1) return fail(CFD_E_ARG, "cfd_time_pass: no gradient pass at order 1");
     if (s->cfg.ref_mode) return fail(CFD_E_ARG, "cfd_time_pass: native mode only");
     hipStream_t st = (hipStream_t)stream;
     const cfd_config_t &c = s->cfg;
@@ -637,7 +661,7 @@ int cfd_time_pass(cfd_solver_t *s, int which, int reps, float *ms, void *stream)
         if (rc) return rc;
         s->dt_valid = true;
     }
-    if (c.order == 2) HIP_TRY(launch_grad(dm, s->dcfg, c.gradient, s->Un, s->grad, c.block, st));
+    if (c.order >= 2) HIP_TRY(launch_grad(dm, s->dcfg, c.gradient, s->Un, s->grad, c.block, st));
     StageArgs a{};
     a.rk = (c.rk == 4) ? 4 : 3;
     a.stage = 0;

@@ -43,6 +43,9 @@ struct DevMesh {
     const int2 *fLR;          // [nfe] {L, R}
     const double *fnx, *fny, *fdl;  // [nfe] unit normal and length (Green-Gauss, WLSQ ghosts)
     const double2 *fw;        // [2*nfe] least-squares coefficient vectors of L (2e) and R (2e+1)
+    // WENO (order 3) only: per face entry the line stencil {L-1, L-2, R+1, R+2} (renumbered
+    // cells through opposite faces of quadrilaterals), all -1 where the line is incomplete
+    const int4 *fst;          // [nfe]
     const uint16_t *slots;    // [4*nc] (local face << 1) | (cell is R); 0xFFFF = none
     const double *area, *inv_area;  // [nc]
     // Batched meshes (several independent instances concatenated; null for one mesh):

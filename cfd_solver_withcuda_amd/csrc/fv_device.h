@@ -407,6 +407,57 @@ __device__ __forceinline__ void reconstruct_side(const FState &c, const FState &
     out.U[3] = Wf[3] * kInvGm1 + Wf[0] * vmag;
 }
 
+// WENO_Reconstruction (src/WENO2D.cpp:9-79): fifth-order WENO-JS (epsilon 1e-6, p = 2) of one
+// variable on the stencil a..e centred on c; shift 0 gives the value at the edge between c and d,
+// shift 1 the value at the edge between b and c. Squares are products (the reference's
+// pow(x, 2)), so device and host agree bit for bit.
+__device__ __forceinline__ double weno5(double a, double b, double c, double d, double e, int shift) {
+    if (!isfinite(a) || !isfinite(b) || !isfinite(c) || !isfinite(d) || !isfinite(e)) return c;
+    const double eps = 1e-6;
+    double d0, d1, d2, v0, v1, v2;
+    if (shift == 0) {
+        d0 = 3.0 / 10.0;
+        d1 = 6.0 / 10.0;
+        d2 = 1.0 / 10.0;
+        v2 = (2.0 * a - 7.0 * b + 11.0 * c) / 6.0;
+        v1 = (-1.0 * b + 5.0 * c + 2.0 * d) / 6.0;
+        v0 = (2.0 * c + 5.0 * d - 1.0 * e) / 6.0;
+    } else {
+        d0 = 1.0 / 10.0;
+        d1 = 6.0 / 10.0;
+        d2 = 3.0 / 10.0;
+        v2 = (-1.0 * a + 5.0 * b + 2.0 * c) / 6.0;
+        v1 = (2.0 * b + 5.0 * c - 1.0 * d) / 6.0;
+        v0 = (11.0 * c - 7.0 * d + 2.0 * e) / 6.0;
+    }
+    const double s2a = a - 2.0 * b + c, s2b = a - 4.0 * b + 3.0 * c;
+    const double s1a = b - 2.0 * c + d, s1b = b - d;
+    const double s0a = c - 2.0 * d + e, s0b = 3.0 * c - 4.0 * d + e;
+    const double b2 = (13.0 / 12.0) * (s2a * s2a) + (1.0 / 4.0) * (s2b * s2b);
+    const double b1 = (13.0 / 12.0) * (s1a * s1a) + (1.0 / 4.0) * (s1b * s1b);
+    const double b0 = (13.0 / 12.0) * (s0a * s0a) + (1.0 / 4.0) * (s0b * s0b);
+    const double e0 = eps + b0, e1 = eps + b1, e2 = eps + b2;
+    const double a0 = d0 / (e0 * e0);
+    const double a1 = d1 / (e1 * e1);
+    const double a2 = d2 / (e2 * e2);
+    const double sum = a0 + a1 + a2;
+    double U;
+    if (sum < eps) {
+        U = (v0 + v1 + v2) / 3.0;
+    } else {
+        const double w0 = a0 / sum, w1 = a1 / sum, w2 = a2 / sum;
+        U = w0 * v0 + w1 * v1 + w2 * v2;
+    }
+    return isfinite(U) ? U : c;
+}
+
+// Face state from WENO-reconstructed conserved values (Calculate_Primitive_Variables on U_L /
+// U_R, src/WENO2D.cpp:482-489); a non-positive density or pressure falls back to the cell state.
+__device__ __forceinline__ void weno_face_state(const double *U, const FState &cell, FState &out) {
+    state_from_U(U[0], U[1], U[2], U[3], out);
+    if (!(out.W[0] > 0.0) || !(out.W[3] > 0.0)) out = cell;
+}
+
 // ------------------------------------------------------------------------------------------
 // Laminar viscous terms (2-D Navier-Stokes, Stokes hypothesis).
 // ------------------------------------------------------------------------------------------

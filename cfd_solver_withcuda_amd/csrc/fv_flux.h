@@ -1,6 +1,6 @@
 // Flux pass of the native finite-volume path, shared by the translation units that
 // instantiate it (fv_kernels.hip: inviscid first and second order; fv_visc.hip: second order
-// with viscous terms), plus the tile helpers the other kernels use.
+// with viscous terms; fv_weno.hip: WENO5, order 3), plus the tile helpers the other kernels use.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -166,11 +166,11 @@ __device__ __forceinline__ void flux_face(const DevMesh &m, const DevCfg &cfg,
     const int lL = f.L - c0;
     const bool locL = (unsigned)lL < (unsigned)nct;
     FState SL, SR;
-    // Tile cells come from LDS: primitives always, conserved values at first order only (the
-    // second-order face states form their conserved values from the reconstructed primitives).
+    // Tile cells come from LDS: primitives always, conserved values at first order and for WENO
+    // (the MUSCL face states form their conserved values from the reconstructed primitives).
     if (locL) {
         for (int q = 0; q < 5; q++) SL.W[q] = sW[q * T + lL];
-        if constexpr (ORDER == 1)
+        if constexpr (ORDER != 2)
             for (int q = 0; q < 4; q++) SL.U[q] = sU[q * T + lL];
     } else {
         load_state(Us, f.L, SL);
@@ -195,14 +195,49 @@ __device__ __forceinline__ void flux_face(const DevMesh &m, const DevCfg &cfg,
         locR = (unsigned)lR < (unsigned)nct;
         if (locR) {
             for (int q = 0; q < 5; q++) SR.W[q] = sW[q * T + lR];
-            if constexpr (ORDER == 1)
+            if constexpr (ORDER != 2)
                 for (int q = 0; q < 4; q++) SR.U[q] = sU[q * T + lR];
         } else {
             load_state(Us, f.R, SR);
         }
     }
     double F[4];
-    if constexpr (ORDER == 2) {
+    if constexpr (ORDER == 3) {
+        // WENO5 on the conserved variables along the line of quadrilaterals through the face,
+        // cells L-2, L-1, L | R, R+1, R+2, where the layout found one; MUSCL below otherwise
+        const int4 q4 = boundary ? make_int4(-1, -1, -1, -1) : m.fst[e];
+        if (q4.x >= 0) {
+            auto cons = [&](int c, double *u) {
+                const int l = c - c0;
+                if ((unsigned)l < (unsigned)nct) {
+                    for (int q = 0; q < 4; q++) u[q] = sU[q * T + l];
+                } else {
+                    const double4 v = Us[c];
+                    u[0] = v.x;
+                    u[1] = v.y;
+                    u[2] = v.z;
+                    u[3] = v.w;
+                }
+            };
+            double Lm2[4], Lm1[4], Rp1[4], Rp2[4];
+            cons(q4.y, Lm2);
+            cons(q4.x, Lm1);
+            cons(q4.z, Rp1);
+            cons(q4.w, Rp2);
+            double UL[4], UR[4];
+            for (int q = 0; q < 4; q++) {
+                UL[q] = weno5(Lm2[q], Lm1[q], SL.U[q], SR.U[q], Rp1[q], 0);
+                UR[q] = weno5(Lm1[q], SL.U[q], SR.U[q], Rp1[q], Rp2[q], 1);
+            }
+            FState fL, fR;
+            weno_face_state(UL, SL, fL);
+            weno_face_state(UR, SR, fR);
+            face_flux<FLUX>(false, cfg.roe_bdiss, fL, fR, nx, ny, dl, F);
+            for (int q = 0; q < 4; q++) sF[q * FT + j] = F[q];
+            return;
+        }
+    }
+    if constexpr (ORDER >= 2) {
         double gL[8];
         if (glds && locL) {
             for (int q = 0; q < 8; q++) gL[q] = sG[q * T + lL];
@@ -272,12 +307,12 @@ __global__ void __launch_bounds__(256) CFD_FLUX_ATTR flux_tile_kernel(DevMesh m,
     const int c0 = m.tile_cstart[tile];
     const int nct = m.tile_cstart[tile + 1] - c0;
     const int fs = m.tile_fstart[tile], nft = m.tile_fstart[tile + 1] - fs;
-    const bool glds = (ORDER == 2) && cfg.grad_lds;
+    const bool glds = (ORDER >= 2) && cfg.grad_lds;
     const int fz = (ORDER == 2 && st.freeze) ? st.freeze[m.tile_inst ? m.tile_inst[tile] : 0] : 0;
     double *sW = smem;                          // [5][T] primitives (rho, u, v, p, a)
-    double *sU = sW + 5 * T;                    // [4][T] conserved (first order only)
-    double *sF = sU + (ORDER == 1 ? 4 * T : 0); // [4][FT] face fluxes
-    double *sG = sF + 4 * FT;                   // [8][T] (second order, LDS-staged gradients)
+    double *sU = sW + 5 * T;                    // [4][T] conserved (first order and WENO)
+    double *sF = sU + (ORDER != 2 ? 4 * T : 0); // [4][FT] face fluxes
+    double *sG = sF + 4 * FT;                   // [8][T] (MUSCL, LDS-staged gradients)
     double *sN = sG + (glds ? 8 * T : 0);       // [3][FT] face normal and length (time steps)
     // phase C forms time steps from the face geometry: staged here from the face records of
     // phase B instead of read again from HBM
@@ -313,15 +348,15 @@ __global__ void __launch_bounds__(256) CFD_FLUX_ATTR flux_tile_kernel(DevMesh m,
     }
     FaceRec f0{}, f1{};
     const bool h0 = k < nft, h1 = k + B < nft;
-    if (h0) load_face<true, ORDER == 2>(m, fs + k, f0);
-    if (h1) load_face<true, ORDER == 2>(m, fs + k + B, f1);
+    if (h0) load_face<true, ORDER >= 2>(m, fs + k, f0);
+    if (h1) load_face<true, ORDER >= 2>(m, fs + k + B, f1);
 
     // ---- A: stage the tile's states ----------------------------------------------------
     if (hasCell) {
         FState s;
         state_from_U(u4.x, u4.y, u4.z, u4.w, s);
         for (int q = 0; q < 5; q++) sW[q * T + k] = s.W[q];
-        if constexpr (ORDER == 1)
+        if constexpr (ORDER != 2)
             for (int q = 0; q < 4; q++) sU[q * T + k] = s.U[q];
         if (glds) {
             double g[8];
@@ -342,7 +377,7 @@ __global__ void __launch_bounds__(256) CFD_FLUX_ATTR flux_tile_kernel(DevMesh m,
     }
     for (int j = k + 2 * B; j < nft; j += B) {
         FaceRec f;
-        load_face<true, ORDER == 2>(m, fs + j, f);
+        load_face<true, ORDER >= 2>(m, fs + j, f);
         stageN(j, f);
         flux_face<FLUX, ORDER, LIM, VISC>(m, cfg, Us, grad, st.Uexit, sW, sU, sG, sF, glds, T, FT, c0, nct, j, f, fz, st.psi, fs + j, fsv);
     }

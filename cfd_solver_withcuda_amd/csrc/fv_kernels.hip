@@ -558,8 +558,8 @@ size_t grad_smem_bytes(const DevMesh &m, int grad_type) {
 }
 
 size_t flux_smem_bytes(const DevMesh &m, int order, int grad_lds) {
-    size_t d = 5 * (size_t)m.tile + (order == 1 ? 4 * (size_t)m.tile : 0) + 7 * (size_t)m.max_tile_faces;
-    if (order == 2 && grad_lds) d += 8 * (size_t)m.tile;
+    size_t d = 5 * (size_t)m.tile + (order != 2 ? 4 * (size_t)m.tile : 0) + 7 * (size_t)m.max_tile_faces;
+    if (order >= 2 && grad_lds) d += 8 * (size_t)m.tile;
     // the block reductions of phase C need 4 doubles per thread (blockDim <= 256)
     if (d < 4 * 256) d = 4 * 256;
     return sizeof(double) * d;
@@ -579,6 +579,7 @@ hipError_t launch_flux(const DevMesh &m, const DevCfg &c, int flux, int order, i
                        const double4 *Us, const double *grad, const StageArgs &st, int block,
                        hipStream_t s) {
     const size_t sm = flux_smem_bytes(m, order, c.grad_lds);
+    if (order == 3) return launch_flux_weno(m, c, flux, lim, mode, Us, grad, st, block, sm, s);
     if (order == 2 && c.viscous) return launch_flux_visc(m, c, flux, lim, mode, Us, grad, st, block, sm, s);
     if (order == 1) return launch_flux_family<1, false>(m, c, flux, lim, mode, Us, grad, st, block, sm, s);
     return launch_flux_family<2, false>(m, c, flux, lim, mode, Us, grad, st, block, sm, s);

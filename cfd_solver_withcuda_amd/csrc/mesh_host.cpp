@@ -415,6 +415,39 @@ void build_layout(const RefMesh &m, int tile, int renumber, Layout &L, int face_
     L.tile_fstart[L.ntiles] = (int)L.fL.size();
 }
 
+// The physical cell across the face opposite slot f of quadrilateral c, or -1.
+static int cell_behind(const RefMesh &m, int c, int f) {
+    if (c < 0 || f < 0 || m.nf[c] != 4) return -1;
+    const int j = m.nbr[4 * (size_t)c + (f + 2) % 4];
+    return (j >= 0 && j < m.nc) ? j : -1;
+}
+
+// Slot of cell c whose neighbour is nb, or -1.
+static int slot_to(const RefMesh &m, int c, int nb) {
+    if (c < 0) return -1;
+    for (int f = 0; f < m.nf[c]; f++)
+        if (m.nbr[4 * (size_t)c + f] == nb) return f;
+    return -1;
+}
+
+void line_stencils(const RefMesh &m, const Layout &L, std::vector<int> &st) {
+    const size_t n = L.fL.size();
+    st.assign(4 * n, -1);
+    for (size_t e = 0; e < n; e++) {
+        if (L.fR[e] < 0) continue;
+        const int a = L.perm[L.fL[e]], b = L.perm[L.fR[e]];
+        const int l1 = cell_behind(m, a, slot_to(m, a, b));
+        const int l2 = cell_behind(m, l1, slot_to(m, l1, a));
+        const int r1 = cell_behind(m, b, slot_to(m, b, a));
+        const int r2 = cell_behind(m, r1, slot_to(m, r1, b));
+        if (l1 < 0 || l2 < 0 || r1 < 0 || r2 < 0) continue;
+        st[4 * e] = L.iperm[l1];
+        st[4 * e + 1] = L.iperm[l2];
+        st[4 * e + 2] = L.iperm[r1];
+        st[4 * e + 3] = L.iperm[r2];
+    }
+}
+
 void pack_face_records(const Layout &L, int cell_off, std::vector<double> &geo, std::vector<double> &w,
                        std::vector<int> &lr) {
     const size_t n = L.fL.size();

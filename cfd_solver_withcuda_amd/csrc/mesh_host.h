@@ -52,4 +52,11 @@ void build_layout(const RefMesh &m, int tile, int renumber, Layout &L, int face_
 void pack_face_records(const Layout &L, int cell_off, std::vector<double> &geo, std::vector<double> &w,
                        std::vector<int> &lr);
 
+// WENO line stencils of the layout's face entries: st gets 4 renumbered cells per entry,
+// {L-1, L-2, R+1, R+2}, where L-1 is the neighbour of L across the face opposite the entry's face
+// (L a quadrilateral), L-2 the neighbour of L-1 across the face opposite the one it shares with L,
+// and likewise on the R side; all four are -1 when the entry is a boundary face or any of them is
+// missing (a triangle on the line, or a ghost).
+void line_stencils(const RefMesh &m, const Layout &L, std::vector<int> &st);
+
 }  // namespace cfd

@@ -36,9 +36,11 @@ typedef struct cfd_solver_s cfd_solver_t;
  * Local_Time_Stepping / CFL flags, src/Configuration_Read.cpp:23-87, made explicit). */
 typedef struct {
     int flux;      /* 0 Roe (Harten entropy fix), 1 HLLC, 2 LLF, 3 AUSM */
-    int order;     /* 1 first order, 2 MUSCL */
-    int limiter;   /* 0 none, 1 minmod, 2 Van Leer, 3 superbee (order 2) */
-    int gradient;  /* 0 Green-Gauss, 1 weighted least squares (order 2) */
+    int order;     /* 1 first order, 2 MUSCL, 3 WENO5 (WENO_Reconstruction, src/WENO2D.cpp:9-79,
+                      on the conserved variables along lines of quadrilaterals through opposite
+                      faces; MUSCL on faces without a complete five-cell line; native iteration) */
+    int limiter;   /* 0 none, 1 minmod, 2 Van Leer, 3 superbee (MUSCL faces) */
+    int gradient;  /* 0 Green-Gauss, 1 weighted least squares (MUSCL faces) */
     int roe_bdiss; /* Roe dissipation on boundary faces (0 = the reference's behaviour) */
     double cfl;
     int bc[6];     /* ghost type for face kinds 1..5 (LEFT, RIGHT, TOP, BOTTOM, WALL):

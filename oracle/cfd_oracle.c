@@ -1044,13 +1044,95 @@ static double viscous_dt_term(const ONatCfg *cfg, double rho, double p, double s
     return 4.0 * ((fmax(4.0 / (3.0 * rho), GAMMA / rho) * mu * (1.0 / cfg->prandtl) * sdl2) / area);
 }
 
+/* ------------------------------------------------------------------------------------------
+ * WENO5 (order 3)
+ * ---------------------------------------------------------------------------------------- */
+/* WENO_Reconstruction (src/WENO2D.cpp:9-79): fifth-order WENO-JS (epsilon 1e-6, p = 2) of one
+ * variable on the stencil a..e centred on c; shift 0 gives the value at the edge between c and d,
+ * shift 1 at the edge between b and c. pow(x, 2) is written as x * x. */
+double or_weno5(double a, double b, double c, double d, double e, int shift) {
+    if (!isfinite(a) || !isfinite(b) || !isfinite(c) || !isfinite(d) || !isfinite(e)) return c;
+    const double eps = 1e-6;
+    double d0, d1, d2, v0, v1, v2;
+    if (shift == 0) {
+        d0 = 3.0 / 10.0;
+        d1 = 6.0 / 10.0;
+        d2 = 1.0 / 10.0;
+        v2 = (2.0 * a - 7.0 * b + 11.0 * c) / 6.0;
+        v1 = (-1.0 * b + 5.0 * c + 2.0 * d) / 6.0;
+        v0 = (2.0 * c + 5.0 * d - 1.0 * e) / 6.0;
+    } else {
+        d0 = 1.0 / 10.0;
+        d1 = 6.0 / 10.0;
+        d2 = 3.0 / 10.0;
+        v2 = (-1.0 * a + 5.0 * b + 2.0 * c) / 6.0;
+        v1 = (2.0 * b + 5.0 * c - 1.0 * d) / 6.0;
+        v0 = (11.0 * c - 7.0 * d + 2.0 * e) / 6.0;
+    }
+    double s2a = a - 2.0 * b + c, s2b = a - 4.0 * b + 3.0 * c;
+    double s1a = b - 2.0 * c + d, s1b = b - d;
+    double s0a = c - 2.0 * d + e, s0b = 3.0 * c - 4.0 * d + e;
+    double b2 = (13.0 / 12.0) * (s2a * s2a) + (1.0 / 4.0) * (s2b * s2b);
+    double b1 = (13.0 / 12.0) * (s1a * s1a) + (1.0 / 4.0) * (s1b * s1b);
+    double b0 = (13.0 / 12.0) * (s0a * s0a) + (1.0 / 4.0) * (s0b * s0b);
+    double e0 = eps + b0, e1 = eps + b1, e2 = eps + b2;
+    double a0 = d0 / (e0 * e0);
+    double a1 = d1 / (e1 * e1);
+    double a2 = d2 / (e2 * e2);
+    double sum = a0 + a1 + a2;
+    double U;
+    if (sum < eps) {
+        U = (v0 + v1 + v2) / 3.0;
+    } else {
+        double w0 = a0 / sum, w1 = a1 / sum, w2 = a2 / sum;
+        U = w0 * v0 + w1 * v1 + w2 * v2;
+    }
+    return isfinite(U) ? U : c;
+}
+
+/* The physical cell across the face opposite slot f of quadrilateral c, or -1. */
+static int cell_behind(const OMesh *m, int c, int f) {
+    if (c < 0 || f < 0 || m->nf[c] != 4) return -1;
+    int j = m->nbr[4 * c + (f + 2) % 4];
+    return (j >= 0 && j < m->nc) ? j : -1;
+}
+
+static int slot_to(const OMesh *m, int c, int nb) {
+    if (c < 0) return -1;
+    for (int f = 0; f < m->nf[c]; f++)
+        if (m->nbr[4 * c + f] == nb) return f;
+    return -1;
+}
+
+/* Line stencil of the interior face between L and R through opposite faces of quadrilaterals:
+ * st = {L-1, L-2, R+1, R+2}; returns 0 when any of them is missing. */
+static int line_stencil(const OMesh *m, int L, int R, int *st) {
+    int l1 = cell_behind(m, L, slot_to(m, L, R));
+    int l2 = cell_behind(m, l1, slot_to(m, l1, L));
+    int r1 = cell_behind(m, R, slot_to(m, R, L));
+    int r2 = cell_behind(m, r1, slot_to(m, r1, R));
+    if (l1 < 0 || l2 < 0 || r1 < 0 || r2 < 0) return 0;
+    st[0] = l1;
+    st[1] = l2;
+    st[2] = r1;
+    st[3] = r2;
+    return 1;
+}
+
+/* Face state from reconstructed conserved values (src/WENO2D.cpp:482-489); a non-positive
+ * density or pressure falls back to the cell state. */
+static void weno_face_state(const double *U, const FState *cell, FState *out) {
+    state_from_U(U, out);
+    if (!(out->W[0] > 0.0) || !(out->W[3] > 0.0)) *out = *cell;
+}
+
 void on_residual(const OMesh *m, const ONatCfg *cfg, const double *U, const double *U_exit_src,
                  double *R, double *dt, double *grad) {
     const int nc = m->nc;
     FState *S = (FState *)malloc(sizeof(FState) * nc);
     for (int c = 0; c < nc; c++) state_from_U(&U[4 * c], &S[c]);
     double *W5 = NULL;
-    if (cfg->order == 2) {
+    if (cfg->order >= 2) {
         W5 = (double *)malloc(sizeof(double) * 5 * nc);
         for (int c = 0; c < nc; c++)
             for (int k = 0; k < 5; k++) W5[5 * c + k] = S[c].W[k];
@@ -1067,7 +1149,23 @@ void on_residual(const OMesh *m, const ONatCfg *cfg, const double *U, const doub
         if (boundary && bc == OBC_EXIT && U_exit_src) state_from_U(&U_exit_src[4 * L], &srcL);
         if (boundary) ghost_fstate(bc, &srcL, nx, ny, cfg->fs, &SR);
         else SR = S[Rc];
-        if (cfg->order == 2) {
+        int st[4];
+        if (cfg->order == 3 && !boundary && line_stencil(m, L, Rc, st)) {
+            /* WENO5 on the conserved variables, cells L-2, L-1, L | R, R+1, R+2 */
+            double UL[4], UR[4];
+            for (int k = 0; k < 4; k++) {
+                UL[k] = or_weno5(U[4 * st[1] + k], U[4 * st[0] + k], U[4 * L + k], U[4 * Rc + k],
+                                 U[4 * st[2] + k], 0);
+                UR[k] = or_weno5(U[4 * st[0] + k], U[4 * L + k], U[4 * Rc + k], U[4 * st[2] + k],
+                                 U[4 * st[3] + k], 1);
+            }
+            FState fL, fR;
+            weno_face_state(UL, &SL, &fL);
+            weno_face_state(UR, &SR, &fR);
+            face_flux(cfg, 0, &fL, &fR, nx, ny, dl, &Ff[4 * fi]);
+            continue;
+        }
+        if (cfg->order >= 2) {
             double dx = m->ctr[3 * Rc] - m->ctr[3 * L];
             double dy = m->ctr[3 * Rc + 1] - m->ctr[3 * L + 1];
             FState fL, fR;
@@ -1081,7 +1179,7 @@ void on_residual(const OMesh *m, const ONatCfg *cfg, const double *U, const doub
                 reconstruct_side(cfg, &SR, &SL, &grad[8 * Rc], dx, dy, -1.0, phiL ? phiL + 4 : NULL, &fR);
             }
             face_flux(cfg, boundary, &fL, &fR, nx, ny, dl, &Ff[4 * fi]);
-            if (cfg->viscous) {
+            if (cfg->viscous && cfg->order == 2) {
                 /* cell-centre states (ghost on a boundary face, with the cell's own gradient) */
                 double Fv[4];
                 const double *gR = boundary ? &grad[8 * L] : &grad[8 * Rc];

@@ -58,7 +58,7 @@ enum { OGRAD_GG = 0, OGRAD_WLSQ = 1 };
 
 typedef struct {
     int flux;        /* OFLUX_* */
-    int order;       /* 1 or 2 */
+    int order;       /* 1, 2 (MUSCL) or 3 (WENO5 on lines of quads, MUSCL elsewhere) */
     int limiter;     /* OLIM_* */
     int gradient;    /* OGRAD_* */
     int roe_bdiss;   /* 1: Roe dissipation on boundary faces; 0: none (reference behaviour) */
@@ -76,6 +76,8 @@ typedef struct {
 
 /* ---- primitive / ghost helpers --------------------------------------------------------- */
 void or_prim(const double *U4, double *prim11);
+/* WENO5-JS of one variable (src/WENO2D.cpp:9-79); shift 0 right edge of c, 1 left edge. */
+double or_weno5(double a, double b, double c, double d, double e, int shift);
 void on_ghost_state(int bc, const double *Win5, double nx, double ny, const double *fs4,
                     double *Wg5);
 /* conserved ghost state of a cell with conserved state U4 across a face with unit normal n */

@@ -137,6 +137,8 @@ class Oracle:
         L.om_nfaces.restype = ctypes.c_int
         L.om_nfaces.argtypes = [ctypes.c_void_p]
         L.on_ghost_cons.argtypes = [ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double, _dp, _dp]
+        L.or_weno5.restype = ctypes.c_double
+        L.or_weno5.argtypes = [ctypes.c_double] * 5 + [ctypes.c_int]
         L.or_limiter_slope.restype = ctypes.c_double
         L.or_limiter_slope.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double]
         cfgp = ctypes.POINTER(ONatCfg)
@@ -180,6 +182,9 @@ class Oracle:
         for i in range(len(U)):
             self.lib.or_prim(_d(U[i]), _d(out[i]))
         return out
+
+    def weno5(self, a, b, c, d, e, shift: int) -> float:
+        return self.lib.or_weno5(a, b, c, d, e, shift)
 
     def limiter_slope(self, lim: str, a: float, b: float) -> float:
         return self.lib.or_limiter_slope(OLIM[lim], a, b)
@@ -317,6 +322,8 @@ class Reference:
         L.ref_set_state_U.argtypes = [_dp]
         L.ref_residual.argtypes = [_dp, _dp]
         L.ref_ausm.argtypes = [_dp]
+        L.ref_weno.restype = ctypes.c_double
+        L.ref_weno.argtypes = [ctypes.c_double] * 5 + [ctypes.c_int]
         L.ref_step.argtypes = [_dp, _dp, _dp]
         L.ref_get_state.argtypes = [_dp, _dp]
         L.ref_get_delU.argtypes = [_dp]
@@ -387,6 +394,10 @@ class Reference:
         P = np.zeros((nt, 11))
         self.lib.ref_get_state(_d(U), _d(P))
         return U, P
+
+    def weno(self, a, b, c, d, e, shift: int) -> float:
+        """The reference's WENO_Reconstruction of one variable."""
+        return self.lib.ref_weno(a, b, c, d, e, shift)
 
     def gg_gradient(self, var: int) -> np.ndarray:
         g = np.zeros((self.nphys, 2))

@@ -43,6 +43,7 @@ void createOutputDirectories() {}
 bool Read_GmshVTK_Grid(const string &GridFileName);
 void Ausm_Flux(const int &Cell_No);
 void Calculate_Gradient_At_Cell_Center(int &Current_Cell_Index, int &Grad_Type, V_D &grad);
+void WENO_Reconstruction(double &a, double &b, double &c, double &d, double &e, int &shift, double &U);
 
 // Mesh loading: only the legacy-VTK branch is needed by the harness.
 bool Load_Mesh(const std::string &path) { return Read_GmshVTK_Grid(path); }
@@ -277,4 +278,11 @@ void ref_write_solution(const char *path, int iters, int n1, int n2, int n3) {
     Write_Solution(std::string(path), 1);
 }
 void ref_write_error(const char *path) { Write_Error_File(std::string(path)); }
+
+// WENO_Reconstruction (src/WENO2D.cpp:9-79) of one variable on the stencil a..e.
+double ref_weno(double a, double b, double c, double d, double e, int shift) {
+    double U = 0.0;
+    WENO_Reconstruction(a, b, c, d, e, shift, U);
+    return U;
+}
 }  // extern "C"

@@ -17,6 +17,10 @@ Fixtures:
                     Roe-1O residual, and its Write_Error_File rows of 3 TVD-RK3 steps
 
     python tests/golden/make_golden.py io    # only the two text files
+    python tests/golden/make_golden.py weno  # only weno_ref.npz
+
+  weno_ref.npz      WENO_Reconstruction (src/WENO2D.cpp:9-79) of 1203 five-cell stencils
+                    (smooth, stepped, constant, random), both shifts
 """
 from __future__ import annotations
 
@@ -95,10 +99,31 @@ def io_files(ref):
         ref.write_error(err_path)
 
 
+def weno_vectors(ref):
+    """WENO_Reconstruction (src/WENO2D.cpp:9-79) on smooth, stepped, constant and random stencils,
+    both shifts: weno_ref.npz (inputs [n, 5], outputs [n, 2])."""
+    rng = np.random.default_rng(2024)
+    x = np.arange(5, dtype=np.float64)
+    rows = [np.full(5, 1.3), np.array([1.0, 1.0, 1.0, 0.125, 0.125]), np.array([0.1, 0.1, 2.5, 2.5, 2.5])]
+    for _ in range(400):
+        c = rng.normal(size=4) * 10 ** rng.uniform(-4, 2)
+        rows.append(c[0] + c[1] * x + c[2] * x ** 2 + c[3] * x ** 3)          # smooth
+        k = rng.integers(1, 5)
+        rows.append(np.where(np.arange(5) < k, rng.uniform(0.1, 3), rng.uniform(0.1, 3)))  # step
+        rows.append(rng.normal(size=5) * 10 ** rng.uniform(-6, 4))             # random
+    inp = np.array(rows)
+    out = np.array([[ref.weno(*r, 0), ref.weno(*r, 1)] for r in inp])
+    np.savez_compressed(os.path.join(HERE, "weno_ref.npz"), stencil=inp, value=out)
+    print("wrote weno_ref", inp.shape)
+
+
 def main():
     ref = Reference()
     if sys.argv[1:] == ["io"]:
         io_files(ref)
+        return
+    if sys.argv[1:] == ["weno"]:
+        weno_vectors(ref)
         return
     out = {}
     for name in GEN_MESHES:
