@@ -1,6 +1,6 @@
 // Host mesh processing under the address and undefined-behaviour sanitizers (built with g++, no
-// GPU code): connectivity, tiled layouts in several shapes, the packed face records, and the
-// rejection of malformed input, on meshes given as files.
+// GPU code): connectivity, tiled layouts in several shapes, the packed face records, the WENO line
+// stencils, and the rejection of malformed input, on meshes given as files.
 //
 //   mesh_checks <mesh.bin> [<mesh.bin> ...]
 // mesh.bin: int32 npts, int32 ncells, npts x 3 doubles, ncells x 4 int32 (-1 pads triangles).
@@ -75,6 +75,30 @@ static void check_layout(const RefMesh &m, int tile, int renumber, int budget) {
             last = grp;
         }
     }
+    // WENO line stencils: all four cells or none; each a neighbour of the previous one on the line
+    std::vector<int> st;
+    line_stencils(m, L, st);
+    CHECK(st.size() == 4 * L.fL.size());
+    auto adjacent = [&](int a, int b) {  // renumbered cells
+        const int ca = L.perm[a], cb = L.perm[b];
+        for (int f = 0; f < m.nf[ca]; f++)
+            if (m.nbr[4 * (size_t)ca + f] == cb) return true;
+        return false;
+    };
+    size_t with_line = 0;
+    for (size_t e = 0; e < L.fL.size(); e++) {
+        const int *q = &st[4 * e];
+        const bool none = q[0] < 0 && q[1] < 0 && q[2] < 0 && q[3] < 0;
+        bool all = true;
+        for (int i = 0; i < 4; i++) all = all && q[i] >= 0 && q[i] < nc;
+        CHECK(none || all);
+        if (!all) continue;
+        with_line++;
+        CHECK(L.fR[e] >= 0);
+        CHECK(adjacent(L.fL[e], q[0]) && adjacent(q[0], q[1]));
+        CHECK(adjacent(L.fR[e], q[2]) && adjacent(q[2], q[3]));
+    }
+    CHECK(with_line > 0);
     std::vector<double> geo, w;
     std::vector<int> lr;
     pack_face_records(L, 7, geo, w, lr);
