@@ -26,14 +26,18 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 # rocprofv3 summaries of the flux kernel per workload, with calibrated FETCH_SIZE + WRITE_SIZE
-PROFILES = {"bump": "profiles/flux_tile_kernel_bump1400x700.md",
-            "ffs": "profiles/flux_tile_kernel_ffs4680x1560.md"}
+PROFILES = {("bump", "muscl"): "profiles/flux_tile_kernel_bump1400x700.md",
+            ("ffs", "muscl"): "profiles/flux_tile_kernel_ffs4680x1560.md",
+            ("bump", "weno"): "profiles/flux_tile_kernel_bump1400x700_weno5.md",
+            ("bump", "viscous"): "profiles/flux_tile_kernel_bump1400x700_viscous.md"}
 
 
-def measured_traffic(workload, config):
+def measured_traffic(key, config):
     """HBM bytes per launch of the flux kernel recorded in the committed profile summary for
-    this workload, when that summary was taken with the same launch configuration."""
-    path = os.path.join(ROOT, PROFILES[workload])
+    this workload and scheme, when that summary was taken with the same launch configuration."""
+    if key not in PROFILES:
+        return None
+    path = os.path.join(ROOT, PROFILES[key])
     if not os.path.exists(path):
         return None
     vals = {}
@@ -78,6 +82,10 @@ def parse():
     p.add_argument("--grad-lds", type=int, default=0)
     p.add_argument("--gradient", default="", choices=["", "gg", "wlsq"],
                    help="override the workload's gradient (bump: wlsq, ffs: gg)")
+    p.add_argument("--scheme", default="muscl", choices=["muscl", "weno", "viscous"],
+                   help="bump/ffs workloads: MUSCL (default); WENO5 (order 3, MUSCL on faces without a "
+                        "five-cell line of quadrilaterals; the bump then has no triangles); MUSCL with "
+                        "laminar viscous terms (Re 1e4 per unit length) and no-slip walls")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--kernel-reps", type=int, default=20)
     p.add_argument("--copy-calibration", action="store_true",
@@ -86,11 +94,18 @@ def parse():
     return p.parse_args()
 
 
-def workload(name):
+def workload(name, scheme="muscl"):
     from cfd_solver_withcuda_amd import meshgen
     from cfd_solver_withcuda_amd.solver import freestream
 
-    if name == "bump":
+    if name == "bump" and scheme == "weno":
+        # all (jittered) quadrilaterals, so that WENO5 serves every face away from the boundary
+        m = meshgen.bump_mesh(1400, 700, 0.10, seed=1, tri_frac=0.0)
+        fs = freestream(1.65)
+        cfg = dict(flux="hllc", order=2, limiter="vanleer", gradient="wlsq", cfl=0.8, rk=3, lts=1,
+                   bc=("freestream", "exit", "wall", "wall", "wall"), fs=fs)
+        desc = "bump_1400x700_quads_M1.65_hllc_muscl_vanleer_wlsq_lts_rk3"
+    elif name == "bump":
         m = meshgen.bump_mesh(1400, 700, 0.10, seed=1)
         fs = freestream(1.65)
         cfg = dict(flux="hllc", order=2, limiter="vanleer", gradient="wlsq", cfl=0.8, rk=3, lts=1,
@@ -109,9 +124,25 @@ def flux_pass_bytes(C, F, cfg):
     """Algorithmic bytes of the first-stage launch of the fused flux/stage pass. Per cell: U (32)
     + gradients (64) + area, 1/area (16) + face slots (8) read; new U (32) written, plus dt/area
     (8) with local time stepping and the accumulator (32) with RK4. Per face: L,R (8) + normal,
-    length (24) + centre offset (16)."""
-    per_cell = 32 + 64 + 16 + 8 + 32 + (8 if cfg["lts"] else 0) + (32 if cfg["rk"] == 4 else 0)
-    return per_cell * C + 48 * F
+    length (24) + centre offset (16), plus the line stencil (16) for WENO5, whose faces read no
+    gradients (only the MUSCL faces without a five-cell line do; not counted). The viscous terms
+    read nothing more."""
+    weno = cfg["order"] == 3
+    per_cell = 32 + (0 if weno else 64) + 16 + 8 + 32 + (8 if cfg["lts"] else 0) + (32 if cfg["rk"] == 4 else 0)
+    return per_cell * C + (64 if weno else 48) * F
+
+
+def apply_scheme(cfg, desc, scheme):
+    """The workload's configuration for --scheme (MUSCL as defined, WENO5, or viscous)."""
+    if scheme == "weno":
+        cfg = dict(cfg, order=3)
+        desc = desc.replace("_muscl_", "_weno5_")
+    elif scheme == "viscous":
+        rho, u = cfg["fs"][0], float(np.hypot(cfg["fs"][1], cfg["fs"][2]))
+        cfg = dict(cfg, viscous="constant", mu=rho * u / 1e4, prandtl=0.72, r_gas=1.0,
+                   bc=("freestream", "exit", "symmetry", "noslip_wall", "noslip_wall"))
+        desc = desc + "_laminar_Re1e4_noslip"
+    return cfg, desc
 
 
 def residual_contract(C, F, gradient, ms):
@@ -136,8 +167,9 @@ def cpu_baseline(m, cfg, U0, budget_s=15.0):
 
     orc = Oracle()
     om = orc.mesh(m.points, m.cells)
+    visc = {k: cfg[k] for k in ("viscous", "mu", "prandtl", "r_gas") if k in cfg}
     ocfg = make_cfg(flux=cfg["flux"], order=cfg["order"], limiter=cfg["limiter"],
-                    gradient=cfg["gradient"], roe_bdiss=1, cfl=cfg["cfl"], bc=cfg["bc"], fs=cfg["fs"])
+                    gradient=cfg["gradient"], roe_bdiss=1, cfl=cfg["cfl"], bc=cfg["bc"], fs=cfg["fs"], **visc)
     U = np.ascontiguousarray(U0.copy())
     n = 0
     t0 = time.perf_counter()
@@ -369,7 +401,8 @@ def main():
         return sod_bench(args)
     if args.workload == "steady":
         return steady_bench(args)
-    m, cfg, desc = workload(args.workload)
+    m, cfg, desc = workload(args.workload, args.scheme)
+    cfg, desc = apply_scheme(cfg, desc, args.scheme)
     if args.gradient and args.gradient != cfg["gradient"]:
         desc = desc.replace("_" + cfg["gradient"] + "_", "_" + args.gradient + "_")
         cfg["gradient"] = args.gradient
@@ -407,7 +440,8 @@ def main():
 
     t_step = ms_steps / 1e3 / args.steps
     launch_cfg = (f"workload={args.workload} tile={args.tile} face_budget={args.face_budget} "
-                  f"block={args.block} grad_lds={args.grad_lds}" + (f" gradient={args.gradient}" if args.gradient else ""))
+                  f"block={args.block} grad_lds={args.grad_lds}" + (f" gradient={args.gradient}" if args.gradient else "")
+                  + (f" scheme={args.scheme}" if args.scheme != "muscl" else ""))
     value = F * nres / t_step
     bflux = flux_pass_bytes(C, F, cfg)
     achieved = bflux / (ms_flux / 1e3) / 1e9
@@ -435,10 +469,10 @@ def main():
         "finite": ok,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(args.workload, launch_cfg),
+                     "traffic": measured_traffic((args.workload, args.scheme), launch_cfg),
                      "kernel": "flux_tile_kernel (fused face flux + gather + RK stage)",
                      "algorithmic_bytes_per_launch": bflux, "avg_launch_ms": ms_flux,
-                     "traffic_source": PROFILES[args.workload]},
+                     "traffic_source": PROFILES.get((args.workload, args.scheme))},
         "grad_pass": {"avg_launch_ms": ms_grad,
                       "algorithmic_bytes_per_launch": grad_pass_bytes(C, F, cfg["gradient"]),
                       "achieved_GBs": grad_pass_bytes(C, F, cfg["gradient"]) / (ms_grad / 1e3) / 1e9},

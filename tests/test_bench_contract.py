@@ -21,8 +21,10 @@ def test_more_than_one_gpu_is_refused():
 def test_algorithmic_byte_counts():
     C, F = 1000, 1800
     # flux pass: 152 B/cell (+8 with local time stepping, +32 with RK4) and 48 B/face
-    assert bench.flux_pass_bytes(C, F, dict(lts=1, rk=3)) == 160 * C + 48 * F
-    assert bench.flux_pass_bytes(C, F, dict(lts=0, rk=4)) == 184 * C + 48 * F
+    assert bench.flux_pass_bytes(C, F, dict(lts=1, rk=3, order=2)) == 160 * C + 48 * F
+    assert bench.flux_pass_bytes(C, F, dict(lts=0, rk=4, order=2)) == 184 * C + 48 * F
+    # WENO5 adds the 16-byte line stencil per face and reads no gradients on its faces
+    assert bench.flux_pass_bytes(C, F, dict(lts=1, rk=3, order=3)) == 96 * C + 64 * F
     # gradient pass: 104 B/cell (+8 for Green-Gauss's 1/area), 40 B/face WLSQ, 32 B/face GG
     assert bench.grad_pass_bytes(C, F, "wlsq") == 104 * C + 40 * F
     assert bench.grad_pass_bytes(C, F, "gg") == 112 * C + 32 * F
@@ -31,5 +33,5 @@ def test_algorithmic_byte_counts():
 def test_committed_profiles_carry_the_traffic_of_the_default_launch():
     launch = "workload={w} tile=128 face_budget=256 block=128 grad_lds=0"
     for w in ("bump", "ffs"):
-        t = bench.measured_traffic(w, launch.format(w=w))
+        t = bench.measured_traffic((w, "muscl"), launch.format(w=w))
         assert t is not None and t > 0, w
