@@ -89,11 +89,15 @@ struct cfd_mesh_s {
     // second order)
     int *d_rc_nf = nullptr, *d_rc_nbr = nullptr;
     double *d_rc_dist = nullptr, *d_rc_nrm = nullptr, *d_rc_len = nullptr, *d_rc_area = nullptr;
-    int4 *d_fst = nullptr;  // WENO line stencils of the face entries (built on first use)
+    int *d_fst = nullptr;  // WENO line stencils of the face entries, 4 per entry (built on first use)
+    // WENO: the tiles owning a cell of a face without a line stencil (the only gradients the
+    // MUSCL fallback faces read), as a tile list for the gradient pass of the time steps
+    int *d_gtiles = nullptr, *d_gcount = nullptr;
+    int ngtiles = 0;
     DevMesh dev{};
     ~cfd_mesh_s() {
         void *ps[] = {d_tile_cstart, d_tile_fstart, d_fgeo, d_fw, d_fnx, d_fny, d_fdl, d_fLR, d_slots, d_area, d_inv_area, d_gcell, d_gkind, d_gnrm,
-                      d_rc_nf, d_rc_nbr, d_rc_dist, d_rc_nrm, d_rc_len, d_rc_area, d_fst};
+                      d_rc_nf, d_rc_nbr, d_rc_dist, d_rc_nrm, d_rc_len, d_rc_area, d_fst, d_gtiles, d_gcount};
         for (void *p : ps)
             if (p) (void)hipFree(p);
     }
@@ -317,19 +321,43 @@ static int check_config(const cfd_mesh_s *m, const cfd_config_t *c) {
 // WENO line stencils of the mesh's face entries on the device (first order-3 configuration).
 static int ensure_line_stencils(cfd_mesh_s *m) {
     if (m->d_fst) return 0;
+    const Layout &L = m->lay;
     std::vector<int> st;
-    line_stencils(m->ref, m->lay, st);
-    int4 *d = nullptr;
-    HIP_TRY(hipMalloc((void **)&d, std::max<size_t>(st.size(), 4) * sizeof(int)));
-    if (!st.empty()) {
-        hipError_t e = hipMemcpy(d, st.data(), st.size() * sizeof(int), hipMemcpyHostToDevice);
-        if (e != hipSuccess) {
-            (void)hipFree(d);
-            return hip_fail(e, "ensure_line_stencils hipMemcpy");
+    line_stencils(m->ref, L, st);
+    std::vector<int> tile_of(L.perm.size());
+    for (int t = 0; t < L.ntiles; t++)
+        for (int k = L.tile_cstart[t]; k < L.tile_cstart[t + 1]; k++) tile_of[k] = t;
+    std::vector<char> need(L.ntiles, 0);
+    for (size_t e = 0; e < L.fL.size(); e++)
+        if (st[4 * e] < 0) {
+            need[tile_of[L.fL[e]]] = 1;
+            if (L.fR[e] >= 0) need[tile_of[L.fR[e]]] = 1;
         }
+    std::vector<int> gt;
+    for (int t = 0; t < L.ntiles; t++)
+        if (need[t]) gt.push_back(t);
+    const int count = (int)gt.size();
+    HIP_TRY(upload(&m->d_gtiles, gt.empty() ? std::vector<int>{0} : gt));
+    HIP_TRY(upload(&m->d_gcount, std::vector<int>{count}));
+    m->ngtiles = count;
+    HIP_TRY(upload(&m->d_fst, st));
+    m->dev.fst = reinterpret_cast<const int4 *>(m->d_fst);
+    return 0;
+}
+
+// Gradient pass of a time step: at order 3 only over the tiles whose gradients a MUSCL face
+// reads (none on an all-quadrilateral line mesh away from its boundary).
+static int step_grad(cfd_solver_s *s, const double4 *in, hipStream_t st) {
+    const cfd_config_t &c = s->cfg;
+    if (c.order == 2) {
+        HIP_TRY(launch_grad(s->m->dev, s->dcfg, c.gradient, in, s->grad, c.block, st));
+    } else if (c.order == 3 && s->m->ngtiles > 0) {
+        DevMesh dm = s->m->dev;
+        dm.tile_list = s->m->d_gtiles;
+        dm.tile_count = s->m->d_gcount;
+        dm.ntiles = s->m->ngtiles;
+        HIP_TRY(launch_grad(dm, s->dcfg, c.gradient, in, s->grad, c.block, st));
     }
-    m->d_fst = d;
-    m->dev.fst = d;
     return 0;
 }
 
@@ -577,7 +605,7 @@ static int native_step(cfd_solver_s *s, bool want_err, hipStream_t st) {
     for (int k = 0; k < nst; k++) {
         const double4 *in = (nst == 4) ? ins4[k] : ins3[k];
         double4 *out = (nst == 4) ? outs4[k] : outs3[k];
-        if (c.order >= 2) HIP_TRY(launch_grad(dm, s->dcfg, c.gradient, in, s->grad, c.block, st));
+        if (int rc = step_grad(s, in, st)) return rc;
         StageArgs a{};
         a.rk = nst;
         a.stage = k;
