@@ -108,14 +108,18 @@ def write_vtk_solution(path: str, points: np.ndarray, cells: np.ndarray, U: np.n
             f.write(f"{_g(a)}\t{_g(b)}\t0\n")
 
 
-# The reference's Dissipation_Type (src/Net_Flux.cpp:74-91): 1 LLF, 3 ROE are on this path.
-_DISSIPATION = {1: "llf", 3: "roe"}
+# The reference's Dissipation_Type (src/Net_Flux.cpp:74-91): 1 LLF, 3 ROE are on this path; 101
+# HLLC and 102 AUSM are the values libcfdhost's driver accepts for the other two fluxes.
+_DISSIPATION = {1: "llf", 3: "roe", 101: "hllc", 102: "ausm"}
 
 
 def config_from_json(path: str) -> dict:
     """make_config() keyword arguments from the reference's solver JSON (comments allowed, as
-    the reference's jsoncpp reader allows them). Keys read: Simulation.CFL, Solver.Dissipation_Type,
-    Solver.Is_Second_Order, Solver.Local_Time_Stepping, Solver.Time_Accurate."""
+    the reference's jsoncpp reader allows them). Keys read (src/Configuration_Read.cpp:28-51):
+    Simulation.CFL, Solver.Dissipation_Type, Solver.Is_Second_Order, Solver.Local_Time_Stepping,
+    Solver.Is_WENO (order 3; minmod / Green-Gauss on the faces without a quadrilateral line) and
+    Solver.Is_Viscous (laminar terms with Sutherland's law, Pr 0.72, R 287; the caller gives the
+    walls ghost type "noslip_wall", the reference's Is_Viscous_Wall)."""
     text = open(path).read()
     text = re.sub(r"//[^\n]*", "", text)
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
@@ -127,8 +131,14 @@ def config_from_json(path: str) -> dict:
     kw = dict(flux=_DISSIPATION[diss], order=2 if sol.get("Is_Second_Order", False) else 1,
               cfl=float(sim.get("CFL", 0.5)), lts=1 if sol.get("Local_Time_Stepping", False) else 0,
               rk=3)
-    if kw["order"] == 2:
+    if sol.get("Is_WENO", False):
+        kw["order"] = 3
+    if kw["order"] >= 2:
         kw.update(limiter="minmod", gradient="gg")
+    if sol.get("Is_Viscous", False):
+        if kw["order"] != 2:
+            raise ValueError("Is_Viscous needs Is_Second_Order without Is_WENO on this path")
+        kw.update(viscous="sutherland", prandtl=0.72, r_gas=287.0)
     return kw
 
 
