@@ -73,6 +73,7 @@ struct cfd_batch_s {
     double *d_fgeo = nullptr, *d_fw = nullptr, *d_area = nullptr, *d_inv_area = nullptr;
     double *d_fnx = nullptr, *d_fny = nullptr, *d_fdl = nullptr;
     int *d_fLR = nullptr;
+    int *d_fst = nullptr;  // WENO line stencils, 4 per face entry (order 3)
     uint16_t *d_slots = nullptr;
     int *d_tile_inst = nullptr, *d_inst_tile = nullptr, *d_tile_list = nullptr, *d_tile_count = nullptr;
     int *d_inst_off = nullptr, *d_status = nullptr, *d_steps = nullptr;
@@ -100,7 +101,7 @@ struct cfd_batch_s {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         if (stream) (void)hipStreamDestroy(stream);
-        void *ps[] = {d_tile_cstart, d_tile_fstart, d_fgeo, d_fw, d_fnx, d_fny, d_fdl, d_fLR, d_area, d_inv_area, d_slots, d_tile_inst, d_inst_tile,
+        void *ps[] = {d_tile_cstart, d_tile_fstart, d_fgeo, d_fw, d_fnx, d_fny, d_fdl, d_fLR, d_fst, d_area, d_inv_area, d_slots, d_tile_inst, d_inst_tile,
                       d_tile_list, d_tile_count, d_inst_off, d_status, d_steps, d_inst_fs, d_err4,
                       Un, Ua, Ub, acc, grad, hbuf, err_partial, d_fz, d_fz_since, d_fz_best, psi};
         for (void *p : ps)
@@ -118,7 +119,9 @@ struct InstanceInput {
 
 int check_batch_config(const cfd_config_t *c) {
     if (c->flux < 0 || c->flux > 3) return capi_fail(CFD_E_ARG, "batch config: flux must be 0..3");
-    if (c->order != 1 && c->order != 2) return capi_fail(CFD_E_ARG, "batch config: order must be 1 or 2");
+    if (c->order < 1 || c->order > 3) return capi_fail(CFD_E_ARG, "batch config: order must be 1, 2 or 3 (WENO5)");
+    if (c->order == 3 && (c->freeze_tol > 0.0 || c->freeze_stall > 0))
+        return capi_fail(CFD_E_ARG, "batch config: limiter freezing needs order 2");
     if (c->limiter < 0 || c->limiter > 3) return capi_fail(CFD_E_ARG, "batch config: limiter must be 0..3");
     if (c->gradient < 0 || c->gradient > 1) return capi_fail(CFD_E_ARG, "batch config: gradient must be 0 or 1");
     if (!(c->cfl > 0.0)) return capi_fail(CFD_E_ARG, "batch config: cfl must be positive");
@@ -139,6 +142,7 @@ int build_batch(const std::vector<InstanceInput> &in, const double *fs4, const c
     *out = nullptr;
     const int ni = (int)in.size();
     std::vector<Layout> lays(ni);
+    std::vector<std::vector<int>> sts(ni);  // WENO line stencils (order 3)
     std::vector<std::string> errs(ni);
     {
         std::atomic<int> next{0};
@@ -150,6 +154,7 @@ int build_batch(const std::vector<InstanceInput> &in, const double *fs4, const c
                     RefMesh rm;
                     build_ref_mesh(in[i].npts, in[i].pts, in[i].ncells, in[i].cells, rm);
                     build_layout(rm, tile, 1, lays[i], face_budget);
+                    if (cfg.order == 3) line_stencils(rm, lays[i], sts[i]);
                 } catch (const std::exception &e) {
                     errs[i] = e.what();
                 }
@@ -213,7 +218,8 @@ int build_batch(const std::vector<InstanceInput> &in, const double *fs4, const c
     BALLOC(b->Ua, nc);
     BALLOC(b->Ub, nc);
     if (cfg.rk == 4) BALLOC(b->acc, nc);
-    if (cfg.order == 2) BALLOC(b->grad, 8 * nc);
+    if (cfg.order >= 2) BALLOC(b->grad, 8 * nc);
+    if (cfg.order == 3) BALLOC(b->d_fst, 4 * nfe);
     BALLOC(b->hbuf, nc);
     BALLOC(b->err_partial, 4 * nt);
     if (cfg.order == 2 && (cfg.freeze_tol > 0.0 || cfg.freeze_stall > 0)) {
@@ -236,6 +242,16 @@ int build_batch(const std::vector<InstanceInput> &in, const double *fs4, const c
         std::vector<double> geo, w;
         std::vector<int> lr;
         pack_face_records(L, cb, geo, w, lr);
+        if (cfg.order == 3) {
+            std::vector<int> &st = sts[i];
+            for (int &v : st)
+                if (v >= 0) v += cb;
+            if ((e = h2d(b->d_fst + 4 * (size_t)fb, st.data(), 4 * nfi)) != hipSuccess) {
+                delete b;
+                return capi_hip_fail(e, "cfd_batch_create upload");
+            }
+            std::vector<int>().swap(st);
+        }
         for (size_t c = 0; c < nci; c++) b->iperm[cb + c] = cb + L.iperm[c];
         if ((e = h2d(b->d_tile_cstart + tb, tc.data(), nti)) != hipSuccess ||
             (e = h2d(b->d_tile_fstart + tb, tf.data(), nti)) != hipSuccess ||
@@ -276,6 +292,7 @@ int build_batch(const std::vector<InstanceInput> &in, const double *fs4, const c
     d.fgeo = reinterpret_cast<const double2 *>(b->d_fgeo);
     d.fw = reinterpret_cast<const double2 *>(b->d_fw);
     d.fLR = reinterpret_cast<const int2 *>(b->d_fLR);
+    d.fst = reinterpret_cast<const int4 *>(b->d_fst);
     d.fnx = b->d_fnx;
     d.fny = b->d_fny;
     d.fdl = b->d_fdl;
@@ -329,7 +346,7 @@ int batch_step_launches(cfd_batch_s *b, double tol) {
     for (int k = 0; k < nst; k++) {
         const double4 *in = (nst == 4) ? ins4[k] : ins3[k];
         double4 *out = (nst == 4) ? outs4[k] : outs3[k];
-        if (c.order == 2) BHIP(launch_grad(b->dev, b->dcfg, c.gradient, in, b->grad, b->block, b->stream));
+        if (c.order >= 2) BHIP(launch_grad(b->dev, b->dcfg, c.gradient, in, b->grad, b->block, b->stream));
         StageArgs a{};
         a.rk = nst;
         a.stage = k;

@@ -1,7 +1,8 @@
 """GPU parity of WENO5 (order 3) against the CPU oracle: residual, local time step and gradients
 bit for bit on uniform quads, on a jittered mixed bump mesh (faces without a complete line fall
 back to MUSCL) and on a triangle-heavy mesh, for every flux; independence of the tiling; fused
-RK3/RK4 steps and a Sod march bit for bit; unsupported combinations rejected."""
+RK3/RK4 steps and a Sod march bit for bit; batched WENO solves bit for bit as single solvers;
+unsupported combinations rejected."""
 import numpy as np
 import pytest
 
@@ -127,8 +128,27 @@ def test_weno_unsupported_combinations_rejected(bad):
         Solver(Mesh(m.points, m.cells), make_config(**kw))
 
 
-def test_weno_not_batched():
+def test_weno_batch_bit_exact_vs_single_solvers():
+    """Three ramps of uneven size (quadrilateral lines and triangles) in one batch: 5 WENO steps
+    equal each mesh's own solver bit for bit."""
+    print("case weno batch vs single, 3 ramps, 5 steps", flush=True)
+    specs = [(24, 12, 8.0, 2.0, 41), (40, 18, 15.0, 3.0, 42), (16, 10, 5.0, 1.6, 43)]
+    meshes = [meshgen.ramp_mesh(nx, ny, th, seed=sd) for nx, ny, th, _, sd in specs]
+    fs = np.array([freestream(M) for *_, M, _ in specs])
+    kw = dict(flux="hllc", order=3, limiter="vanleer", gradient="wlsq", cfl=0.8, rk=3, lts=1,
+              bc=("freestream", "exit", "exit", "wall", "wall"), block=128)
+    b = Batch(meshes, fs, make_config(**kw))
+    b.solve(tol=-1.0, max_steps=5, check_every=5)
+    Ub = b.get_state()
+    for i, m in enumerate(meshes):
+        s = Solver(Mesh(m.points, m.cells, tile=128, face_budget=256), make_config(fs=fs[i], **kw))
+        s.set_state(np.tile(cons_from_prim(np.array([fs[i][0]]), fs[i][1], fs[i][2], fs[i][3]), (len(m.cells), 1)))
+        s.step(5)
+        assert np.array_equal(Ub[i], s.get_state()), i
+
+
+def test_weno_batch_rejects_freezing():
     m = meshgen.ramp_mesh(16, 8, 10.0, seed=1)
-    with pytest.raises(CfdError, match="order"):
-        Batch([m], np.array([freestream(2.0)]), make_config(flux="hllc", order=3, rk=3, lts=1,
+    with pytest.raises(CfdError, match="freezing"):
+        Batch([m], np.array([freestream(2.0)]), make_config(flux="hllc", order=3, rk=3, lts=1, freeze_stall=20,
                                                                bc=("freestream", "exit", "exit", "wall", "wall")))
