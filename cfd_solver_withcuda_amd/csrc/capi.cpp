@@ -297,7 +297,6 @@ int cfd_mesh_layout_export(const cfd_mesh_t *m, int *tile_cstart, int *tile_fsta
 static int check_config(const cfd_mesh_s *m, const cfd_config_t *c) {
     if (c->flux < 0 || c->flux > 3) return fail(CFD_E_ARG, "config: flux must be 0..3");
     if (c->order < 1 || c->order > 3) return fail(CFD_E_ARG, "config: order must be 1, 2 or 3 (WENO5)");
-    if (c->order == 3 && c->ref_mode) return fail(CFD_E_ARG, "config: WENO5 (order 3) exists in the native iteration only");
     if (c->order == 3 && (c->freeze_tol > 0.0 || c->freeze_stall > 0))
         return fail(CFD_E_ARG, "config: limiter freezing needs order 2");
     if (c->limiter < 0 || c->limiter > 3) return fail(CFD_E_ARG, "config: limiter must be 0..3");

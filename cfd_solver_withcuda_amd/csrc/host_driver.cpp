@@ -18,6 +18,7 @@ int No_Physical_Cells = 0, Total_No_Cells = 0;
 double CFL = 0.5;
 int Dissipation_Type = 3;
 bool Is_Second_Order = false, Local_Time_Stepping = false, Is_Inlet_SubSonic = false, Is_Exit_SubSonic = false;
+bool Is_WENO = false;
 int Limiter_Type = 2, Gradient_Type = 1;
 double Limiter_Zeta = 0.5, Limiter_Zeta1 = 0.5;
 InletCondition inletCond;
@@ -50,8 +51,11 @@ cfd_config_t config(int order, int rk, int ref_mode) {
     case 102: c.flux = 3; break;  // AUSM
     default: throw std::runtime_error("Dissipation_Type must be 1 (LLF), 3 (ROE), 101 (HLLC) or 102 (AUSM)");
     }
+    // WENO faces use the Rusanov (LLF) flux whatever Dissipation_Type (Calculate_Face_WENO_Flux,
+    // src/WENO2D.cpp:458-555)
+    if (order == 3) c.flux = 2;
     c.order = order;
-    c.limiter = (order == 2) ? Limiter_Type : 0;
+    c.limiter = (order >= 2) ? Limiter_Type : 0;
     c.gradient = Gradient_Type;
     c.roe_bdiss = 0;
     c.cfl = CFL;
@@ -76,6 +80,10 @@ cfd_config_t config(int order, int rk, int ref_mode) {
     return c;
 }
 
+// The iterations' flux evaluation: WENO5 when Is_WENO (src/Numerical_Method.cpp:31-53, 350-360),
+// else second or first order.
+int scheme_order() { return Is_WENO ? 3 : (Is_Second_Order ? 2 : 1); }
+
 void configure(int order, int rk, int ref_mode) {
     cfd_config_t c = config(order, rk, ref_mode);
     check(cfd_solver_configure(g_solver, &c), "configure");
@@ -97,7 +105,7 @@ void sync_state() {
 
 void one_iteration(int rk, const char *where) {
     need_mesh(where);
-    configure(Is_Second_Order ? 2 : 1, rk, 1);
+    configure(scheme_order(), rk, 1);
     sync_state();
     const std::vector<double> before = g_dev;
     double dt = 0.0;
@@ -193,7 +201,7 @@ bool Read_VTK_Grid(const std::string &file) {
 
 void Apply_Boundary_Conditions() {
     need_mesh("Apply_Boundary_Conditions");
-    configure(Is_Second_Order ? 2 : 1, 3, 0);
+    configure(scheme_order(), 3, 0);
     sync_state();
     const int ng = Total_No_Cells - No_Physical_Cells;
     std::vector<double> g(4 * (size_t)ng);
@@ -204,6 +212,7 @@ void Apply_Boundary_Conditions() {
 
 void Evaluate_Cell_Net_Flux_1O() { residual(1, "Evaluate_Cell_Net_Flux_1O"); }
 void Evaluate_Cell_Net_Flux_2O() { residual(2, "Evaluate_Cell_Net_Flux_2O"); }
+void Evaluate_Cell_Net_Flux_WENO() { residual(3, "Evaluate_Cell_Net_Flux_WENO"); }
 void Explicit_Method() { one_iteration(1, "Explicit_Method"); }
 void Runge_Kutta_Method() { one_iteration(3, "Runge_Kutta_Method"); }
 

@@ -4,6 +4,7 @@
 //   Read_VTK_Grid(file) + Form_Cells()  src/Read_Gmsh_File.cpp:11-144, src/Grid_Computations.cpp:319-392
 //   Apply_Boundary_Conditions()         src/Boundary_Conditions.cpp:12-61     (include/Boundary_Conditions.h:11)
 //   Evaluate_Cell_Net_Flux_1O/2O()      src/Net_Flux.cpp:46-128               (include/Flux.h:11-12)
+//   Evaluate_Cell_Net_Flux_WENO()       src/WENO2D.cpp:395-456
 //   Explicit_Method()                   src/Numerical_Method.cpp:26-103       (include/Solver.h:11)
 //   Runge_Kutta_Method()                src/Numerical_Method.cpp:342-457      (include/Solver.h:12)
 //   Estimate_Error(), Update()          src/Error_Estimate_Update.cpp:10-44   (include/Error_Update.h:10-11)
@@ -40,6 +41,7 @@ extern int No_Physical_Cells, Total_No_Cells;
 extern double CFL;
 extern int Dissipation_Type;  // 1 LLF, 3 ROE (reference); 101 HLLC, 102 AUSM (this library)
 extern bool Is_Second_Order, Local_Time_Stepping, Is_Inlet_SubSonic, Is_Exit_SubSonic;
+extern bool Is_WENO;           // WENO5 in the iterations (Rusanov flux, src/WENO2D.cpp:458-555)
 extern int Limiter_Type;      // second order: 1 minmod, 2 Van Leer, 3 superbee
 extern int Gradient_Type;     // second order: 0 Green-Gauss, 1 weighted least squares
 extern double Limiter_Zeta, Limiter_Zeta1;  // the reference's ROE_2O limiter coefficients
@@ -60,6 +62,9 @@ bool Form_Cells(int npts, const double *pts3, int ncells, const int *cells4);
 void Apply_Boundary_Conditions();
 void Evaluate_Cell_Net_Flux_1O();
 void Evaluate_Cell_Net_Flux_2O();
+// WENO5 net flux and time step (src/WENO2D.cpp:395-456): along lines of quadrilaterals, MUSCL
+// (Limiter_Type, Gradient_Type) on the other faces, where the reference leaves zero.
+void Evaluate_Cell_Net_Flux_WENO();
 void Explicit_Method();
 void Runge_Kutta_Method();
 void Estimate_Error();

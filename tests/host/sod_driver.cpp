@@ -2,20 +2,23 @@
 // host driver (include/cfd_host.hpp): Apply_Boundary_Conditions -> Runge_Kutta_Method ->
 // Estimate_Error -> Update, Roe first order with Harten's fix, CFL 0.5, supersonic inlet.
 //
-//   sod_driver <grid.vtk> <U0.bin> <nsteps> <out.bin> [solution.txt error.txt]
+//   sod_driver <grid.vtk> <U0.bin> <nsteps> <out.bin> [solution.txt error.txt | weno]
 // U0.bin: No_Physical_Cells x 4 doubles. out.bin: final U (N x 4), then per step Error[0..3]
 // and Min_dt (nsteps x 5 doubles), then the first call's Cells_Net_Flux (N x 4) and
-// Cells_del_t (N) from Evaluate_Cell_Net_Flux_1O.
+// Cells_del_t (N) from Evaluate_Cell_Net_Flux_1O (with `weno`: Is_WENO set, the iterations use
+// WENO5 and the first call is Evaluate_Cell_Net_Flux_WENO).
 #include <cstdio>
 #include <fstream>
 #include <iostream>
+#include <string>
 #include <vector>
 
 #include "cfd_host.hpp"
 
 int main(int argc, char **argv) {
-    if (argc != 5 && argc != 7) {
-        std::cerr << "usage: sod_driver grid.vtk U0.bin nsteps out.bin [solution.txt error.txt]\n";
+    const bool weno = argc == 6 && std::string(argv[5]) == "weno";
+    if (argc != 5 && argc != 7 && !weno) {
+        std::cerr << "usage: sod_driver grid.vtk U0.bin nsteps out.bin [solution.txt error.txt | weno]\n";
         return 2;
     }
     if (!Read_VTK_Grid(argv[1])) {
@@ -38,8 +41,12 @@ int main(int argc, char **argv) {
     Local_Time_Stepping = false;
     inletCond.Rho = 1.0;
     inletCond.P = 1.0;
+    Is_WENO = weno;
     std::vector<double> out;
-    Evaluate_Cell_Net_Flux_1O();
+    if (weno)
+        Evaluate_Cell_Net_Flux_WENO();
+    else
+        Evaluate_Cell_Net_Flux_1O();
     std::vector<V_D> R0 = Cells_Net_Flux;
     V_D dt0 = Cells_del_t;
     if (argc == 7) {  // the reference's files of the initial state

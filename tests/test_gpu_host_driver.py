@@ -45,3 +45,39 @@ def test_host_driver_sod_vs_reference_golden(tmp_path):
     # Write_Solution / Write_Error_File byte for byte as the reference writes them
     assert sol.read_text() == open(os.path.join(ROOT, "tests", "golden", "sod_solution_ref.txt")).read()
     assert errf.read_text() == open(os.path.join(ROOT, "tests", "golden", "sod_error_ref.txt")).read()
+
+
+def test_host_driver_weno_vs_oracle(tmp_path):
+    """Is_WENO set: Evaluate_Cell_Net_Flux_WENO of the Sod state and 10 reference iterations
+    (Runge_Kutta_Method, Rusanov flux on WENO5 states, MUSCL/Van Leer + WLSQ on the faces without a
+    quadrilateral line) equal the oracle's bit for bit."""
+    from oracle.oracle import Oracle, make_cfg
+
+    assert os.path.exists(EXE), "build first: python -m cfd_solver_withcuda_amd.build"
+    g = np.load(os.path.join(ROOT, "tests", "golden", "sod_roe.npz"), allow_pickle=False)
+    m = meshgen.shock_tube_mesh(100, 10)
+    vtk = tmp_path / "sod.vtk"
+    meshgen.write_vtk(str(vtk), m)
+    u0 = tmp_path / "U0.bin"
+    U0 = np.ascontiguousarray(g["U0"], np.float64)
+    U0.tofile(u0)
+    out = tmp_path / "out.bin"
+    nsteps = 10
+    r = subprocess.run([EXE, str(vtk), str(u0), str(nsteps), str(out), "weno"], capture_output=True, text=True,
+                       timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    n = len(m.cells)
+    d = np.fromfile(out, np.float64)
+    U = d[:4 * n].reshape(n, 4)
+    R0 = d[4 * n + 5 * nsteps:8 * n + 5 * nsteps].reshape(n, 4)
+    dt0 = d[8 * n + 5 * nsteps:]
+    om = Oracle().mesh(m.points, m.cells)
+    cfg = make_cfg(flux="llf", order=3, limiter="vanleer", gradient="wlsq", roe_bdiss=0, cfl=0.5,
+                   bc=("inlet_ref", "exit", "wall", "wall", "wall"), fs=(1.0, 0.0, 0.0, 1.0))
+    Ro, dto, _ = om.residual(cfg, U0)
+    assert np.array_equal(R0, Ro) and np.array_equal(dt0, dto)
+    Uo = U0.copy()
+    for _ in range(nsteps):
+        om.step(cfg, Uo, rk=3, lts=0, ref=True)
+    assert np.array_equal(U, Uo), float(np.abs(U - Uo).max())

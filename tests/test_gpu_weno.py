@@ -2,7 +2,7 @@
 bit for bit on uniform quads, on a jittered mixed bump mesh (faces without a complete line fall
 back to MUSCL) and on a triangle-heavy mesh, for every flux; independence of the tiling; fused
 RK3/RK4 steps and a Sod march bit for bit; batched WENO solves bit for bit as single solvers;
-unsupported combinations rejected."""
+the reference's iteration with WENO stages; unsupported combinations rejected."""
 import numpy as np
 import pytest
 
@@ -118,8 +118,25 @@ def test_weno_sod_march_vs_oracle(orc):
     assert np.array_equal(s.get_state(), Uo)
 
 
-@pytest.mark.parametrize("bad", [dict(ref_mode=1), dict(viscous="constant", mu=1e-3), dict(freeze_tol=1e-3),
-                                 dict(freeze_stall=50)])
+@pytest.mark.parametrize("rk", [1, 3])
+def test_weno_reference_iteration_vs_oracle(orc, rk):
+    """The reference's iteration (ref_mode: per-stage global dt, delta form, exit ghosts from U^n)
+    with WENO5 stages, forward Euler or RK3: 8 steps bit for bit."""
+    print(f"case weno reference iteration rk={rk}", flush=True)
+    m = MESHES["quads"]()
+    om = orc.mesh(m.points, m.cells)
+    U0 = smooth_state(om.arrays().ctr, om.nphys)
+    kw = dict(flux="llf", order=3, limiter="vanleer", gradient="wlsq", cfl=0.5, bc=BCS, fs=FS)
+    Uo = np.ascontiguousarray(U0.copy())
+    for _ in range(8):
+        om.step(make_cfg(**kw), Uo, rk=rk, lts=0, ref=True)
+    s = Solver(Mesh(m.points, m.cells, tile=64), make_config(rk=rk, lts=0, ref_mode=1, **kw))
+    s.set_state(U0)
+    s.step(8)
+    assert np.array_equal(s.get_state(), Uo), rel(s.get_state(), Uo)
+
+
+@pytest.mark.parametrize("bad", [dict(viscous="constant", mu=1e-3), dict(freeze_tol=1e-3), dict(freeze_stall=50)])
 def test_weno_unsupported_combinations_rejected(bad):
     m = meshgen.shock_tube_mesh(8, 4)
     kw = dict(flux="hllc", order=3, limiter="vanleer", gradient="wlsq", cfl=0.5, fs=FS, bc=BCS)
