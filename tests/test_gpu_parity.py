@@ -270,3 +270,42 @@ def test_reference_second_order_roe_vs_golden():
     assert np.array_equal(dt, g["dt"])
     s.step(3)
     assert np.all(np.isfinite(s.get_state()))
+
+
+def test_steps_follow_reconfiguration_and_freezing(orc, bump):
+    """Changing the Courant number, the flux and the time stepping between steps of one solver,
+    then switching limiter freezing on (which allocates its buffers), gives the oracle's states bit
+    for bit after every segment."""
+    print("case steps across reconfiguration", flush=True)
+    om = orc.mesh(bump.points, bump.cells)
+    U0 = smooth_state(om.arrays().ctr, om.nphys)
+    base = dict(order=2, limiter="vanleer", gradient="wlsq", bc=BCS_NATIVE, fs=FS, roe_bdiss=1)
+    segs = [dict(flux="hllc", cfl=0.6, rk=3, lts=1), dict(flux="hllc", cfl=0.4, rk=3, lts=1),
+            dict(flux="roe", cfl=0.5, rk=4, lts=0), dict(flux="roe", cfl=0.5, rk=3, lts=0)]
+    s = Solver(Mesh(bump.points, bump.cells, tile=64), make_config(**base, **segs[0]))
+    s.set_state(U0)
+    Uo = np.ascontiguousarray(U0.copy())
+    for i, sg in enumerate(segs):
+        s.configure(make_config(**base, **sg))
+        s.step(3)
+        ocfg = make_cfg(flux=sg["flux"], order=2, limiter="vanleer", gradient="wlsq", roe_bdiss=1,
+                        cfl=sg["cfl"], bc=BCS_NATIVE, fs=FS)
+        for _ in range(3):
+            om.step(ocfg, Uo, rk=sg["rk"], lts=sg["lts"])
+        assert np.array_equal(s.get_state(), Uo), (i, rel(s.get_state(), Uo))
+    # freezing: record on the next step, then reuse (buffers allocated between captured steps)
+    s.configure(make_config(**base, **segs[0]))
+    s.set_freeze(1)
+    s.step(1)
+    s.set_freeze(2)
+    s.step(2)
+    phi = np.zeros(8 * om.nfaces, np.float32)
+    ocfg = make_cfg(flux="hllc", order=2, limiter="vanleer", gradient="wlsq", roe_bdiss=1, cfl=0.6,
+                    bc=BCS_NATIVE, fs=FS)
+    ocfg.phi = phi.ctypes.data
+    ocfg.freeze = 1
+    om.step(ocfg, Uo, rk=3, lts=1)
+    ocfg.freeze = 2
+    for _ in range(2):
+        om.step(ocfg, Uo, rk=3, lts=1)
+    assert np.array_equal(s.get_state(), Uo), rel(s.get_state(), Uo)
