@@ -56,7 +56,7 @@ int capi_hip_fail(hipError_t e, const char *where) { return hip_fail(e, where); 
 int capi_check_viscous(const cfd_config_t *c, const char *who) {
     if (c->viscous < 0 || c->viscous > 2) return fail(CFD_E_ARG, std::string(who) + ": viscous must be 0, 1 or 2");
     if (!c->viscous) return 0;
-    if (c->order != 2) return fail(CFD_E_ARG, std::string(who) + ": viscous terms need order 2 (cell gradients)");
+    if (c->order < 2) return fail(CFD_E_ARG, std::string(who) + ": viscous terms need order 2 or 3 (cell gradients)");
     if (c->ref_mode) return fail(CFD_E_ARG, std::string(who) + ": viscous terms exist in the native iteration only");
     if (!(c->prandtl > 0.0) || !std::isfinite(c->prandtl)) return fail(CFD_E_ARG, std::string(who) + ": prandtl must be positive");
     if (!(c->r_gas > 0.0) || !std::isfinite(c->r_gas)) return fail(CFD_E_ARG, std::string(who) + ": r_gas must be positive");
@@ -349,6 +349,8 @@ static int ensure_line_stencils(cfd_mesh_s *m) {
 static int step_grad(cfd_solver_s *s, const double4 *in, hipStream_t st) {
     const cfd_config_t &c = s->cfg;
     if (c.order == 2) {
+        HIP_TRY(launch_grad(s->m->dev, s->dcfg, c.gradient, in, s->grad, c.block, st));
+    } else if (c.order == 3 && c.viscous) {  // every face reads gradients
         HIP_TRY(launch_grad(s->m->dev, s->dcfg, c.gradient, in, s->grad, c.block, st));
     } else if (c.order == 3 && s->m->ngtiles > 0) {
         DevMesh dm = s->m->dev;

@@ -1,6 +1,7 @@
 // Flux pass of the native finite-volume path, shared by the translation units that
 // instantiate it (fv_kernels.hip: inviscid first and second order; fv_visc.hip: second order
-// with viscous terms; fv_weno.hip: WENO5, order 3), plus the tile helpers the other kernels use.
+// with viscous terms; fv_weno.hip and fv_weno_visc.hip: WENO5, order 3, without and with viscous
+// terms), plus the tile helpers the other kernels use.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -233,6 +234,22 @@ __device__ __forceinline__ void flux_face(const DevMesh &m, const DevCfg &cfg,
             weno_face_state(UL, SL, fL);
             weno_face_state(UR, SR, fR);
             face_flux<FLUX>(false, cfg.roe_bdiss, fL, fR, nx, ny, dl, F);
+            if constexpr (VISC) {
+                double gL[8], gR[8];
+                if (glds && locL) {
+                    for (int q = 0; q < 8; q++) gL[q] = sG[q * T + lL];
+                } else {
+                    load_grad(grad, f.L, gL);
+                }
+                if (glds && locR) {
+                    for (int q = 0; q < 8; q++) gR[q] = sG[q * T + lR];
+                } else {
+                    load_grad(grad, f.R, gR);
+                }
+                double Fv[4];
+                viscous_flux(cfg, SL.W, SR.W, gL, gR, nx, ny, dl, f.dx, f.dy, Fv);
+                for (int q = 0; q < 4; q++) F[q] = F[q] - Fv[q];
+            }
             for (int q = 0; q < 4; q++) sF[q * FT + j] = F[q];
             return;
         }

@@ -47,6 +47,10 @@ hipError_t launch_flux_visc(const DevMesh &m, const DevCfg &c, int flux, int lim
 hipError_t launch_flux_weno(const DevMesh &m, const DevCfg &c, int flux, int lim, int mode,
                             const double4 *Us, const double *grad, const StageArgs &st, int block,
                             size_t sm, hipStream_t s);
+// WENO5 with viscous terms (fv_weno_visc.hip).
+hipError_t launch_flux_weno_visc(const DevMesh &m, const DevCfg &c, int flux, int lim, int mode,
+                                 const double4 *Us, const double *grad, const StageArgs &st, int block,
+                                 size_t sm, hipStream_t s);
 hipError_t launch_fill(double *p, double v, int n, hipStream_t s);
 hipError_t launch_err_finalize(const double *partial, int n, double *err4, hipStream_t s);
 hipError_t launch_ghosts(int n, const int *gcell, const double2 *gnrm, const int *gkind,

@@ -579,7 +579,9 @@ hipError_t launch_flux(const DevMesh &m, const DevCfg &c, int flux, int order, i
                        const double4 *Us, const double *grad, const StageArgs &st, int block,
                        hipStream_t s) {
     const size_t sm = flux_smem_bytes(m, order, c.grad_lds);
-    if (order == 3) return launch_flux_weno(m, c, flux, lim, mode, Us, grad, st, block, sm, s);
+    if (order == 3)
+        return c.viscous ? launch_flux_weno_visc(m, c, flux, lim, mode, Us, grad, st, block, sm, s)
+                         : launch_flux_weno(m, c, flux, lim, mode, Us, grad, st, block, sm, s);
     if (order == 2 && c.viscous) return launch_flux_visc(m, c, flux, lim, mode, Us, grad, st, block, sm, s);
     if (order == 1) return launch_flux_family<1, false>(m, c, flux, lim, mode, Us, grad, st, block, sm, s);
     return launch_flux_family<2, false>(m, c, flux, lim, mode, Us, grad, st, block, sm, s);

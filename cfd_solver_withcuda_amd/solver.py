@@ -34,7 +34,7 @@ def make_config(flux="roe", order=1, limiter="vanleer", gradient="gg", roe_bdiss
                 xcd_remap=1, grad_lds=1, freeze_tol=0.0, freeze_stall=0, zeta=0.5, zeta1=0.5,
                 viscous="off", mu=0.0, prandtl=0.72, r_gas=287.0) -> Config:
     """Scheme settings (cfd_config_t). viscous: "off" (Euler), "constant" (viscosity mu) or
-    "sutherland" (Sutherland's law in T = p / (rho r_gas)); laminar viscous terms need order 2."""
+    "sutherland" (Sutherland's law in T = p / (rho r_gas)); laminar viscous terms need order 2 or 3."""
     c = Config()
     c.flux = FLUX[flux]
     c.order = int(order)

@@ -66,7 +66,7 @@ typedef struct {
      * Evaluate_Cell_Net_Flux_2O (Roe dissipation on MinMod-limited differences,
      * src/Roe_Scheme.cpp:257-447, src/Limiters.cpp:97-200) with Limiter_Zeta / Limiter_Zeta1. */
     double zeta, zeta1;
-    /* Laminar viscous terms (order 2, native mode): 0 off (Euler), 1 constant viscosity mu,
+    /* Laminar viscous terms (order 2 or 3, native mode): 0 off (Euler), 1 constant viscosity mu,
      * 2 Sutherland's law in T = p / (rho r_gas). Stress tensor and heat flux as the reference's
      * Viscous_Flux_on_Face (src/Grid_Refine_Functions.cpp:418-479) with dimensional mu,
      * conductivity k = mu gamma r_gas / ((gamma - 1) prandtl); the local time step adds the

@@ -1163,6 +1163,13 @@ void on_residual(const OMesh *m, const ONatCfg *cfg, const double *U, const doub
             weno_face_state(UL, &SL, &fL);
             weno_face_state(UR, &SR, &fR);
             face_flux(cfg, 0, &fL, &fR, nx, ny, dl, &Ff[4 * fi]);
+            if (cfg->viscous) {
+                double Fv[4];
+                double dx = m->ctr[3 * Rc] - m->ctr[3 * L];
+                double dy = m->ctr[3 * Rc + 1] - m->ctr[3 * L + 1];
+                viscous_flux(cfg, SL.W, SR.W, &grad[8 * L], &grad[8 * Rc], nx, ny, dl, dx, dy, Fv);
+                for (int k = 0; k < 4; k++) Ff[4 * fi + k] = Ff[4 * fi + k] - Fv[k];
+            }
             continue;
         }
         if (cfg->order >= 2) {
@@ -1179,7 +1186,7 @@ void on_residual(const OMesh *m, const ONatCfg *cfg, const double *U, const doub
                 reconstruct_side(cfg, &SR, &SL, &grad[8 * Rc], dx, dy, -1.0, phiL ? phiL + 4 : NULL, &fR);
             }
             face_flux(cfg, boundary, &fL, &fR, nx, ny, dl, &Ff[4 * fi]);
-            if (cfg->viscous && cfg->order == 2) {
+            if (cfg->viscous) {
                 /* cell-centre states (ghost on a boundary face, with the cell's own gradient) */
                 double Fv[4];
                 const double *gR = boundary ? &grad[8 * L] : &grad[8 * Rc];
@@ -1206,7 +1213,7 @@ void on_residual(const OMesh *m, const ONatCfg *cfg, const double *U, const doub
             denom += (fabs(vn) + a0) * m->len[4 * c + f];
             sdl2 += m->len[4 * c + f] * m->len[4 * c + f];
         }
-        if (cfg->order == 2 && cfg->viscous) denom += viscous_dt_term(cfg, S[c].W[0], S[c].W[3], sdl2, m->area[c]);
+        if (cfg->order >= 2 && cfg->viscous) denom += viscous_dt_term(cfg, S[c].W[0], S[c].W[3], sdl2, m->area[c]);
         dt[c] = (denom <= 0.0 || !isfinite(denom)) ? 0.0 : cfg->cfl * m->area[c] / denom;
     }
     free(Ff);

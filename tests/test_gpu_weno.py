@@ -136,7 +136,30 @@ def test_weno_reference_iteration_vs_oracle(orc, rk):
     assert np.array_equal(s.get_state(), Uo), rel(s.get_state(), Uo)
 
 
-@pytest.mark.parametrize("bad", [dict(viscous="constant", mu=1e-3), dict(freeze_tol=1e-3), dict(freeze_stall=50)])
+@pytest.mark.parametrize("mesh", ["quads", "bump_mixed"])
+def test_weno_with_viscous_terms_vs_oracle(orc, mesh):
+    """WENO5 faces with laminar viscous terms and no-slip walls: residual, time step and gradients
+    bit for bit, then 4 LTS RK3 steps bit for bit."""
+    print(f"case weno viscous mesh={mesh}", flush=True)
+    m = MESHES[mesh]()
+    om = orc.mesh(m.points, m.cells)
+    U0 = smooth_state(om.arrays().ctr, om.nphys)
+    kw = dict(flux="hllc", order=3, limiter="vanleer", gradient="wlsq", roe_bdiss=1, cfl=0.6, fs=FS,
+              bc=("freestream", "exit", "symmetry", "noslip_wall", "noslip_wall"), viscous="constant", mu=2e-3)
+    Ro, dto, go = om.residual(make_cfg(**kw), U0)
+    s = Solver(Mesh(m.points, m.cells, tile=64), make_config(rk=3, lts=1, **kw))
+    s.set_state(U0)
+    R, dt, g = s.residual()
+    assert np.array_equal(g, go) and np.array_equal(dt, dto)
+    assert np.array_equal(R, Ro), rel(R, Ro)
+    Uo = np.ascontiguousarray(U0.copy())
+    for _ in range(4):
+        om.step(make_cfg(**kw), Uo, rk=3, lts=1)
+    s.step(4)
+    assert np.array_equal(s.get_state(), Uo), rel(s.get_state(), Uo)
+
+
+@pytest.mark.parametrize("bad", [dict(freeze_tol=1e-3), dict(freeze_stall=50)])
 def test_weno_unsupported_combinations_rejected(bad):
     m = meshgen.shock_tube_mesh(8, 4)
     kw = dict(flux="hllc", order=3, limiter="vanleer", gradient="wlsq", cfl=0.5, fs=FS, bc=BCS)

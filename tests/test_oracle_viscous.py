@@ -186,3 +186,32 @@ def test_shear_wave_decays_at_the_viscous_rate(orc):
     amp = float(np.sum(U[:, 1] / U[:, 0] * shape) / np.sum(shape * shape))
     want = Uw * np.exp(-(mu / rho) * np.pi ** 2 * t)
     assert abs(amp / want - 1) < 2e-3, (amp, want, t)
+
+
+def test_viscous_terms_on_weno_faces(orc):
+    """WENO5 (order 3) with viscous terms: mu = 0 is the inviscid WENO residual bit for bit, and the
+    shear wave decays at the viscous rate with WENO faces in the interior of the line mesh."""
+    m = meshgen.shock_tube_mesh(16, 8, length=1.0, height=0.5, seed=3, jitter=0.1)
+    om = orc.mesh(m.points, m.cells)
+    x, y = om.arrays().ctr[: om.nphys, 0], om.arrays().ctr[: om.nphys, 1]
+    U = cons_from_prim(1 + 0.1 * np.sin(3 * x), 0.3 + 0.1 * y, 0.05 * np.cos(2 * x), 1 + 0.05 * x * y)
+    cfg = dict(flux="hllc", order=3, limiter="vanleer", gradient="wlsq", bc=BCS)
+    R1, dt1, _ = om.residual(make_cfg(**cfg, viscous="constant", mu=0.0), U)
+    R0, dt0, _ = om.residual(make_cfg(**cfg), U)
+    assert np.array_equal(R1, R0) and np.array_equal(dt1, dt0)
+
+    nx, ny = 2, 32
+    m = meshgen.shock_tube_mesh(nx, ny, length=nx / ny, height=1.0)
+    om = orc.mesh(m.points, m.cells)
+    y = om.arrays().ctr[: om.nphys, 1]
+    rho, p, mu, Uw = 1.0, 1.0 / GAMMA, 0.01, 0.05
+    shape = np.cos(np.pi * y)
+    U = np.ascontiguousarray(cons_from_prim(np.full_like(y, rho), Uw * shape, np.zeros_like(y), np.full_like(y, p)))
+    c = make_cfg(flux="hllc", order=3, limiter="vanleer", gradient="wlsq", cfl=0.8, bc=BCS, viscous="constant", mu=mu)
+    t = 0.0
+    while t < 0.5:
+        _, dt = om.step(c, U, rk=3, lts=0)
+        t += dt
+    amp = float(np.sum(U[:, 1] / U[:, 0] * shape) / np.sum(shape * shape))
+    assert abs(amp / (Uw * np.exp(-(mu / rho) * np.pi ** 2 * t)) - 1) < 2e-3
+
