@@ -5,6 +5,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -94,6 +95,7 @@ struct cfd_mesh_s {
     // MUSCL fallback faces read), as a tile list for the gradient pass of the time steps
     int *d_gtiles = nullptr, *d_gcount = nullptr;
     int ngtiles = 0;
+    std::mutex lazy;  // guards the arrays built on first use (solvers may share the mesh)
     DevMesh dev{};
     ~cfd_mesh_s() {
         void *ps[] = {d_tile_cstart, d_tile_fstart, d_fgeo, d_fw, d_fnx, d_fny, d_fdl, d_fLR, d_slots, d_area, d_inv_area, d_gcell, d_gkind, d_gnrm,
@@ -319,7 +321,8 @@ static int check_config(const cfd_mesh_s *m, const cfd_config_t *c) {
 
 // WENO line stencils of the mesh's face entries on the device (first order-3 configuration).
 static int ensure_line_stencils(cfd_mesh_s *m) {
-    if (m->d_fst) return 0;
+    std::lock_guard<std::mutex> lock(m->lazy);
+    if (m->d_fst) return 0;  // the last array uploaded
     const Layout &L = m->lay;
     std::vector<int> st;
     line_stencils(m->ref, L, st);
@@ -336,8 +339,8 @@ static int ensure_line_stencils(cfd_mesh_s *m) {
     for (int t = 0; t < L.ntiles; t++)
         if (need[t]) gt.push_back(t);
     const int count = (int)gt.size();
-    HIP_TRY(upload(&m->d_gtiles, gt.empty() ? std::vector<int>{0} : gt));
-    HIP_TRY(upload(&m->d_gcount, std::vector<int>{count}));
+    if (!m->d_gtiles) HIP_TRY(upload(&m->d_gtiles, gt.empty() ? std::vector<int>{0} : gt));
+    if (!m->d_gcount) HIP_TRY(upload(&m->d_gcount, std::vector<int>{count}));
     m->ngtiles = count;
     HIP_TRY(upload(&m->d_fst, st));
     m->dev.fst = reinterpret_cast<const int4 *>(m->d_fst);
@@ -451,7 +454,8 @@ static StageArgs resid_args(cfd_solver_s *s, bool want_min, const double4 *uexit
 }
 
 static int ensure_ref_cells(cfd_mesh_s *m) {
-    if (m->d_rc_nf) return 0;
+    std::lock_guard<std::mutex> lock(m->lazy);
+    if (m->d_rc_area) return 0;  // the last array uploaded
     const RefMesh &R = m->ref;
     const Layout &L = m->lay;
     const int nc = R.nc;
@@ -470,12 +474,12 @@ static int ensure_ref_cells(cfd_mesh_s *m) {
         }
         area[k] = R.area[c];
     }
-    HIP_TRY(upload(&m->d_rc_nf, nf));
-    HIP_TRY(upload(&m->d_rc_nbr, nbr));
-    HIP_TRY(upload(&m->d_rc_dist, dist));
-    HIP_TRY(upload(&m->d_rc_nrm, nrm));
-    HIP_TRY(upload(&m->d_rc_len, len));
-    HIP_TRY(upload(&m->d_rc_area, area));
+    if (!m->d_rc_nf) HIP_TRY(upload(&m->d_rc_nf, nf));
+    if (!m->d_rc_nbr) HIP_TRY(upload(&m->d_rc_nbr, nbr));
+    if (!m->d_rc_dist) HIP_TRY(upload(&m->d_rc_dist, dist));
+    if (!m->d_rc_nrm) HIP_TRY(upload(&m->d_rc_nrm, nrm));
+    if (!m->d_rc_len) HIP_TRY(upload(&m->d_rc_len, len));
+    if (!m->d_rc_area) HIP_TRY(upload(&m->d_rc_area, area));
     return 0;
 }
 

@@ -79,3 +79,38 @@ def test_two_solvers_on_two_streams_match_sequential_runs():
     finally:
         for x in st:
             h.hipStreamDestroy(x)
+
+
+def test_solvers_sharing_a_mesh_build_its_lazy_arrays_once():
+    """Two solvers on one mesh, created from two host threads with WENO5 (whose line stencils the
+    mesh builds on first use): both march like a solver made alone."""
+    print("case shared mesh, two threads, order 3", flush=True)
+    m = meshgen.bump_mesh(60, 30, 0.06, seed=5)
+    fs = freestream(1.8)
+    kw = dict(flux="hllc", order=3, limiter="vanleer", gradient="wlsq", cfl=0.8, rk=3, lts=1, roe_bdiss=1,
+              bc=BCS, fs=fs, block=128)
+    U0 = np.tile(cons_from_prim(np.array([fs[0]]), fs[1], fs[2], fs[3]), (len(m.cells), 1))
+    ref = Solver(Mesh(m.points, m.cells, tile=128, face_budget=256), make_config(**kw))
+    ref.set_state(U0)
+    ref.step(20)
+    want = ref.get_state()
+    mesh = Mesh(m.points, m.cells, tile=128, face_budget=256)
+    out, errs = [None, None], []
+
+    def run(i):
+        try:
+            s = Solver(mesh, make_config(**kw))
+            s.set_state(U0)
+            s.step(20)
+            out[i] = s.get_state()
+        except Exception as e:  # reported below
+            errs.append(e)
+
+    th = [threading.Thread(target=run, args=(i,)) for i in range(2)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    assert not errs, errs
+    for u in out:
+        assert np.array_equal(u, want)
