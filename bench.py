@@ -112,11 +112,12 @@ def workload(name, scheme="muscl"):
                    bc=("freestream", "exit", "wall", "wall", "wall"), fs=fs)
         desc = "bump_1400x700_mixed_M1.65_hllc_muscl_vanleer_wlsq_lts_rk3"
     else:
+        # second-order Roe, minmod, Green-Gauss, RK4 at CFL 0.4
         m = meshgen.ffs_mesh(1.0 / 1560, seed=2)
         fs = freestream(3.0)
-        cfg = dict(flux="hllc", order=2, limiter="vanleer", gradient="gg", cfl=0.4, rk=4, lts=0,
+        cfg = dict(flux="roe", order=2, limiter="minmod", gradient="gg", cfl=0.4, rk=4, lts=0,
                    bc=("freestream", "exit", "wall", "wall", "wall"), fs=fs)
-        desc = "ffs_4680x1560_mixed_M3_hllc_muscl_vanleer_gg_rk4"
+        desc = "ffs_4680x1560_mixed_M3_roe_muscl_minmod_gg_rk4"
     return m, cfg, desc
 
 
@@ -420,9 +421,17 @@ def main():
     # The timed steps run on a developed flow, not on the uniform start: on a uniform state every
     # limiter sees zero differences and skips its division, and every face takes the same flux
     # branch, which would flatter the kernels.
-    develop = args.develop if args.develop >= 0 else {"bump": 2000, "ffs": 300}[args.workload]
+    develop = args.develop if args.develop >= 0 else {"bump": 2000, "ffs": 1500}[args.workload]
     t_dev = time.perf_counter()
-    if develop:
+    if develop and args.workload == "ffs":
+        # From the impulsive start the second-order scheme of this workload drives the pressure at
+        # the step corner negative within about 20 steps; the flow is developed at first order and
+        # the second-order scheme is timed on it.
+        s.configure(make_config(roe_bdiss=1, block=args.block, xcd_remap=1, grad_lds=args.grad_lds,
+                                **dict(cfg, order=1)))
+        s.step(develop, stream=stream)
+        s.configure(make_config(roe_bdiss=1, block=args.block, xcd_remap=1, grad_lds=args.grad_lds, **cfg))
+    elif develop:
         s.step(develop, stream=stream)
     U_dev = s.get_state()
     t_dev = time.perf_counter() - t_dev
@@ -457,7 +466,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": f"synthetic (seeded generated mesh; the state marched {develop} steps from freestream, untimed, "
+        "data": f"synthetic (seeded generated mesh; the state marched {develop} "
+                f"{'first-order ' if args.workload == 'ffs' else ''}steps from freestream, untimed, "
                 f"before the warmup)",
         "value_is": "FP64 face-flux evals/s of the bump workload (unique faces x residual evaluations per s); "
                     "the time-to-residual part is the time_to_residual object",

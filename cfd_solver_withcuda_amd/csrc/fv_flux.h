@@ -17,7 +17,7 @@
 #define CFD_FLUX_WAVES_PER_EU 0
 #endif
 #ifndef CFD_FLUX_PREFETCH_CELL
-#define CFD_FLUX_PREFETCH_CELL 1
+#define CFD_FLUX_PREFETCH_CELL 0
 #endif
 #if CFD_FLUX_WAVES_PER_EU > 0
 #define CFD_FLUX_ATTR __attribute__((amdgpu_waves_per_eu(CFD_FLUX_WAVES_PER_EU)))
