@@ -53,8 +53,10 @@ def measured_traffic(key, config):
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=40)
-    p.add_argument("--warmup", type=int, default=5)
+    p.add_argument("--steps", type=int, default=None,
+                   help="timed steps (default 40; 5 for the ffs workload, whose second-order march is "
+                        "timed on a first-order-developed flow before it turns unstable)")
+    p.add_argument("--warmup", type=int, default=None, help="untimed warm-up steps (default 5; 1 for ffs)")
     p.add_argument("--workload", default="bump", choices=["bump", "ffs", "batch", "sod", "steady"])
     p.add_argument("--thickness", type=float, default=0.04, help="steady workload: bump thickness")
     p.add_argument("--cfl", type=float, default=None,
@@ -91,7 +93,12 @@ def parse():
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run a 512 MiB device copy (read 512 MiB + write 512 MiB) five times, "
                         "to scale the FETCH_SIZE / WRITE_SIZE counters")
-    return p.parse_args()
+    a = p.parse_args()
+    if a.steps is None:
+        a.steps = 5 if a.workload == "ffs" else 40
+    if a.warmup is None:
+        a.warmup = 1 if a.workload == "ffs" else 5
+    return a
 
 
 def workload(name, scheme="muscl"):
