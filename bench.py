@@ -57,7 +57,7 @@ def parse():
                    help="timed steps (default 40; 5 for the ffs workload, whose second-order march is "
                         "timed on a first-order-developed flow before it turns unstable)")
     p.add_argument("--warmup", type=int, default=None, help="untimed warm-up steps (default 5; 1 for ffs)")
-    p.add_argument("--workload", default="bump", choices=["bump", "ffs", "batch", "sod", "steady"])
+    p.add_argument("--workload", default="bump", choices=["bump", "ffs", "ffs_hllc", "batch", "sod", "steady"])
     p.add_argument("--thickness", type=float, default=0.04, help="steady workload: bump thickness")
     p.add_argument("--cfl", type=float, default=None,
                    help="steady and batch workloads: Courant number (default 1.2)")
@@ -95,9 +95,9 @@ def parse():
                         "to scale the FETCH_SIZE / WRITE_SIZE counters")
     a = p.parse_args()
     if a.steps is None:
-        a.steps = 5 if a.workload == "ffs" else 40
+        a.steps = 5 if a.workload.startswith("ffs") else 40
     if a.warmup is None:
-        a.warmup = 1 if a.workload == "ffs" else 5
+        a.warmup = 1 if a.workload.startswith("ffs") else 5
     return a
 
 
@@ -118,6 +118,14 @@ def workload(name, scheme="muscl"):
         cfg = dict(flux="hllc", order=2, limiter="vanleer", gradient="wlsq", cfl=0.8, rk=3, lts=1,
                    bc=("freestream", "exit", "wall", "wall", "wall"), fs=fs)
         desc = "bump_1400x700_mixed_M1.65_hllc_muscl_vanleer_wlsq_lts_rk3"
+    elif name == "ffs_hllc":
+        # the roofline run on the step mesh: HLLC + MUSCL/Van Leer (WLSQ; --gradient gg for
+        # Green-Gauss), with the step workload's RK4 at CFL 0.4
+        m = meshgen.ffs_mesh(1.0 / 1560, seed=2)
+        fs = freestream(3.0)
+        cfg = dict(flux="hllc", order=2, limiter="vanleer", gradient="wlsq", cfl=0.4, rk=4, lts=0,
+                   bc=("freestream", "exit", "wall", "wall", "wall"), fs=fs)
+        desc = "ffs_4680x1560_mixed_M3_hllc_muscl_vanleer_wlsq_rk4"
     else:
         # second-order Roe, minmod, Green-Gauss, RK4 at CFL 0.4
         m = meshgen.ffs_mesh(1.0 / 1560, seed=2)
@@ -428,9 +436,9 @@ def main():
     # The timed steps run on a developed flow, not on the uniform start: on a uniform state every
     # limiter sees zero differences and skips its division, and every face takes the same flux
     # branch, which would flatter the kernels.
-    develop = args.develop if args.develop >= 0 else {"bump": 2000, "ffs": 1500}[args.workload]
+    develop = args.develop if args.develop >= 0 else {"bump": 2000, "ffs": 1500, "ffs_hllc": 1500}[args.workload]
     t_dev = time.perf_counter()
-    if develop and args.workload == "ffs":
+    if develop and args.workload.startswith("ffs"):
         # From the impulsive start the second-order scheme of this workload drives the pressure at
         # the step corner negative within about 20 steps; the flow is developed at first order and
         # the second-order scheme is timed on it.
@@ -474,7 +482,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": f"synthetic (seeded generated mesh; the state marched {develop} "
-                f"{'first-order ' if args.workload == 'ffs' else ''}steps from freestream, untimed, "
+                f"{'first-order ' if args.workload.startswith('ffs') else ''}steps from freestream, untimed, "
                 f"before the warmup)",
         "value_is": "FP64 face-flux evals/s of the bump workload (unique faces x residual evaluations per s); "
                     "the time-to-residual part is the time_to_residual object",

@@ -1,8 +1,9 @@
 """Builds libcfdhip.so in-tree with hipcc for gfx950.
 
-    python -m cfd_solver_withcuda_amd.build [--out PATH] [-D NAME=VALUE ...]
+    python -m cfd_solver_withcuda_amd.build [--out PATH] [-D NAME=VALUE ...] [--flags "FLAGS"]
 
 -D sets the kernels' compile-time tuning macros (e.g. CFD_FLUX_WAVES_PER_EU, CFD_FLUX_PREFETCH_CELL);
+--flags appends compiler options to the HIP sources' command lines (they override the defaults);
 --out writes a separately tuned library elsewhere (load it with CFD_HIP_LIB=PATH). HIP sources are compiled with floating-point contraction off, so that every FP64 kernel
 reproduces the CPU restatement's rounding, and with packed FP32 ops disabled.
 """
@@ -22,7 +23,7 @@ ARCH = os.environ.get("CFD_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["fv_kernels.hip", "fv_visc.hip", "fv_weno.hip", "fv_weno_visc.hip", "ref_kernels.hip",
                "mlp_kernels.hip"]
-CPP_SOURCES = ["mesh_host.cpp", "capi.cpp", "capi_ref.cpp", "capi_batch.cpp", "capi_mlp.cpp"]
+CPP_SOURCES = ["mesh_host.cpp", "capi.cpp", "capi_batch.cpp", "capi_mlp.cpp"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-function",
           f"-I{os.path.join(ROOT, 'include')}"]
@@ -36,12 +37,12 @@ def _hipcc() -> str:
     return "hipcc"
 
 
-def _compile(src: str, build_dir: str = BUILD, defines: tuple = ()) -> str:
+def _compile(src: str, build_dir: str = BUILD, defines: tuple = (), flags: tuple = ()) -> str:
     path = os.path.join(CSRC, src)
     obj = os.path.join(build_dir, os.path.splitext(src)[0] + ".o")
     cmd = [_hipcc()] + COMMON + [f"-D{d}" for d in defines]
     if src.endswith(".hip"):
-        cmd += HIP_FLAGS
+        cmd += HIP_FLAGS + list(flags)
     cmd += ["-c", path, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -53,12 +54,12 @@ def sources() -> list[str]:
     return [s for s in HIP_SOURCES + CPP_SOURCES if os.path.exists(os.path.join(CSRC, s))]
 
 
-def build(verbose: bool = True, out: str = LIB, defines: tuple = ()) -> str:
+def build(verbose: bool = True, out: str = LIB, defines: tuple = (), flags: tuple = ()) -> str:
     build_dir = BUILD if out == LIB else os.path.join(os.path.dirname(os.path.abspath(out)), "_build")
     os.makedirs(build_dir, exist_ok=True)
     srcs = sources()
     with ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
-        objs = list(ex.map(lambda src: _compile(src, build_dir, tuple(defines)), srcs))
+        objs = list(ex.map(lambda src: _compile(src, build_dir, tuple(defines), tuple(flags)), srcs))
     LIB_OUT = os.path.abspath(out)
     tmp = LIB_OUT + ".tmp"
     cmd = [_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
@@ -67,7 +68,8 @@ def build(verbose: bool = True, out: str = LIB, defines: tuple = ()) -> str:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB_OUT)
     if verbose:
-        print(f"built {LIB_OUT} from {', '.join(srcs)}" + (f" with {' '.join(defines)}" if defines else ""))
+        print(f"built {LIB_OUT} from {', '.join(srcs)}" + (f" with {' '.join(defines)}" if defines else "")
+              + (f" and {' '.join(flags)}" if flags else ""))
     return LIB_OUT
 
 
@@ -107,7 +109,8 @@ if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=LIB)
     ap.add_argument("-D", dest="defines", action="append", default=[])
+    ap.add_argument("--flags", default="")
     a = ap.parse_args()
-    build(out=a.out, defines=tuple(a.defines))
+    build(out=a.out, defines=tuple(a.defines), flags=tuple(a.flags.split()))
     if a.out == LIB:
         build_host()

@@ -107,6 +107,10 @@ struct cfd_mesh_s {
 
 struct cfd_solver_s {
     cfd_mesh_s *m = nullptr;
+    // the solver's own copy of the mesh's device view, taken under the mesh's lock when the
+    // solver is configured (other solvers sharing the mesh may add lazily built arrays)
+    DevMesh dev{};
+    int ngtiles = 0;
     cfd_config_t cfg{};
     DevCfg dcfg{};
     double4 *Un = nullptr, *Ua = nullptr, *Ub = nullptr, *acc = nullptr, *R = nullptr;
@@ -117,6 +121,7 @@ struct cfd_solver_s {
     double4 *Uall = nullptr; // [cells + ghosts] state for the reference's second order
     int *fz = nullptr;       // [0] freezing state, [1] steps since the best Error[0]
     double *fz_best = nullptr;
+    bool fz_explicit = false;  // cfd_set_freeze chose a non-zero mode
     int cur = 0;
     bool dt_valid = false;
     bool has_state = false;
@@ -352,14 +357,14 @@ static int ensure_line_stencils(cfd_mesh_s *m) {
 static int step_grad(cfd_solver_s *s, const double4 *in, hipStream_t st) {
     const cfd_config_t &c = s->cfg;
     if (c.order == 2) {
-        HIP_TRY(launch_grad(s->m->dev, s->dcfg, c.gradient, in, s->grad, c.block, st));
+        HIP_TRY(launch_grad(s->dev, s->dcfg, c.gradient, in, s->grad, c.block, st));
     } else if (c.order == 3 && c.viscous) {  // every face reads gradients
-        HIP_TRY(launch_grad(s->m->dev, s->dcfg, c.gradient, in, s->grad, c.block, st));
-    } else if (c.order == 3 && s->m->ngtiles > 0) {
-        DevMesh dm = s->m->dev;
+        HIP_TRY(launch_grad(s->dev, s->dcfg, c.gradient, in, s->grad, c.block, st));
+    } else if (c.order == 3 && s->ngtiles > 0) {
+        DevMesh dm = s->dev;
         dm.tile_list = s->m->d_gtiles;
         dm.tile_count = s->m->d_gcount;
-        dm.ntiles = s->m->ngtiles;
+        dm.ntiles = s->ngtiles;
         HIP_TRY(launch_grad(dm, s->dcfg, c.gradient, in, s->grad, c.block, st));
     }
     return 0;
@@ -370,6 +375,22 @@ int cfd_solver_configure(cfd_solver_t *s, const cfd_config_t *cfg) {
     int rc = check_config(s->m, cfg);
     if (rc) return rc;
     if (cfg->order == 3 && (rc = ensure_line_stencils(s->m))) return rc;
+    {
+        std::lock_guard<std::mutex> lock(s->m->lazy);
+        s->dev = s->m->dev;
+        s->ngtiles = s->m->ngtiles;
+    }
+    // recorded limiter factors belong to the scheme that recorded them: a change of scheme or of
+    // the freezing settings returns the march to the evaluated limiter
+    const cfd_config_t &o = s->cfg;
+    const bool scheme_changed = o.flux != cfg->flux || o.order != cfg->order || o.limiter != cfg->limiter ||
+                                o.gradient != cfg->gradient || o.ref_mode != cfg->ref_mode ||
+                                o.viscous != cfg->viscous || o.freeze_tol != cfg->freeze_tol ||
+                                o.freeze_stall != cfg->freeze_stall;
+    if (s->fz && scheme_changed) {
+        s->fz_explicit = false;
+        if ((rc = freeze_init(s, 0))) return rc;
+    }
     s->cfg = *cfg;
     for (int k = 0; k < 6; k++) s->dcfg.bc[k] = cfg->bc[k];
     for (int k = 0; k < 4; k++) s->dcfg.fs[k] = cfg->fs[k];
@@ -421,6 +442,8 @@ int cfd_set_state(cfd_solver_t *s, const double *U) {
     const int nc = s->m->ref.nc;
     std::vector<double> h(4 * (size_t)nc);
     for (int k = 0; k < nc; k++) std::memcpy(&h[4 * (size_t)k], &U[4 * (size_t)s->m->lay.perm[k]], 4 * sizeof(double));
+    // steps launched on any stream (also a non-blocking one) may still read the state
+    HIP_TRY(hipDeviceSynchronize());
     HIP_TRY(hipMemcpy(s->Un, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
     s->has_state = true;
     s->dt_valid = false;
@@ -511,10 +534,10 @@ static int residual_pass(cfd_solver_s *s, const double4 *in, double *dtmin, cons
     const cfd_config_t &c = s->cfg;
     const int order = order_override ? order_override : c.order;
     if (c.ref_mode && order == 2 && c.flux == FLUX_ROE) return ref_roe2o_pass(s, in, dtmin, uexit, st);
-    if (order >= 2) HIP_TRY(launch_grad(s->m->dev, s->dcfg, c.gradient, in, s->grad, c.block, st));
+    if (order >= 2) HIP_TRY(launch_grad(s->dev, s->dcfg, c.gradient, in, s->grad, c.block, st));
     StageArgs a = resid_args(s, dtmin != nullptr, uexit);
-    HIP_TRY(launch_flux(s->m->dev, s->dcfg, c.flux, order, c.limiter, MODE_RESID, in, s->grad, a, c.block, st));
-    if (dtmin) HIP_TRY(launch_min_reduce(s->dt_partial, s->m->dev.ntiles, dtmin, st));
+    HIP_TRY(launch_flux(s->dev, s->dcfg, c.flux, order, c.limiter, MODE_RESID, in, s->grad, a, c.block, st));
+    if (dtmin) HIP_TRY(launch_min_reduce(s->dt_partial, s->dev.ntiles, dtmin, st));
     return 0;
 }
 
@@ -569,12 +592,14 @@ static bool auto_freeze(const cfd_solver_s *s) {
 // Allocates the limiter-freezing buffers and sets the state to `mode` (best Error[0] reset).
 static int freeze_init(cfd_solver_s *s, int mode) {
     if (!s->psi) {
-        HIP_TRY(hipMalloc((void **)&s->psi, 8 * (size_t)s->m->dev.nfe * sizeof(float)));
+        HIP_TRY(hipMalloc((void **)&s->psi, 8 * (size_t)s->dev.nfe * sizeof(float)));
         HIP_TRY(hipMalloc((void **)&s->fz, 2 * sizeof(int)));
         HIP_TRY(hipMalloc((void **)&s->fz_best, sizeof(double)));
     }
     const int h[2] = {mode, 0};
     const double b = 1.7976931348623157e308;
+    // steps in flight on any stream may still read the freezing state
+    HIP_TRY(hipDeviceSynchronize());
     HIP_TRY(hipMemcpy(s->fz, h, sizeof(h), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(s->fz_best, &b, sizeof(b), hipMemcpyHostToDevice));
     return 0;
@@ -583,6 +608,7 @@ static int freeze_init(cfd_solver_s *s, int mode) {
 int cfd_set_freeze(cfd_solver_t *s, int mode) {
     if (!s || mode < 0 || mode > 2) return fail(CFD_E_ARG, "cfd_set_freeze: mode must be 0, 1 or 2");
     if (s->cfg.order != 2 || s->cfg.ref_mode) return fail(CFD_E_ARG, "cfd_set_freeze: limiter freezing needs order 2 in native mode");
+    s->fz_explicit = (mode != 0);
     return freeze_init(s, mode);
 }
 
@@ -596,7 +622,7 @@ static int native_step(cfd_solver_s *s, bool want_err, hipStream_t st) {
             if (rc) return rc;
         }
     }
-    const DevMesh &dm = s->m->dev;
+    const DevMesh &dm = s->dev;
     if (!c.lts && !s->dt_valid) {
         int rc = residual_pass(s, s->Un, s->scal + s->cur, nullptr, st, dt_pass_order(c));
         if (rc) return rc;
@@ -623,7 +649,8 @@ static int native_step(cfd_solver_s *s, bool want_err, hipStream_t st) {
         a.dt_global = s->scal + s->cur;
         a.err_partial = (a.last && want_err) ? s->err_partial : nullptr;
         a.dt_partial = (a.last && !c.lts) ? s->dt_partial : nullptr;
-        a.freeze = s->fz;
+        // recorded factors are used only while freezing is automatic or was set explicitly
+        a.freeze = (auto_freeze(s) || s->fz_explicit) ? s->fz : nullptr;
         a.psi = s->psi;
         HIP_TRY(launch_flux(dm, s->dcfg, c.flux, c.order, c.limiter, MODE_STAGE, in, s->grad, a, c.block, st));
     }
@@ -688,7 +715,7 @@ int cfd_time_pass(cfd_solver_t *s, int which, int reps, float *ms, void *stream)
     if (s->cfg.ref_mode) return fail(CFD_E_ARG, "cfd_time_pass: native mode only");
     hipStream_t st = (hipStream_t)stream;
     const cfd_config_t &c = s->cfg;
-    const DevMesh &dm = s->m->dev;
+    const DevMesh &dm = s->dev;
     if (!c.lts && !s->dt_valid) {
         int rc = residual_pass(s, s->Un, s->scal + s->cur, nullptr, st, dt_pass_order(c));
         if (rc) return rc;

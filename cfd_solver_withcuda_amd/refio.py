@@ -115,30 +115,50 @@ _DISSIPATION = {1: "llf", 3: "roe", 101: "hllc", 102: "ausm"}
 
 def config_from_json(path: str) -> dict:
     """make_config() keyword arguments from the reference's solver JSON (comments allowed, as
-    the reference's jsoncpp reader allows them). Keys read (src/Configuration_Read.cpp:28-51):
-    Simulation.CFL, Solver.Dissipation_Type, Solver.Is_Second_Order, Solver.Local_Time_Stepping,
-    Solver.Is_WENO (order 3; minmod / Green-Gauss on the faces without a quadrilateral line) and
-    Solver.Is_Viscous (laminar terms with Sutherland's law, Pr 0.72, R 287; the caller gives the
-    walls ghost type "noslip_wall", the reference's Is_Viscous_Wall)."""
+    the reference's jsoncpp reader allows them), with the reference's meaning of each key
+    (src/Configuration_Read.cpp:23-87, src/Solver.cpp:93-118,190-205):
+
+    - Simulation.CFL; Solver.Local_Time_Stepping;
+    - Solver.Dissipation_Type: 1 LLF, 3 ROE (101 HLLC and 102 AUSM are this library's values);
+    - Solver.Time_Accurate: Runge_Kutta_Method (the reference's TVD-RK3 in delta form, rk 3) when
+      true, else Explicit_Method (forward Euler, rk 1), both as the reference's iteration
+      (ref_mode 1);
+    - Solver.Is_Second_Order: Evaluate_Cell_Net_Flux_2O; with ROE that is ROE_2O (Roe dissipation
+      on MinMod-limited differences) with LimiterCoefficients.Limiter_Zeta / Limiter_Zeta1; other
+      fluxes use MUSCL with minmod and Green-Gauss gradients;
+    - Solver.Is_WENO: WENO5 faces (order 3) with the Rusanov (LLF) flux whatever Dissipation_Type,
+      as the reference's Calculate_Face_WENO_Flux (src/WENO2D.cpp:458-555);
+    - Solver.Is_Viscous: laminar terms with Sutherland's law (Pr 0.72, R 287) at order 2 or 3; they
+      exist in the native iteration only (ref_mode 0, TVD-RK3); the caller gives the walls ghost
+      type "noslip_wall", the reference's Is_Viscous_Wall.
+
+    A key the file does not hold reads as false or 0, as jsoncpp's asBool() / asDouble() return
+    for a missing member; a CFL that is missing or not positive is rejected."""
     text = open(path).read()
     text = re.sub(r"//[^\n]*", "", text)
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     d = json.loads(text)
-    sim, sol = d.get("Simulation", {}), d.get("Solver", {})
-    diss = int(sol.get("Dissipation_Type", 3))
+    sim, sol, lim = d.get("Simulation", {}), d.get("Solver", {}), d.get("LimiterCoefficients", {})
+    diss = int(sol.get("Dissipation_Type", 0))
     if diss not in _DISSIPATION:
         raise ValueError(f"Dissipation_Type {diss} is not on this path (1 LLF or 3 ROE)")
-    kw = dict(flux=_DISSIPATION[diss], order=2 if sol.get("Is_Second_Order", False) else 1,
-              cfl=float(sim.get("CFL", 0.5)), lts=1 if sol.get("Local_Time_Stepping", False) else 0,
-              rk=3)
-    if sol.get("Is_WENO", False):
-        kw["order"] = 3
-    if kw["order"] >= 2:
+    cfl = float(sim.get("CFL", 0.0))
+    if not cfl > 0.0:
+        raise ValueError("Simulation.CFL must be positive")
+    order = 3 if sol.get("Is_WENO", False) else (2 if sol.get("Is_Second_Order", False) else 1)
+    kw = dict(flux="llf" if order == 3 else _DISSIPATION[diss], order=order, cfl=cfl,
+              lts=1 if sol.get("Local_Time_Stepping", False) else 0,
+              rk=3 if sol.get("Time_Accurate", False) else 1, ref_mode=1)
+    if order >= 2:
         kw.update(limiter="minmod", gradient="gg")
+    if order == 2 and kw["flux"] == "roe":
+        kw.update(zeta=float(lim.get("Limiter_Zeta", 0.0)), zeta1=float(lim.get("Limiter_Zeta1", 0.0)))
     if sol.get("Is_Viscous", False):
-        if kw["order"] != 2:
-            raise ValueError("Is_Viscous needs Is_Second_Order without Is_WENO on this path")
-        kw.update(viscous="sutherland", prandtl=0.72, r_gas=287.0)
+        if order < 2:
+            raise ValueError("Is_Viscous needs Is_Second_Order or Is_WENO (cell gradients)")
+        kw.update(viscous="sutherland", prandtl=0.72, r_gas=287.0, ref_mode=0, rk=3)
+        kw.pop("zeta", None)
+        kw.pop("zeta1", None)
     return kw
 
 

@@ -196,6 +196,52 @@ def test_sod_reference_iteration_vs_golden():
     assert abs(dt.sum() - g["time_hist"][-1]) <= 1e-12 * g["time_hist"][-1]
 
 
+SOD_JSON = """{
+    "TestCase": {"Test_Case": 0, "Test_Case_Name": "Sod_Shock_Tube"},
+    "Simulation": {"Initialize_Type": 0, "Is_Implicit_Method": false, "Total_Iterations": 25,
+                   "CFL": 0.5, "Terminating_Time": 0.2, "Is_Time_Dependent": true},
+    "Solver": {"Solver_Type": 0, "Solver_Name": "Euler", "Is_Conservative": true, "Is_Viscous": false,
+               "Limiter_Case": 0, "Area_Weighted_Average": 1, "Flux_Type": 2, "NUM_FLUX_COMPONENTS": 4,
+               "Is_Second_Order": false, "Time_Accurate": true, "Local_Time_Stepping": false,
+               "Non_Dimensional_Form": false, "Is_WENO": false, "Dissipation_Type": 3,
+               "Is_MOVERS_1": false, "Enable_Entropy_Fix": true},
+    "LimiterCoefficients": {"Limiter_Zeta": 0.5, "Limiter_Zeta1": 0.5
+                            // "Epsilon": 1.0e-6
+                           }
+}"""
+
+
+def test_reference_json_drives_sod_golden(tmp_path):
+    """A solver file in the reference's schema (json_Files/Solver_Config.json) selects the
+    reference's iteration (Time_Accurate: TVD-RK3 in delta form; Dissipation_Type 3: Roe), which
+    reproduces the reference's own 25-step Sod histories."""
+    from cfd_solver_withcuda_amd import refio
+
+    p = tmp_path / "Solver_Config.json"
+    p.write_text(SOD_JSON)
+    kw = refio.config_from_json(str(p))
+    assert kw["ref_mode"] == 1 and kw["rk"] == 3 and kw["flux"] == "roe" and kw["order"] == 1
+    m, g = _sod_setup()
+    s = Solver(Mesh(m.points, m.cells, tile=128), make_config(fs=(1.0, 0.0, 0.0, 1.0), **kw))
+    s.set_state(g["U0"])
+    err, dt = s.step(25, history=True)
+    assert np.allclose(err, g["err_hist"], rtol=1e-10, atol=1e-13)
+    assert np.allclose(dt, g["dt_hist"], rtol=1e-13, atol=0)
+    assert rel(s.get_state(), g["U_step25"]) <= 1e-12
+    # Time_Accurate false is the reference's Explicit_Method: forward Euler with the global dt
+    p.write_text(SOD_JSON.replace('"Time_Accurate": true', '"Time_Accurate": false'))
+    kw = refio.config_from_json(str(p))
+    assert kw["rk"] == 1
+    s.configure(make_config(fs=(1.0, 0.0, 0.0, 1.0), **kw))
+    s.set_state(g["U0"])
+    s.step(3)
+    om = Oracle().mesh(m.points, m.cells)
+    Uo = np.ascontiguousarray(g["U0"].copy())
+    for _ in range(3):
+        om.step(make_cfg(flux="roe", order=1, cfl=0.5, fs=(1.0, 0, 0, 1.0)), Uo, rk=1, lts=0, ref=True)
+    assert np.array_equal(s.get_state(), Uo)
+
+
 def test_sod_reference_iteration_vs_oracle_bit_exact(orc):
     m, g = _sod_setup()
     om = orc.mesh(m.points, m.cells)
@@ -308,4 +354,48 @@ def test_steps_follow_reconfiguration_and_freezing(orc, bump):
     ocfg.freeze = 2
     for _ in range(2):
         om.step(ocfg, Uo, rk=3, lts=1)
+    assert np.array_equal(s.get_state(), Uo), rel(s.get_state(), Uo)
+
+
+def test_reconfiguration_after_freezing_evaluates_the_limiter(orc, bump):
+    """Recorded limiter factors belong to the scheme that recorded them: after freezing (mode 2)
+    a change of limiter makes the next steps evaluate the new limiter, as the oracle does without
+    freezing; the explicit mode survives a reconfiguration that keeps the scheme."""
+    print("case reconfiguration after freezing", flush=True)
+    om = orc.mesh(bump.points, bump.cells)
+    U0 = smooth_state(om.arrays().ctr, om.nphys)
+    base = dict(order=2, gradient="wlsq", bc=BCS_NATIVE, fs=FS, roe_bdiss=1, flux="hllc", cfl=0.6, rk=3, lts=1)
+    s = Solver(Mesh(bump.points, bump.cells, tile=64), make_config(limiter="vanleer", **base))
+    s.set_state(U0)
+    s.set_freeze(1)
+    s.step(1)
+    s.set_freeze(2)
+    s.step(1)
+    U1 = s.get_state()
+    s.configure(make_config(limiter="minmod", **base))
+    s.step(2)
+    Uo = np.ascontiguousarray(U1.copy())
+    ocfg = make_cfg(flux="hllc", order=2, limiter="minmod", gradient="wlsq", roe_bdiss=1, cfl=0.6,
+                    bc=BCS_NATIVE, fs=FS)
+    for _ in range(2):
+        om.step(ocfg, Uo, rk=3, lts=1)
+    assert np.array_equal(s.get_state(), Uo), rel(s.get_state(), Uo)
+    # same scheme, new Courant number: the recorded factors stay in use
+    s.configure(make_config(limiter="minmod", **base))
+    s.set_state(U0)
+    s.set_freeze(1)
+    s.step(1)
+    s.set_freeze(2)
+    s.configure(make_config(limiter="minmod", **dict(base, cfl=0.5)))
+    s.step(1)
+    phi = np.zeros(8 * om.nfaces, np.float32)
+    Uo = np.ascontiguousarray(U0.copy())
+    ocfg = make_cfg(flux="hllc", order=2, limiter="minmod", gradient="wlsq", roe_bdiss=1, cfl=0.6,
+                    bc=BCS_NATIVE, fs=FS)
+    ocfg.phi = phi.ctypes.data
+    ocfg.freeze = 1
+    om.step(ocfg, Uo, rk=3, lts=1)
+    ocfg.freeze = 2
+    ocfg.cfl = 0.5
+    om.step(ocfg, Uo, rk=3, lts=1)
     assert np.array_equal(s.get_state(), Uo), rel(s.get_state(), Uo)

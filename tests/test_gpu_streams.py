@@ -18,6 +18,7 @@ BCS = ("freestream", "exit", "wall", "wall", "wall")
 def hip():
     h = ctypes.CDLL("libamdhip64.so.7")
     h.hipStreamCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+    h.hipStreamCreateWithFlags.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_uint]
     h.hipStreamDestroy.argtypes = [ctypes.c_void_p]
     h.hipStreamSynchronize.argtypes = [ctypes.c_void_p]
     return h
@@ -114,3 +115,29 @@ def test_solvers_sharing_a_mesh_build_its_lazy_arrays_once():
     assert not errs, errs
     for u in out:
         assert np.array_equal(u, want)
+
+
+def test_set_state_after_async_steps_on_a_non_blocking_stream():
+    """cfd_set_state after steps issued (without waiting) on a hipStreamNonBlocking stream: the
+    new state replaces the old one only after those steps, so a march from it equals a march from
+    the same state on a fresh solver."""
+    print("case set_state after async steps on a non-blocking stream", flush=True)
+    h = hip()
+    st = ctypes.c_void_p()
+    assert h.hipStreamCreateWithFlags(ctypes.byref(st), 1) == 0  # hipStreamNonBlocking
+    try:
+        a, b = solvers()[0], solvers()[0]
+        U0 = a.get_state()
+        Unew = U0.copy()
+        Unew[:, 0] *= 1.01
+        Unew[:, 3] *= 1.02
+        for _ in range(5):
+            a.step(20, stream=st.value)
+            a.set_state(Unew)
+        a.step(10, stream=st.value)
+        assert h.hipStreamSynchronize(st) == 0
+        b.set_state(Unew)
+        b.step(10)
+        assert np.array_equal(a.get_state(), b.get_state())
+    finally:
+        h.hipStreamDestroy(st)

@@ -56,18 +56,39 @@ def test_vtk_solution_layout(tmp_path):
 
 
 def test_config_from_reference_json(tmp_path):
+    """Keys keep the reference's meaning (src/Configuration_Read.cpp:23-87, src/Solver.cpp:93-118)."""
     p = tmp_path / "cfg.json"
     p.write_text('{ "Simulation": {"CFL": 0.4, // comment\n "Total_Iterations": 10},'
-                 ' "Solver": {"Dissipation_Type": 3, "Is_Second_Order": false, "Local_Time_Stepping": true} }')
-    kw = refio.config_from_json(str(p))
-    assert kw == dict(flux="roe", order=1, cfl=0.4, lts=1, rk=3)
-    p.write_text('{ "Simulation": {"CFL": 0.6}, "Solver": {"Dissipation_Type": 1, "Is_Second_Order": true,'
-                 ' "Is_WENO": true} }')
-    assert refio.config_from_json(str(p)) == dict(flux="llf", order=3, cfl=0.6, lts=0, rk=3,
+                 ' "Solver": {"Dissipation_Type": 3, "Is_Second_Order": false, "Local_Time_Stepping": true,'
+                 ' "Time_Accurate": true} }')
+    assert refio.config_from_json(str(p)) == dict(flux="roe", order=1, cfl=0.4, lts=1, rk=3, ref_mode=1)
+    # Time_Accurate false (or missing) is Explicit_Method: forward Euler
+    p.write_text('{ "Simulation": {"CFL": 0.4}, "Solver": {"Dissipation_Type": 3} }')
+    assert refio.config_from_json(str(p)) == dict(flux="roe", order=1, cfl=0.4, lts=0, rk=1, ref_mode=1)
+    # Is_Second_Order with ROE is ROE_2O with the limiter coefficients
+    p.write_text('{ "Simulation": {"CFL": 0.5}, "Solver": {"Dissipation_Type": 3, "Is_Second_Order": true,'
+                 ' "Time_Accurate": true}, "LimiterCoefficients": {"Limiter_Zeta": 0.5, "Limiter_Zeta1": 0.25} }')
+    assert refio.config_from_json(str(p)) == dict(flux="roe", order=2, cfl=0.5, lts=0, rk=3, ref_mode=1,
+                                                  limiter="minmod", gradient="gg", zeta=0.5, zeta1=0.25)
+    # WENO faces use the Rusanov flux whatever Dissipation_Type
+    p.write_text('{ "Simulation": {"CFL": 0.6}, "Solver": {"Dissipation_Type": 3, "Is_Second_Order": true,'
+                 ' "Is_WENO": true, "Time_Accurate": true} }')
+    assert refio.config_from_json(str(p)) == dict(flux="llf", order=3, cfl=0.6, lts=0, rk=3, ref_mode=1,
                                                   limiter="minmod", gradient="gg")
-    p.write_text('{ "Solver": {"Dissipation_Type": 101, "Is_Second_Order": true, "Is_Viscous": true} }')
-    assert refio.config_from_json(str(p)) == dict(flux="hllc", order=2, cfl=0.5, lts=0, rk=3, limiter="minmod",
-                                                  gradient="gg", viscous="sutherland", prandtl=0.72, r_gas=287.0)
-    p.write_text('{ "Solver": {"Dissipation_Type": 3, "Is_WENO": true, "Is_Viscous": true} }')
-    with pytest.raises(ValueError):
-        refio.config_from_json(str(p))
+    # viscous terms run in the native iteration, at order 2 or 3
+    p.write_text('{ "Simulation": {"CFL": 0.5}, "Solver": {"Dissipation_Type": 101, "Is_Second_Order": true,'
+                 ' "Is_Viscous": true} }')
+    assert refio.config_from_json(str(p)) == dict(flux="hllc", order=2, cfl=0.5, lts=0, rk=3, ref_mode=0,
+                                                  limiter="minmod", gradient="gg", viscous="sutherland",
+                                                  prandtl=0.72, r_gas=287.0)
+    p.write_text('{ "Simulation": {"CFL": 0.5}, "Solver": {"Dissipation_Type": 3, "Is_WENO": true,'
+                 ' "Is_Viscous": true} }')
+    assert refio.config_from_json(str(p)) == dict(flux="llf", order=3, cfl=0.5, lts=0, rk=3, ref_mode=0,
+                                                  limiter="minmod", gradient="gg", viscous="sutherland",
+                                                  prandtl=0.72, r_gas=287.0)
+    for bad in ('{ "Simulation": {"CFL": 0.5}, "Solver": {"Dissipation_Type": 3, "Is_Viscous": true} }',
+                '{ "Solver": {"Dissipation_Type": 3} }',
+                '{ "Simulation": {"CFL": 0.5}, "Solver": {"Dissipation_Type": 2} }'):
+        p.write_text(bad)
+        with pytest.raises(ValueError):
+            refio.config_from_json(str(p))
