@@ -69,6 +69,7 @@ def lib() -> ctypes.CDLL:
     _sig(L, "cfd_get_state", _i, [_vp, _dp])
     _sig(L, "cfd_state_device", _i, [_vp, ctypes.POINTER(_vp)])
     _sig(L, "cfd_residual", _i, [_vp, _dp, _dp, _dp, _vp])
+    _sig(L, "cfd_residual_noalloc", _i, [_vp, cfgp, _vp, _vp, _vp, _vp, _vp])
     _sig(L, "cfd_step", _i, [_vp, _i, _dp, _dp, _vp])
     _sig(L, "cfd_apply_bc", _i, [_vp, _dp, _vp])
     _sig(L, "cfd_set_freeze", _i, [_vp, _i])

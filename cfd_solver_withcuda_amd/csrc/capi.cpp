@@ -10,6 +10,7 @@
 #include <string>
 #include <vector>
 
+#include "capi_internal.h"
 #include "fv_kernels.h"
 #include "mesh_host.h"
 
@@ -74,36 +75,6 @@ void capi_fill_viscous(DevCfg &d, const cfd_config_t &c) {
     d.inv_pr = c.viscous ? 1.0 / c.prandtl : 0.0;
 }
 }  // namespace cfd
-
-struct cfd_mesh_s {
-    RefMesh ref;
-    Layout lay;
-    bool on_device = false;
-    int *d_tile_cstart = nullptr, *d_tile_fstart = nullptr;
-    double *d_fgeo = nullptr, *d_fw = nullptr, *d_fnx = nullptr, *d_fny = nullptr, *d_fdl = nullptr;
-    int *d_fLR = nullptr;
-    uint16_t *d_slots = nullptr;
-    double *d_area = nullptr, *d_inv_area = nullptr;
-    int *d_gcell = nullptr, *d_gkind = nullptr;  // ghost k: renumbered owner cell, face kind
-    double *d_gnrm = nullptr;                    // ghost k: unit normal of the owner's face
-    // the reference's per-cell arrays in renumbered order (built on first use by the reference's
-    // second order)
-    int *d_rc_nf = nullptr, *d_rc_nbr = nullptr;
-    double *d_rc_dist = nullptr, *d_rc_nrm = nullptr, *d_rc_len = nullptr, *d_rc_area = nullptr;
-    int *d_fst = nullptr;  // WENO line stencils of the face entries, 4 per entry (built on first use)
-    // WENO: the tiles owning a cell of a face without a line stencil (the only gradients the
-    // MUSCL fallback faces read), as a tile list for the gradient pass of the time steps
-    int *d_gtiles = nullptr, *d_gcount = nullptr;
-    int ngtiles = 0;
-    std::mutex lazy;  // guards the arrays built on first use (solvers may share the mesh)
-    DevMesh dev{};
-    ~cfd_mesh_s() {
-        void *ps[] = {d_tile_cstart, d_tile_fstart, d_fgeo, d_fw, d_fnx, d_fny, d_fdl, d_fLR, d_slots, d_area, d_inv_area, d_gcell, d_gkind, d_gnrm,
-                      d_rc_nf, d_rc_nbr, d_rc_dist, d_rc_nrm, d_rc_len, d_rc_area, d_fst, d_gtiles, d_gcount};
-        for (void *p : ps)
-            if (p) (void)hipFree(p);
-    }
-};
 
 struct cfd_solver_s {
     cfd_mesh_s *m = nullptr;
@@ -301,6 +272,7 @@ int cfd_mesh_layout_export(const cfd_mesh_t *m, int *tile_cstart, int *tile_fsta
     return 0;
 }
 
+
 static int check_config(const cfd_mesh_s *m, const cfd_config_t *c) {
     if (c->flux < 0 || c->flux > 3) return fail(CFD_E_ARG, "config: flux must be 0..3");
     if (c->order < 1 || c->order > 3) return fail(CFD_E_ARG, "config: order must be 1, 2 or 3 (WENO5)");
@@ -392,13 +364,7 @@ int cfd_solver_configure(cfd_solver_t *s, const cfd_config_t *cfg) {
         if ((rc = freeze_init(s, 0))) return rc;
     }
     s->cfg = *cfg;
-    for (int k = 0; k < 6; k++) s->dcfg.bc[k] = cfg->bc[k];
-    for (int k = 0; k < 4; k++) s->dcfg.fs[k] = cfg->fs[k];
-    s->dcfg.cfl = cfg->cfl;
-    s->dcfg.roe_bdiss = cfg->roe_bdiss;
-    s->dcfg.xcd_remap = cfg->xcd_remap;
-    s->dcfg.grad_lds = cfg->grad_lds;
-    capi_fill_viscous(s->dcfg, *cfg);
+    capi_fill_devcfg(s->dcfg, *cfg);
     s->dt_valid = false;
     return 0;
 }
@@ -784,3 +750,17 @@ int cfd_run_steady(cfd_solver_t *s, double tol, int max_steps, int check_every, 
 }
 
 }  // extern "C"
+
+namespace cfd {
+int capi_check_config(const cfd_mesh_s *m, const cfd_config_t *c) { return check_config(m, c); }
+
+void capi_fill_devcfg(DevCfg &d, const cfd_config_t &c) {
+    for (int k = 0; k < 6; k++) d.bc[k] = c.bc[k];
+    for (int k = 0; k < 4; k++) d.fs[k] = c.fs[k];
+    d.cfl = c.cfl;
+    d.roe_bdiss = c.roe_bdiss;
+    d.xcd_remap = c.xcd_remap;
+    d.grad_lds = c.grad_lds;
+    capi_fill_viscous(d, c);
+}
+}  // namespace cfd

@@ -124,6 +124,21 @@ int cfd_state_device(cfd_solver_t *s, void **d_U);
  * gradients, order 2; may be NULL). Host outputs in the caller's order. */
 int cfd_residual(cfd_solver_t *s, double *R, double *dt, double *grad, void *stream);
 
+/* Allocation-free residual. No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). It runs cfd_residual's kernels on a caller's device state d_U (renumbered cell order,
+ * 4 doubles per cell; cfd_mesh_permutation gives the order) and writes device outputs: d_R
+ * [4*nphys] net flux and d_dt [nphys] local time step (may be NULL), both renumbered, and d_dtmin
+ * [1] their minimum (may be NULL). The gradients and the time-step partials live in a device
+ * array of the library (one per GPU, held until the process ends), so meshes are limited to
+ * CFD_NOALLOC_MAX_CELLS physical cells; larger ones are rejected with CFD_E_ARG. Order 1 or 2,
+ * native iteration (ref_mode 0); the mesh must live on the current device. Calls that overlap on
+ * one GPU (several streams or threads at once) share that array and race; calls in sequence do
+ * not, since every call overwrites what it reads. */
+#define CFD_NOALLOC_MAX_CELLS (1 << 20)
+int cfd_residual_noalloc(const cfd_mesh_t *m, const cfd_config_t *cfg, const void *d_U, void *d_R,
+                         void *d_dt, void *d_dtmin, void *stream);
+
 /* Sets the limiter-freezing state: 0 off (limiter evaluated), 1 record psi on the next step,
  * 2 use the recorded psi. With freeze_tol = freeze_stall = 0 the state only changes here. */
 int cfd_set_freeze(cfd_solver_t *s, int mode);
