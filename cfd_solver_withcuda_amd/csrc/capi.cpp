@@ -85,6 +85,11 @@ struct cfd_solver_s {
     cfd_config_t cfg{};
     DevCfg dcfg{};
     double4 *Un = nullptr, *Ua = nullptr, *Ub = nullptr, *acc = nullptr, *R = nullptr;
+    // positivity fallback: the last stage's output (swapped with Un after the step), the second
+    // RK4 accumulator, per-cell flags and per-tile marks (allocated on first use)
+    double4 *Uc = nullptr, *acc2 = nullptr;
+    uint8_t *pflag = nullptr;
+    int *dirty = nullptr;
     double *dt = nullptr, *grad = nullptr, *hbuf = nullptr, *err_partial = nullptr, *dt_partial = nullptr;
     double *scal = nullptr;  // [0],[1] global dt slots, [2] stage minimum, [3..6] error norm
     double4 *Ug = nullptr;   // ghost states (reference ghost order), allocated on first use
@@ -97,7 +102,8 @@ struct cfd_solver_s {
     bool dt_valid = false;
     bool has_state = false;
     ~cfd_solver_s() {
-        void *ps[] = {Un, Ua, Ub, acc, R, dt, grad, hbuf, err_partial, dt_partial, scal, Ug, psi, fz, fz_best, Uall};
+        void *ps[] = {Un, Ua, Ub, acc, R, dt, grad, hbuf, err_partial, dt_partial, scal, Ug, psi, fz, fz_best, Uall,
+                      Uc, acc2, pflag, dirty};
         for (void *p : ps)
             if (p) (void)hipFree(p);
     }
@@ -289,6 +295,8 @@ static int check_config(const cfd_mesh_s *m, const cfd_config_t *c) {
     for (int k = 1; k <= 5; k++)
         if (c->bc[k] < 1 || c->bc[k] > 8) return fail(CFD_E_ARG, "config: bc types must be 1..8");
     if (int rc = capi_check_viscous(c, "config")) return rc;
+    if (c->positivity && (c->order != 2 || c->ref_mode || c->viscous))
+        return fail(CFD_E_ARG, "config: the positivity fallback needs order 2, the native iteration and no viscous terms");
     if (m->dev.tile > c->block) return fail(CFD_E_ARG, "config: the mesh tile (cells per tile) must not exceed the block size");
     const size_t sm = flux_smem_bytes(m->dev, c->order, c->grad_lds);
     if (sm > 65536 || grad_smem_bytes(m->dev, c->gradient) > 65536)
@@ -578,6 +586,19 @@ int cfd_set_freeze(cfd_solver_t *s, int mode) {
     return freeze_init(s, mode);
 }
 
+// Buffers of the positivity fallback (first use); the tile marks start cleared and every
+// recomputing pass clears the marks it consumed.
+static int ensure_positivity_buffers(cfd_solver_s *s) {
+    if (s->dirty) return 0;
+    const size_t nc = s->m->ref.nc;
+    HIP_TRY(hipMalloc((void **)&s->Uc, nc * sizeof(double4)));
+    HIP_TRY(hipMalloc((void **)&s->acc2, nc * sizeof(double4)));
+    HIP_TRY(hipMalloc((void **)&s->pflag, nc));
+    HIP_TRY(hipMalloc((void **)&s->dirty, (size_t)s->dev.ntiles * sizeof(int)));
+    HIP_TRY(hipMemset(s->dirty, 0, (size_t)s->dev.ntiles * sizeof(int)));
+    return 0;
+}
+
 // One native step (fused stages). want_err: emit the error norm into scal[3..6].
 static int native_step(cfd_solver_s *s, bool want_err, hipStream_t st) {
     const cfd_config_t &c = s->cfg;
@@ -595,10 +616,20 @@ static int native_step(cfd_solver_s *s, bool want_err, hipStream_t st) {
         s->dt_valid = true;
     }
     const int nst = (c.rk == 4) ? 4 : 3;
+    const bool pos = c.positivity != 0;
+    if (pos) {
+        if (int rc = ensure_positivity_buffers(s)) return rc;
+    }
+    // With the positivity fallback no stage writes an array it reads (its recomputation reads the
+    // stage inputs again): the last stage writes Uc, swapped with Un after the step, and the RK4
+    // accumulator alternates between acc and acc2.
+    double4 *last = pos ? s->Uc : s->Un;
     const double4 *ins3[3] = {s->Un, s->Ua, s->Ub};
-    double4 *outs3[3] = {s->Ua, s->Ub, s->Un};
+    double4 *outs3[3] = {s->Ua, s->Ub, last};
     const double4 *ins4[4] = {s->Un, s->Ua, s->Ub, s->Ua};
-    double4 *outs4[4] = {s->Ua, s->Ub, s->Ua, s->Un};
+    double4 *outs4[4] = {s->Ua, s->Ub, s->Ua, last};
+    double4 *acc_w[4] = {s->acc, pos ? s->acc2 : s->acc, s->acc, s->acc};
+    const double4 *acc_r[4] = {s->acc, s->acc, pos ? s->acc2 : s->acc, s->acc};
     for (int k = 0; k < nst; k++) {
         const double4 *in = (nst == 4) ? ins4[k] : ins3[k];
         double4 *out = (nst == 4) ? outs4[k] : outs3[k];
@@ -610,7 +641,8 @@ static int native_step(cfd_solver_s *s, bool want_err, hipStream_t st) {
         a.last = (k == nst - 1);
         a.Un = s->Un;
         a.Uout = out;
-        a.acc = s->acc;
+        a.acc_in = acc_r[k];
+        a.acc_out = acc_w[k];
         a.hbuf = s->hbuf;
         a.dt_global = s->scal + s->cur;
         a.err_partial = (a.last && want_err) ? s->err_partial : nullptr;
@@ -618,8 +650,16 @@ static int native_step(cfd_solver_s *s, bool want_err, hipStream_t st) {
         // recorded factors are used only while freezing is automatic or was set explicitly
         a.freeze = (auto_freeze(s) || s->fz_explicit) ? s->fz : nullptr;
         a.psi = s->psi;
-        HIP_TRY(launch_flux(dm, s->dcfg, c.flux, c.order, c.limiter, MODE_STAGE, in, s->grad, a, c.block, st));
+        if (pos) {
+            a.pflag = s->pflag;
+            a.dirty = s->dirty;
+            HIP_TRY(launch_flux_pos(dm, s->dcfg, c.flux, c.limiter, 1, in, s->grad, a, c.block, st));
+            HIP_TRY(launch_flux_pos(dm, s->dcfg, c.flux, c.limiter, 2, in, s->grad, a, c.block, st));
+        } else {
+            HIP_TRY(launch_flux(dm, s->dcfg, c.flux, c.order, c.limiter, MODE_STAGE, in, s->grad, a, c.block, st));
+        }
     }
+    if (pos) std::swap(s->Un, s->Uc);
     if (!c.lts) HIP_TRY(launch_min_reduce(s->dt_partial, dm.ntiles, s->scal + (1 - s->cur), st));
     if (want_err) HIP_TRY(launch_err_finalize(s->err_partial, dm.ntiles, s->scal + 3, st));
     if (want_err && auto_freeze(s))
@@ -695,7 +735,8 @@ int cfd_time_pass(cfd_solver_t *s, int which, int reps, float *ms, void *stream)
     a.last = 0;
     a.Un = s->Un;
     a.Uout = s->Ua;
-    a.acc = s->acc;
+    a.acc_in = s->acc;
+    a.acc_out = s->acc;
     a.hbuf = s->hbuf;
     a.dt_global = s->scal + s->cur;
     hipEvent_t e0, e1;

@@ -130,6 +130,7 @@ int check_batch_config(const cfd_config_t *c) {
     if (!(c->freeze_tol >= 0.0) || c->freeze_stall < 0)
         return capi_fail(CFD_E_ARG, "batch config: freeze_tol and freeze_stall must be >= 0");
     if (!c->lts) return capi_fail(CFD_E_ARG, "batch config: batched steady solves use local time stepping (lts = 1)");
+    if (c->positivity) return capi_fail(CFD_E_ARG, "batch config: the positivity fallback exists for single solvers only");
     for (int k = 1; k <= 5; k++)
         if (c->bc[k] < 1 || c->bc[k] > 8) return capi_fail(CFD_E_ARG, "batch config: bc types must be 1..8");
     return capi_check_viscous(c, "batch config");
@@ -354,7 +355,8 @@ int batch_step_launches(cfd_batch_s *b, double tol) {
         a.last = (k == nst - 1);
         a.Un = b->Un;
         a.Uout = out;
-        a.acc = b->acc;
+        a.acc_in = b->acc;
+        a.acc_out = b->acc;
         a.hbuf = b->hbuf;
         a.err_partial = a.last ? b->err_partial : nullptr;
         a.freeze = b->d_fz;

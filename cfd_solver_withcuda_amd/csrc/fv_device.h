@@ -43,6 +43,19 @@ __device__ __forceinline__ void state_from_U(double U0, double U1, double U2, do
     s.U[3] = U3;
 }
 
+// Density and pressure of a conserved state, with state_from_U's arithmetic.
+__device__ __forceinline__ void rho_p_of(const double *U, double &rho, double &p) {
+    double Rho = U[0];
+    double inv = 1.0 / Rho;
+    double v1 = U[1] * inv;
+    double v2 = U[2] * inv;
+    if (fabs(v1) <= 1e-10) v1 = 0.0;
+    if (fabs(v2) <= 1e-10) v2 = 0.0;
+    double vmag = (v1 * v1 + v2 * v2);
+    rho = Rho;
+    p = kGm1 * (U[3] - 0.5 * Rho * vmag);
+}
+
 // Calculate_Computational_Variables case 2 (src/Basic_Functions.cpp:171-206).
 __device__ __forceinline__ void cons_from_prim(double rho, double u, double v, double p, double *U) {
     double vmag = 0.5 * (u * u + v * v);

@@ -91,6 +91,17 @@ __device__ __forceinline__ double block_min(double *red, double v) {
     return red[0];
 }
 
+// Tile of renumbered cell c (tiles are contiguous cell ranges; binary search of tile_cstart).
+__device__ __forceinline__ int tile_of_cell(const DevMesh &m, int c) {
+    int lo = 0, hi = m.ntiles;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (m.tile_cstart[mid] <= c) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
 // reference error term (src/Error_Estimate_Update.cpp:10-28)
 __device__ __forceinline__ double err_term(double d, double u) {
     double t = (d < 1e-9) ? fabs(d) : (fabs(d) / u);
@@ -155,15 +166,31 @@ __device__ __forceinline__ void load_grad(const double *__restrict__ grad, int c
     }
 }
 
-template <int FLUX, int ORDER, int LIM, bool VISC>
+template <int FLUX, int ORDER, int LIM, bool VISC, int POS = 0>
 __device__ __forceinline__ void flux_face(const DevMesh &m, const DevCfg &cfg,
                                           const double4 *__restrict__ Us,
                                           const double *__restrict__ grad, const double4 *Uexit,
                                           const double *sW, const double *sU, const double *sG,
                                           double *sF, bool glds, int T, int FT, int c0, int nct,
                                           int j, const FaceRec &f, int fz, float *psi, int e,
-                                          const double *fsv) {
+                                          const double *fsv, const uint8_t *pflag = nullptr,
+                                          double *sNb = nullptr) {
     const double nx = f.nx, ny = f.ny, dl = f.dl;
+    if constexpr (POS == 2) {
+        // positivity fallback, recomputation: a face of a flagged cell takes the two cells'
+        // first-order states (conserved values as stored, ghost of the left state)
+        const bool bnd = f.R < 0;
+        if (pflag[f.L] || (!bnd && pflag[f.R])) {
+            FState SL, SR;
+            load_state(Us, f.L, SL);
+            if (bnd) ghost_state(cfg.bc[-f.R], SL, nx, ny, fsv, SR);
+            else load_state(Us, f.R, SR);
+            double F[4];
+            face_flux<FLUX>(bnd, cfg.roe_bdiss, SL, SR, nx, ny, dl, F);
+            for (int q = 0; q < 4; q++) sF[q * FT + j] = F[q];
+            return;
+        }
+    }
     const int lL = f.L - c0;
     const bool locL = (unsigned)lL < (unsigned)nct;
     FState SL, SR;
@@ -201,6 +228,15 @@ __device__ __forceinline__ void flux_face(const DevMesh &m, const DevCfg &cfg,
         } else {
             load_state(Us, f.R, SR);
         }
+    }
+    if constexpr (POS == 1) {
+        // positivity fallback, detection: density and pressure of the face's two cells (the
+        // cell's own values on a boundary face, whose other side is a ghost) for the gather's
+        // neighbourhood range
+        sNb[j] = SL.W[0];
+        sNb[FT + j] = SL.W[3];
+        sNb[2 * FT + j] = boundary ? SL.W[0] : SR.W[0];
+        sNb[3 * FT + j] = boundary ? SL.W[3] : SR.W[3];
     }
     double F[4];
     if constexpr (ORDER == 3) {
@@ -308,7 +344,7 @@ __device__ __forceinline__ void flux_face(const DevMesh &m, const DevCfg &cfg,
     for (int q = 0; q < 4; q++) sF[q * FT + j] = F[q];
 }
 
-template <int FLUX, int ORDER, int LIM, int MODE, bool VISC>
+template <int FLUX, int ORDER, int LIM, int MODE, bool VISC, int POS = 0>
 __global__ void __launch_bounds__(256) CFD_FLUX_ATTR flux_tile_kernel(DevMesh m, DevCfg cfg,
                                                         const double4 *__restrict__ Us,
                                                         const double *__restrict__ grad,
@@ -316,6 +352,9 @@ __global__ void __launch_bounds__(256) CFD_FLUX_ATTR flux_tile_kernel(DevMesh m,
     extern __shared__ double smem[];
     const int tile = block_tile(m, cfg.xcd_remap);
     if (tile < 0) return;
+    if constexpr (POS == 2) {
+        if (!st.dirty[tile]) return;  // no face of this tile touches a flagged cell
+    }
     double fsv[4];
     tile_fs(m, cfg, tile, fsv);
     const int T = m.tile;
@@ -331,6 +370,7 @@ __global__ void __launch_bounds__(256) CFD_FLUX_ATTR flux_tile_kernel(DevMesh m,
     double *sF = sU + (ORDER != 2 ? 4 * T : 0); // [4][FT] face fluxes
     double *sG = sF + 4 * FT;                   // [8][T] (MUSCL, LDS-staged gradients)
     double *sN = sG + (glds ? 8 * T : 0);       // [3][FT] face normal and length (time steps)
+    double *sNb = sN + 3 * FT;                  // [4][FT] rho, p of both cells (positivity detection)
     // phase C forms time steps from the face geometry: staged here from the face records of
     // phase B instead of read again from HBM
     const bool needN = (MODE == MODE_RESID) || (st.lts && st.stage == 0) || (st.last && st.dt_partial);
@@ -386,17 +426,17 @@ __global__ void __launch_bounds__(256) CFD_FLUX_ATTR flux_tile_kernel(DevMesh m,
     // ---- B: one flux per face ----------------------------------------------------------
     if (h0) {
         stageN(k, f0);
-        flux_face<FLUX, ORDER, LIM, VISC>(m, cfg, Us, grad, st.Uexit, sW, sU, sG, sF, glds, T, FT, c0, nct, k, f0, fz, st.psi, fs + k, fsv);
+        flux_face<FLUX, ORDER, LIM, VISC, POS>(m, cfg, Us, grad, st.Uexit, sW, sU, sG, sF, glds, T, FT, c0, nct, k, f0, fz, st.psi, fs + k, fsv, st.pflag, sNb);
     }
     if (h1) {
         stageN(k + B, f1);
-        flux_face<FLUX, ORDER, LIM, VISC>(m, cfg, Us, grad, st.Uexit, sW, sU, sG, sF, glds, T, FT, c0, nct, k + B, f1, fz, st.psi, fs + k + B, fsv);
+        flux_face<FLUX, ORDER, LIM, VISC, POS>(m, cfg, Us, grad, st.Uexit, sW, sU, sG, sF, glds, T, FT, c0, nct, k + B, f1, fz, st.psi, fs + k + B, fsv, st.pflag, sNb);
     }
     for (int j = k + 2 * B; j < nft; j += B) {
         FaceRec f;
         load_face<true, ORDER >= 2>(m, fs + j, f);
         stageN(j, f);
-        flux_face<FLUX, ORDER, LIM, VISC>(m, cfg, Us, grad, st.Uexit, sW, sU, sG, sF, glds, T, FT, c0, nct, j, f, fz, st.psi, fs + j, fsv);
+        flux_face<FLUX, ORDER, LIM, VISC, POS>(m, cfg, Us, grad, st.Uexit, sW, sU, sG, sF, glds, T, FT, c0, nct, j, f, fz, st.psi, fs + j, fsv, st.pflag, sNb);
     }
     __syncthreads();
 
@@ -453,7 +493,7 @@ __global__ void __launch_bounds__(256) CFD_FLUX_ATTR flux_tile_kernel(DevMesh m,
             const double Un[4] = {un4.x, un4.y, un4.z, un4.w};
             double Uo[4];
             if (st.rk == 4) {
-                double4 a4 = (st.stage == 0) ? make_double4(0, 0, 0, 0) : st.acc[c];
+                double4 a4 = (st.stage == 0) ? make_double4(0, 0, 0, 0) : st.acc_in[c];
                 double acc[4] = {a4.x, a4.y, a4.z, a4.w};
                 for (int q = 0; q < 4; q++) {
                     switch (st.stage) {
@@ -463,7 +503,7 @@ __global__ void __launch_bounds__(256) CFD_FLUX_ATTR flux_tile_kernel(DevMesh m,
                     default: Uo[q] = Un[q] - (h / 6.0) * (acc[q] + r[q]); break;
                     }
                 }
-                if (st.stage < 3) st.acc[c] = make_double4(acc[0], acc[1], acc[2], acc[3]);
+                if (st.stage < 3) st.acc_out[c] = make_double4(acc[0], acc[1], acc[2], acc[3]);
             } else {
                 const double us[4] = {u4.x, u4.y, u4.z, u4.w};
                 for (int q = 0; q < 4; q++) {
@@ -471,6 +511,42 @@ __global__ void __launch_bounds__(256) CFD_FLUX_ATTR flux_tile_kernel(DevMesh m,
                     case 0: Uo[q] = Un[q] - h * r[q]; break;
                     case 1: Uo[q] = 0.75 * Un[q] + 0.25 * (us[q] - h * r[q]); break;
                     default: Uo[q] = (1.0 / 3.0) * Un[q] + (2.0 / 3.0) * (us[q] - h * r[q]); break;
+                    }
+                }
+            }
+            if constexpr (POS == 1) {
+                // positivity fallback, detection: the stage state must have a positive density and
+                // pressure within the relaxed range of the cell's and its face neighbours' stage
+                // inputs (the same test as the oracle's stage_fallback_flags)
+                double rn, pn;
+                rho_p_of(Uo, rn, pn);
+                double rlo = sW[k], rhi = rlo, plo = sW[3 * T + k], phi = plo;
+                for (int f = 0; f < 4; f++) {
+                    const uint16_t sl = sls[f];
+                    if (sl == 0xFFFF) break;
+                    const int j = sl >> 1;
+                    const bool isR = sl & 1;
+                    const double rj = isR ? sNb[j] : sNb[2 * FT + j];
+                    const double pj = isR ? sNb[FT + j] : sNb[3 * FT + j];
+                    rlo = fmin(rlo, rj);
+                    rhi = fmax(rhi, rj);
+                    plo = fmin(plo, pj);
+                    phi = fmax(phi, pj);
+                }
+                const double dr = fmax(1e-3 * fabs(rhi), 1e-3 * (rhi - rlo));
+                const double dp = fmax(1e-3 * fabs(phi), 1e-3 * (phi - plo));
+                const bool bad = !(rn > 0.0) || !(pn > 0.0) || !(rn >= rlo - dr && rn <= rhi + dr) ||
+                                 !(pn >= plo - dp && pn <= phi + dp);
+                st.pflag[c] = bad ? 1 : 0;
+                if (bad) {
+                    // this tile and the tiles of the cell's face neighbours recompute the stage
+                    st.dirty[tile] = 1;
+                    for (int f = 0; f < 4; f++) {
+                        const uint16_t sl = sls[f];
+                        if (sl == 0xFFFF) break;
+                        const int4 a = reinterpret_cast<const int4 *>(m.fgeo)[3 * (size_t)(fs + (sl >> 1))];
+                        const int o = (sl & 1) ? a.x : a.y;
+                        if (o >= 0 && (unsigned)(o - c0) >= (unsigned)nct) st.dirty[tile_of_cell(m, o)] = 1;
                     }
                 }
             }
@@ -519,6 +595,9 @@ __global__ void __launch_bounds__(256) CFD_FLUX_ATTR flux_tile_kernel(DevMesh m,
             double b = block_min(smem, dtmin);
             if (threadIdx.x == 0) st.dt_partial[tile] = b;
         }
+    }
+    if constexpr (POS == 2) {
+        if (threadIdx.x == 0) st.dirty[tile] = 0;  // read by every thread before the first barrier
     }
 }
 

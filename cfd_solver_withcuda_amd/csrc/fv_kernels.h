@@ -16,7 +16,8 @@ struct StageArgs {
     int last;    // last stage: emit error partials / next-step dt
     const double4 *Un;   // state at the start of the step
     double4 *Uout;       // stage output
-    double4 *acc;        // RK4 accumulator
+    const double4 *acc_in;  // RK4 accumulator read by stages 1..3
+    double4 *acc_out;       // RK4 accumulator written by stages 0..2 (may equal acc_in)
     double *hbuf;        // per-cell dt/area (LTS)
     const double *dt_global;
     double *R_out;
@@ -28,10 +29,22 @@ struct StageArgs {
     // psi [nfe][8] = slope / (grad W . d) per face entry, side (L, R) and variable (FP32).
     const int *freeze;
     float *psi;
+    // Positivity fallback of native MUSCL stages: the detecting launch writes pflag[cell] (1 = the
+    // stage state failed the admissibility check) and marks in dirty[tile] the tiles owning such a
+    // cell or one of its face neighbours; the recomputing launch redoes only marked tiles (with
+    // first-order states on the faces of flagged cells) and clears their marks.
+    uint8_t *pflag;
+    int *dirty;
 };
 
 size_t grad_smem_bytes(const DevMesh &m, int grad_type);
 size_t flux_smem_bytes(const DevMesh &m, int order, int grad_lds);
+// Positivity fallback (order 2, MODE_STAGE, inviscid; fv_pos.hip): pass 1 = the stage with
+// detection, pass 2 = the recomputation of marked tiles. Inputs and outputs of a stage must not
+// alias (pass 2 reads the stage's inputs again).
+hipError_t launch_flux_pos(const DevMesh &m, const DevCfg &c, int flux, int lim, int pass,
+                           const double4 *Us, const double *grad, const StageArgs &st, int block,
+                           hipStream_t s);
 hipError_t launch_grad(const DevMesh &m, const DevCfg &c, int grad_type, const double4 *Us,
                        double *grad, int block, hipStream_t s);
 hipError_t launch_flux(const DevMesh &m, const DevCfg &c, int flux, int order, int lim, int mode,

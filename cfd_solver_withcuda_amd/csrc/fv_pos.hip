@@ -1,0 +1,54 @@
+// Flux pass of the native MUSCL stages with the positivity fallback (order 2, inviscid, every
+// flux and limiter): pass 1 runs the stage and flags the cells whose new state fails the
+// admissibility check, pass 2 recomputes the tiles holding a face of a flagged cell with
+// first-order states on those faces. Compiled apart so that it builds in parallel with the other
+// variants.
+#include "fv_flux.h"
+
+namespace cfd {
+
+namespace {
+
+template <int FLUX, int LIM, int PASS>
+void launch_pos_t(const DevMesh &m, const DevCfg &c, const double4 *Us, const double *grad,
+                  const StageArgs &st, int block, size_t sm, hipStream_t s) {
+    hipLaunchKernelGGL((flux_tile_kernel<FLUX, 2, LIM, MODE_STAGE, false, PASS>), dim3(grid_of(m, c)),
+                       dim3(block), sm, s, m, c, Us, grad, st);
+}
+
+template <int PASS>
+hipError_t launch_pos_pass(const DevMesh &m, const DevCfg &c, int flux, int lim, const double4 *Us,
+                           const double *grad, const StageArgs &st, int block, size_t sm, hipStream_t s) {
+    auto by_lim = [&](auto fluxc) {
+        constexpr int F = decltype(fluxc)::value;
+        switch (lim) {
+        case LIM_NONE: launch_pos_t<F, LIM_NONE, PASS>(m, c, Us, grad, st, block, sm, s); break;
+        case LIM_MINMOD: launch_pos_t<F, LIM_MINMOD, PASS>(m, c, Us, grad, st, block, sm, s); break;
+        case LIM_VANLEER: launch_pos_t<F, LIM_VANLEER, PASS>(m, c, Us, grad, st, block, sm, s); break;
+        default: launch_pos_t<F, LIM_SUPERBEE, PASS>(m, c, Us, grad, st, block, sm, s); break;
+        }
+    };
+    switch (flux) {
+    case FLUX_ROE: by_lim(std::integral_constant<int, FLUX_ROE>{}); break;
+    case FLUX_HLLC: by_lim(std::integral_constant<int, FLUX_HLLC>{}); break;
+    case FLUX_LLF: by_lim(std::integral_constant<int, FLUX_LLF>{}); break;
+    default: by_lim(std::integral_constant<int, FLUX_AUSM>{}); break;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_flux_pos(const DevMesh &m, const DevCfg &c, int flux, int lim, int pass,
+                           const double4 *Us, const double *grad, const StageArgs &st, int block,
+                           hipStream_t s) {
+    size_t sm = flux_smem_bytes(m, 2, c.grad_lds);
+    if (pass == 1) {
+        // the detection's per-face densities and pressures follow the face geometry in LDS
+        sm += sizeof(double) * 4 * (size_t)m.max_tile_faces;
+        return launch_pos_pass<1>(m, c, flux, lim, Us, grad, st, block, sm, s);
+    }
+    return launch_pos_pass<2>(m, c, flux, lim, Us, grad, st, block, sm, s);
+}
+
+}  // namespace cfd

@@ -1126,8 +1126,18 @@ static void weno_face_state(const double *U, const FState *cell, FState *out) {
     if (!(out->W[0] > 0.0) || !(out->W[3] > 0.0)) *out = *cell;
 }
 
+/* The face-loop residual; fo (may be NULL) marks cells whose faces all take first-order states
+ * (the positivity fallback's recomputation). */
+static void residual_fo(const OMesh *m, const ONatCfg *cfg, const double *U, const double *U_exit_src,
+                        double *R, double *dt, double *grad, const unsigned char *fo);
+
 void on_residual(const OMesh *m, const ONatCfg *cfg, const double *U, const double *U_exit_src,
                  double *R, double *dt, double *grad) {
+    residual_fo(m, cfg, U, U_exit_src, R, dt, grad, NULL);
+}
+
+static void residual_fo(const OMesh *m, const ONatCfg *cfg, const double *U, const double *U_exit_src,
+                        double *R, double *dt, double *grad, const unsigned char *fo) {
     const int nc = m->nc;
     FState *S = (FState *)malloc(sizeof(FState) * nc);
     for (int c = 0; c < nc; c++) state_from_U(&U[4 * c], &S[c]);
@@ -1172,7 +1182,8 @@ void on_residual(const OMesh *m, const ONatCfg *cfg, const double *U, const doub
             }
             continue;
         }
-        if (cfg->order >= 2) {
+        const int first = fo && (fo[L] || (!boundary && fo[Rc]));
+        if (cfg->order >= 2 && !first) {
             double dx = m->ctr[3 * Rc] - m->ctr[3 * L];
             double dy = m->ctr[3 * Rc + 1] - m->ctr[3 * L + 1];
             FState fL, fR;
@@ -1271,6 +1282,58 @@ void or_rk_stage(int rk, int stage, int n, const double *Un, const double *Us, d
         }
 }
 
+/* Density and pressure of a conserved state, with the arithmetic of Calculate_Primitive_Variables
+ * (src/Basic_Functions.cpp:85-142). */
+static void rho_p_of(const double *U, double *rho, double *p) {
+    double Rho = U[0];
+    double inv = 1.0 / Rho;
+    double v1 = U[1] * inv;
+    double v2 = U[2] * inv;
+    if (fabs(v1) <= 1e-10) v1 = 0.0;
+    if (fabs(v2) <= 1e-10) v2 = 0.0;
+    double vmag = (v1 * v1 + v2 * v2);
+    *rho = Rho;
+    *p = GM1 * (U[3] - 0.5 * Rho * vmag);
+}
+
+static int g_fallback_cells = 0;
+int on_last_fallback_cells(void) { return g_fallback_cells; }
+
+/* Positivity fallback, detection (no counterpart in the reference; an a-posteriori "MOOD"
+ * check): flag[c] = 1 when the stage state Unew of cell c has a non-positive (or non-finite)
+ * density or pressure, or a density or pressure outside [lo - d, hi + d], where lo and hi are the
+ * minimum and maximum of the stage input Uin over the cell and its physical face neighbours and
+ * d = 1e-3 max(|hi|, hi - lo). Returns the number of flagged cells. */
+static int stage_fallback_flags(const OMesh *m, const double *Uin, const double *Unew,
+                                unsigned char *flag) {
+    const int nc = m->nc;
+    int nb = 0;
+    for (int c = 0; c < nc; c++) {
+        double r0, p0, rn, pn;
+        rho_p_of(&Uin[4 * c], &r0, &p0);
+        rho_p_of(&Unew[4 * c], &rn, &pn);
+        double rlo = r0, rhi = r0, plo = p0, phi = p0;
+        for (int f = 0; f < m->nf[c]; f++) {
+            int j = m->nbr[4 * c + f];
+            if (j < 0 || j >= nc) continue;
+            double rj, pj;
+            rho_p_of(&Uin[4 * j], &rj, &pj);
+            rlo = fmin(rlo, rj);
+            rhi = fmax(rhi, rj);
+            plo = fmin(plo, pj);
+            phi = fmax(phi, pj);
+        }
+        const double dr = fmax(1e-3 * fabs(rhi), 1e-3 * (rhi - rlo));
+        const double dp = fmax(1e-3 * fabs(phi), 1e-3 * (phi - plo));
+        const int bad = !(rn > 0.0) || !(pn > 0.0) || !(rn >= rlo - dr && rn <= rhi + dr) ||
+                        !(pn >= plo - dp && pn <= phi + dp);
+        flag[c] = (unsigned char)bad;
+        nb += bad;
+    }
+    g_fallback_cells += nb;
+    return nb;
+}
+
 void on_step(const OMesh *m, const ONatCfg *cfg, int rk, int lts, double *U, double *err4,
              double *dt_out) {
     const int nc = m->nc;
@@ -1285,15 +1348,34 @@ void on_step(const OMesh *m, const ONatCfg *cfg, int rk, int lts, double *U, dou
     memcpy(Un, U, sizeof(double) * n4);
     memcpy(Us, U, sizeof(double) * n4);
     const int nst = (rk == 4) ? 4 : 3;
+    const int pos = cfg->positivity && cfg->order == 2;
+    double *Uin = pos ? (double *)malloc(sizeof(double) * n4) : NULL;
+    double *acc_in = pos ? (double *)malloc(sizeof(double) * n4) : NULL;
+    unsigned char *flag = pos ? (unsigned char *)malloc(nc) : NULL;
     double dtg = 0.0;
+    g_fallback_cells = 0;
     for (int s = 0; s < nst; s++) {
         on_residual(m, cfg, Us, NULL, R, dt, grad);
         if (s == 0) {
             dtg = min_dt(nc, dt);
             for (int c = 0; c < nc; c++) h[c] = (lts ? dt[c] : dtg) * m->inv_area[c];
         }
-        or_rk_stage(rk == 4 ? 4 : 3, s, nc, Un, Us, acc, R, h, (s == nst - 1) ? U : Us);
+        double *out = (s == nst - 1) ? U : Us;
+        if (pos) {
+            memcpy(Uin, Us, sizeof(double) * n4);
+            memcpy(acc_in, acc, sizeof(double) * n4);
+        }
+        or_rk_stage(rk == 4 ? 4 : 3, s, nc, Un, Us, acc, R, h, out);
+        if (pos && stage_fallback_flags(m, Uin, out, flag)) {
+            /* recompute the stage with first-order states on every face of a flagged cell */
+            residual_fo(m, cfg, Uin, NULL, R, dt, grad, flag);
+            memcpy(acc, acc_in, sizeof(double) * n4);
+            or_rk_stage(rk == 4 ? 4 : 3, s, nc, Un, Uin, acc, R, h, out);
+        }
     }
+    free(Uin);
+    free(acc_in);
+    free(flag);
     for (size_t i = 0; i < n4; i++) acc[i] = U[i] - Un[i];
     or_error(nc, acc, Un, err4);
     *dt_out = dtg;

@@ -72,6 +72,10 @@ typedef struct {
     double mu;       /* dynamic viscosity (viscous == 1) */
     double prandtl;  /* Prandtl number: conductivity k = mu gamma r_gas / ((gamma - 1) Pr) */
     double r_gas;    /* gas constant: T = p / (rho r_gas) */
+    int positivity;  /* a-posteriori first-order fallback of the native MUSCL stages (order 2): a
+                        cell whose stage state has a non-positive density or pressure, or a density
+                        or pressure outside the relaxed range of its own and its face neighbours'
+                        stage-input values, is recomputed with first-order states on all its faces */
 } ONatCfg;
 
 /* ---- primitive / ghost helpers --------------------------------------------------------- */
@@ -91,6 +95,8 @@ double or_limiter_slope(int limiter, double dminus, double dplus);
  * stages copy U_Cells, which still holds U^n). */
 void on_residual(const OMesh *m, const ONatCfg *cfg, const double *U, const double *U_exit_src,
                  double *R, double *dt, double *grad);
+/* cells recomputed at first order by the positivity fallback during the last on_step */
+int on_last_fallback_cells(void);
 void on_gradients(const OMesh *m, const ONatCfg *cfg, const double *W5, double *grad);
 void on_face_flux(const ONatCfg *cfg, int boundary, const double *WL5, const double *WR5,
                   double nx, double ny, double dl, double *F4);

@@ -44,12 +44,12 @@ class ONatCfg(ctypes.Structure):
                 ("gradient", ctypes.c_int), ("roe_bdiss", ctypes.c_int), ("cfl", ctypes.c_double),
                 ("bc", ctypes.c_int * 6), ("fs", ctypes.c_double * 4), ("freeze", ctypes.c_int),
                 ("phi", ctypes.c_void_p), ("viscous", ctypes.c_int), ("mu", ctypes.c_double),
-                ("prandtl", ctypes.c_double), ("r_gas", ctypes.c_double)]
+                ("prandtl", ctypes.c_double), ("r_gas", ctypes.c_double), ("positivity", ctypes.c_int)]
 
 
 def make_cfg(flux="roe", order=1, limiter="vanleer", gradient="gg", roe_bdiss=0, cfl=0.5,
              bc=("inlet_ref", "exit", "wall", "wall", "wall"), fs=(1.0, 0.0, 0.0, 1.0), viscous="off",
-             mu=0.0, prandtl=0.72, r_gas=287.0) -> ONatCfg:
+             mu=0.0, prandtl=0.72, r_gas=287.0, positivity=0) -> ONatCfg:
     c = ONatCfg()
     c.flux = OFLUX[flux]
     c.order = order
@@ -66,6 +66,7 @@ def make_cfg(flux="roe", order=1, limiter="vanleer", gradient="gg", roe_bdiss=0,
     c.mu = mu
     c.prandtl = prandtl
     c.r_gas = r_gas
+    c.positivity = int(positivity)
     return c
 
 
@@ -147,6 +148,8 @@ class Oracle:
         L.on_face_flux.argtypes = [cfgp, ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]
         L.on_step.argtypes = [ctypes.c_void_p, cfgp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]
         L.on_refstep.argtypes = [ctypes.c_void_p, cfgp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]
+        L.on_last_fallback_cells.restype = ctypes.c_int
+        L.on_last_fallback_cells.argtypes = []
         L.or_error.argtypes = [ctypes.c_int, _dp, _dp, _dp]
         L.or_rk_stage.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
         L.or_gg_gradient.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, _dp]
@@ -169,6 +172,10 @@ class Oracle:
     # ---- mesh ----
     def mesh(self, points: np.ndarray, cells: np.ndarray) -> "OracleMesh":
         return OracleMesh(self, points, cells)
+
+    def last_fallback_cells(self) -> int:
+        """Cells the positivity fallback recomputed at first order during the last step."""
+        return int(self.lib.on_last_fallback_cells())
 
     def prim(self, U4) -> np.ndarray:
         U = np.ascontiguousarray(U4, dtype=np.float64)
