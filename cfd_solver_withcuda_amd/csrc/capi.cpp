@@ -295,6 +295,7 @@ static int check_config(const cfd_mesh_s *m, const cfd_config_t *c) {
     for (int k = 1; k <= 5; k++)
         if (c->bc[k] < 1 || c->bc[k] > 8) return fail(CFD_E_ARG, "config: bc types must be 1..8");
     if (int rc = capi_check_viscous(c, "config")) return rc;
+    if (c->positivity < 0 || c->positivity > 2) return fail(CFD_E_ARG, "config: positivity must be 0, 1 or 2");
     if (c->positivity && (c->order != 2 || c->ref_mode || c->viscous))
         return fail(CFD_E_ARG, "config: the positivity fallback needs order 2, the native iteration and no viscous terms");
     if (m->dev.tile > c->block) return fail(CFD_E_ARG, "config: the mesh tile (cells per tile) must not exceed the block size");
@@ -653,6 +654,7 @@ static int native_step(cfd_solver_s *s, bool want_err, hipStream_t st) {
         if (pos) {
             a.pflag = s->pflag;
             a.dirty = s->dirty;
+            a.pos_range = (c.positivity == 1);
             HIP_TRY(launch_flux_pos(dm, s->dcfg, c.flux, c.limiter, 1, in, s->grad, a, c.block, st));
             HIP_TRY(launch_flux_pos(dm, s->dcfg, c.flux, c.limiter, 2, in, s->grad, a, c.block, st));
         } else {
@@ -716,7 +718,7 @@ int cfd_step(cfd_solver_t *s, int nsteps, double *err_hist, double *dt_hist, voi
 int cfd_time_pass(cfd_solver_t *s, int which, int reps, float *ms, void *stream) {
     if (!s || reps <= 0 || !ms) return fail(CFD_E_ARG, "cfd_time_pass: bad arguments");
     if (!s->has_state) return fail(CFD_E_STATE, "cfd_time_pass: state not set");
-    if (which < 0 || which > 2) return fail(CFD_E_ARG, "cfd_time_pass: which must be 0, 1 or 2");
+    if (which < 0 || which > 3) return fail(CFD_E_ARG, "cfd_time_pass: which must be 0, 1, 2 or 3");
     if (which == 0 && s->cfg.order == 1) return fail(CFD_E_ARG, "cfd_time_pass: no gradient pass at order 1");
     if (s->cfg.ref_mode) return fail(CFD_E_ARG, "cfd_time_pass: native mode only");
     hipStream_t st = (hipStream_t)stream;
@@ -746,6 +748,10 @@ int cfd_time_pass(cfd_solver_t *s, int which, int reps, float *ms, void *stream)
     for (int r = 0; r < reps; r++) {
         if (which == 0) HIP_TRY(launch_grad(dm, s->dcfg, c.gradient, s->Un, s->grad, c.block, st));
         else if (which == 1) HIP_TRY(launch_flux(dm, s->dcfg, c.flux, c.order, c.limiter, MODE_STAGE, s->Un, s->grad, a, c.block, st));
+        else if (which == 3) {
+            const StageArgs r = resid_args(s, false, nullptr);
+            HIP_TRY(launch_flux(dm, s->dcfg, c.flux, c.order, c.limiter, MODE_RESID, s->Un, s->grad, r, c.block, st));
+        }
         else {
             int rc = native_step(s, false, st);
             if (rc) return rc;

@@ -535,8 +535,9 @@ __global__ void __launch_bounds__(256) CFD_FLUX_ATTR flux_tile_kernel(DevMesh m,
                 }
                 const double dr = fmax(1e-3 * fabs(rhi), 1e-3 * (rhi - rlo));
                 const double dp = fmax(1e-3 * fabs(phi), 1e-3 * (phi - plo));
-                const bool bad = !(rn > 0.0) || !(pn > 0.0) || !(rn >= rlo - dr && rn <= rhi + dr) ||
-                                 !(pn >= plo - dp && pn <= phi + dp);
+                const bool bad = !(rn > 0.0) || !(pn > 0.0) ||
+                                 (st.pos_range && (!(rn >= rlo - dr && rn <= rhi + dr) ||
+                                                   !(pn >= plo - dp && pn <= phi + dp)));
                 st.pflag[c] = bad ? 1 : 0;
                 if (bad) {
                     // this tile and the tiles of the cell's face neighbours recompute the stage

@@ -35,6 +35,7 @@ struct StageArgs {
     // first-order states on the faces of flagged cells) and clears their marks.
     uint8_t *pflag;
     int *dirty;
+    int pos_range;  // 1: the admissibility check includes the neighbourhood range test
 };
 
 size_t grad_smem_bytes(const DevMesh &m, int grad_type);

@@ -238,9 +238,10 @@ class Solver:
         return None
 
     def time_pass(self, which: str, reps: int, stream: int = 0) -> float:
-        """Milliseconds for `reps` launches of 'grad', 'flux' or 'step' (HIP events)."""
+        """Milliseconds for `reps` launches of 'grad', 'flux' (fused stage), 'step' or 'resid' (the
+        flux pass in residual mode) (HIP events)."""
         ms = ctypes.c_float()
-        check(lib().cfd_time_pass(self.h, dict(grad=0, flux=1, step=2)[which], reps,
+        check(lib().cfd_time_pass(self.h, dict(grad=0, flux=1, step=2, resid=3)[which], reps,
                                   ctypes.byref(ms), ctypes.c_void_p(stream)), "cfd_time_pass")
         return float(ms.value)
 

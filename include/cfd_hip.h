@@ -78,8 +78,9 @@ typedef struct {
      * pressure outside 1e-3 max(|hi|, hi - lo) of the range [lo, hi] of its own and its
      * face neighbours' stage inputs, has the stage recomputed with first-order states on all its
      * faces (an a-posteriori, MOOD-type check; no counterpart in the reference, whose second-order
-     * schemes have no safeguard). With it on, the device buffer of the state alternates between
-     * two arrays from step to step (query cfd_state_device after each step). */
+     * schemes have no safeguard). 2 applies the positivity test only, without the range test. With
+     * it on, the device buffer of the state alternates between two arrays from step to step (query
+     * cfd_state_device after each step). */
     int positivity;
 } cfd_config_t;
 
@@ -169,7 +170,8 @@ int cfd_run_steady(cfd_solver_t *s, double tol, int max_steps, int check_every, 
 
 /* Times `reps` back-to-back launches on `stream` with HIP events: which = 0 the gradient pass,
  * 1 the fused flux/stage pass (first stage, output to scratch; the state is unchanged),
- * 2 whole steps (advances the state). *ms = total elapsed milliseconds. */
+ * 2 whole steps (advances the state), 3 the flux pass in residual mode (R and the local time
+ * step out, as cfd_residual). *ms = total elapsed milliseconds. */
 int cfd_time_pass(cfd_solver_t *s, int which, int reps, float *ms, void *stream);
 
 /* ---- batched steady solves ------------------------------------------------------------------

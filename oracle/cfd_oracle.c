@@ -42,6 +42,9 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 /* Constants formed as the reference forms them (include/definitions.h:62-70). */
 #define GAMMA 1.4
@@ -876,6 +879,7 @@ void on_gradients(const OMesh *m, const ONatCfg *cfg, const double *W5, double *
     if (cfg->gradient == OGRAD_GG) {
         /* Calculate_Gradient_At_Cell_Center (src/Grid_Refine_Functions.cpp:221-246) for each of
          * (rho,u,v,p); ghost faces use the cell's own value */
+#pragma omp parallel for schedule(static)
         for (int c = 0; c < nc; c++) {
             for (int k = 0; k < 4; k++) {
                 double gx = 0.0, gy = 0.0;
@@ -895,6 +899,7 @@ void on_gradients(const OMesh *m, const ONatCfg *cfg, const double *W5, double *
     }
     /* weighted least squares, weights 1/|d|^2, ghost neighbours at the reflected centre with the
      * boundary-condition ghost state */
+#pragma omp parallel for schedule(static)
     for (int c = 0; c < nc; c++) {
         double a = 0.0, b = 0.0, d = 0.0;
         int n = m->nf[c];
@@ -1140,15 +1145,18 @@ static void residual_fo(const OMesh *m, const ONatCfg *cfg, const double *U, con
                         double *R, double *dt, double *grad, const unsigned char *fo) {
     const int nc = m->nc;
     FState *S = (FState *)malloc(sizeof(FState) * nc);
+#pragma omp parallel for schedule(static)
     for (int c = 0; c < nc; c++) state_from_U(&U[4 * c], &S[c]);
     double *W5 = NULL;
     if (cfg->order >= 2) {
         W5 = (double *)malloc(sizeof(double) * 5 * nc);
+#pragma omp parallel for schedule(static)
         for (int c = 0; c < nc; c++)
             for (int k = 0; k < 5; k++) W5[5 * c + k] = S[c].W[k];
         on_gradients(m, cfg, W5, grad);
     }
     double *Ff = (double *)malloc(sizeof(double) * 4 * m->nfaces);
+#pragma omp parallel for schedule(static)
     for (int fi = 0; fi < m->nfaces; fi++) {
         int L = m->fL[fi], Rc = m->fR[fi], s = m->fslot[fi];
         double nx = m->nrm[8 * L + 2 * s], ny = m->nrm[8 * L + 2 * s + 1], dl = m->len[4 * L + s];
@@ -1208,6 +1216,7 @@ static void residual_fo(const OMesh *m, const ONatCfg *cfg, const double *U, con
             face_flux(cfg, boundary, &SL, &SR, nx, ny, dl, &Ff[4 * fi]);
         }
     }
+#pragma omp parallel for schedule(static)
     for (int c = 0; c < nc; c++) {
         double r[4] = {0.0, 0.0, 0.0, 0.0};
         for (int f = 0; f < m->nf[c]; f++) {
@@ -1297,15 +1306,26 @@ static void rho_p_of(const double *U, double *rho, double *p) {
 }
 
 static int g_fallback_cells = 0;
+
+/* Threads the parallel loops of the residual use (OpenMP; 1 when built without it). The loops are
+ * over independent faces and cells, so the results do not depend on the thread count. */
+int or_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
 int on_last_fallback_cells(void) { return g_fallback_cells; }
 
 /* Positivity fallback, detection (no counterpart in the reference; an a-posteriori "MOOD"
  * check): flag[c] = 1 when the stage state Unew of cell c has a non-positive (or non-finite)
  * density or pressure, or a density or pressure outside [lo - d, hi + d], where lo and hi are the
  * minimum and maximum of the stage input Uin over the cell and its physical face neighbours and
- * d = 1e-3 max(|hi|, hi - lo). Returns the number of flagged cells. */
+ * d = 1e-3 max(|hi|, hi - lo) (the range test, when `range` is set). Returns the number of flagged
+ * cells. */
 static int stage_fallback_flags(const OMesh *m, const double *Uin, const double *Unew,
-                                unsigned char *flag) {
+                                unsigned char *flag, int range) {
     const int nc = m->nc;
     int nb = 0;
     for (int c = 0; c < nc; c++) {
@@ -1325,8 +1345,8 @@ static int stage_fallback_flags(const OMesh *m, const double *Uin, const double 
         }
         const double dr = fmax(1e-3 * fabs(rhi), 1e-3 * (rhi - rlo));
         const double dp = fmax(1e-3 * fabs(phi), 1e-3 * (phi - plo));
-        const int bad = !(rn > 0.0) || !(pn > 0.0) || !(rn >= rlo - dr && rn <= rhi + dr) ||
-                        !(pn >= plo - dp && pn <= phi + dp);
+        const int bad = !(rn > 0.0) || !(pn > 0.0) ||
+                        (range && (!(rn >= rlo - dr && rn <= rhi + dr) || !(pn >= plo - dp && pn <= phi + dp)));
         flag[c] = (unsigned char)bad;
         nb += bad;
     }
@@ -1366,7 +1386,7 @@ void on_step(const OMesh *m, const ONatCfg *cfg, int rk, int lts, double *U, dou
             memcpy(acc_in, acc, sizeof(double) * n4);
         }
         or_rk_stage(rk == 4 ? 4 : 3, s, nc, Un, Us, acc, R, h, out);
-        if (pos && stage_fallback_flags(m, Uin, out, flag)) {
+        if (pos && stage_fallback_flags(m, Uin, out, flag, cfg->positivity == 1)) {
             /* recompute the stage with first-order states on every face of a flagged cell */
             residual_fo(m, cfg, Uin, NULL, R, dt, grad, flag);
             memcpy(acc, acc_in, sizeof(double) * n4);

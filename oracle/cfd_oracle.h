@@ -75,7 +75,8 @@ typedef struct {
     int positivity;  /* a-posteriori first-order fallback of the native MUSCL stages (order 2): a
                         cell whose stage state has a non-positive density or pressure, or a density
                         or pressure outside the relaxed range of its own and its face neighbours'
-                        stage-input values, is recomputed with first-order states on all its faces */
+                        stage-input values, is recomputed with first-order states on all its faces;
+                        2: the positivity test alone (no range test) */
 } ONatCfg;
 
 /* ---- primitive / ghost helpers --------------------------------------------------------- */
@@ -97,6 +98,8 @@ void on_residual(const OMesh *m, const ONatCfg *cfg, const double *U, const doub
                  double *R, double *dt, double *grad);
 /* cells recomputed at first order by the positivity fallback during the last on_step */
 int on_last_fallback_cells(void);
+/* threads of the residual's parallel loops (OMP_NUM_THREADS; all host threads by default) */
+int or_threads(void);
 void on_gradients(const OMesh *m, const ONatCfg *cfg, const double *W5, double *grad);
 void on_face_flux(const ONatCfg *cfg, int boundary, const double *WL5, const double *WR5,
                   double nx, double ny, double dl, double *F4);

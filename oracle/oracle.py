@@ -80,7 +80,7 @@ def build_oracle(force: bool = False) -> str:
     newest = max(os.path.getmtime(f) for f in srcs + [os.path.join(HERE, "cfd_oracle.h")])
     if force or not os.path.exists(ORACLE_SO) or os.path.getmtime(ORACLE_SO) < newest:
         subprocess.check_call(["gcc", "-std=c11", "-O2", "-ffp-contract=off", "-fPIC", "-shared",
-                               "-Wall", "-o", ORACLE_SO] + srcs + ["-lm"])
+                               "-fopenmp", "-Wall", "-o", ORACLE_SO] + srcs + ["-lm"])
     return ORACLE_SO
 
 
@@ -150,6 +150,8 @@ class Oracle:
         L.on_refstep.argtypes = [ctypes.c_void_p, cfgp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]
         L.on_last_fallback_cells.restype = ctypes.c_int
         L.on_last_fallback_cells.argtypes = []
+        L.or_threads.restype = ctypes.c_int
+        L.or_threads.argtypes = []
         L.or_error.argtypes = [ctypes.c_int, _dp, _dp, _dp]
         L.or_rk_stage.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
         L.or_gg_gradient.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, _dp]
@@ -172,6 +174,10 @@ class Oracle:
     # ---- mesh ----
     def mesh(self, points: np.ndarray, cells: np.ndarray) -> "OracleMesh":
         return OracleMesh(self, points, cells)
+
+    def threads(self) -> int:
+        """Threads of the residual's parallel loops (OMP_NUM_THREADS, else all host threads)."""
+        return int(self.lib.or_threads())
 
     def last_fallback_cells(self) -> int:
         """Cells the positivity fallback recomputed at first order during the last step."""

@@ -40,24 +40,25 @@ def blast_box():
     return m
 
 
-@pytest.mark.parametrize("scheme,rk,lts,nsteps", [
-    (dict(flux="roe", limiter="minmod", gradient="gg"), 4, 0, 40),
-    (dict(flux="hllc", limiter="vanleer", gradient="wlsq"), 3, 1, 30),
-    (dict(flux="ausm", limiter="superbee", gradient="wlsq"), 3, 0, 30),
+@pytest.mark.parametrize("scheme,rk,lts,nsteps,mode", [
+    (dict(flux="roe", limiter="minmod", gradient="gg"), 4, 0, 40, 1),
+    (dict(flux="hllc", limiter="vanleer", gradient="wlsq"), 3, 1, 30, 1),
+    (dict(flux="ausm", limiter="superbee", gradient="wlsq"), 3, 0, 30, 1),
+    (dict(flux="roe", limiter="minmod", gradient="gg"), 4, 0, 40, 2),
 ])
-def test_step_mesh_with_fallback_matches_oracle(orc, scheme, rk, lts, nsteps):
-    print(f"case positivity step mesh {scheme} rk{rk} lts{lts}", flush=True)
+def test_step_mesh_with_fallback_matches_oracle(orc, scheme, rk, lts, nsteps, mode):
+    print(f"case positivity {mode} step mesh {scheme} rk{rk} lts{lts}", flush=True)
     m, fs, U0 = step_mesh()
     kw = dict(order=2, roe_bdiss=1, cfl=0.4, bc=STEP_BCS, fs=fs, **scheme)
     om = orc.mesh(m.points, m.cells)
     s = Solver(Mesh(m.points, m.cells, tile=128, face_budget=256),
-               make_config(rk=rk, lts=lts, block=128, positivity=1, **kw))
+               make_config(rk=rk, lts=lts, block=128, positivity=mode, **kw))
     s.set_state(U0)
     Uo = U0.copy()
     flagged = 0
     errs = []
     for n in range(nsteps):
-        e, _ = om.step(make_cfg(positivity=1, **kw), Uo, rk=rk, lts=lts)
+        e, _ = om.step(make_cfg(positivity=mode, **kw), Uo, rk=rk, lts=lts)
         errs.append(e)
         flagged += orc.last_fallback_cells()
     err, _ = s.step(nsteps, history=True)
@@ -118,7 +119,7 @@ def test_fallback_configuration_is_checked():
     m, fs, U0 = step_mesh()
     mesh = Mesh(m.points, m.cells, tile=128, face_budget=256)
     for bad in (dict(order=1), dict(order=3), dict(ref_mode=1, rk=3),
-                dict(viscous="constant", mu=1e-3)):
+                dict(viscous="constant", mu=1e-3), dict(positivity=3)):
         kw = dict(dict(flux="roe", order=2, limiter="minmod", gradient="gg", cfl=0.4, bc=STEP_BCS, fs=fs,
                        rk=4, lts=0, block=128, positivity=1), **bad)
         with pytest.raises(Exception):
