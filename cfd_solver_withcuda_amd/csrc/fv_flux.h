@@ -344,7 +344,10 @@ __device__ __forceinline__ void flux_face(const DevMesh &m, const DevCfg &cfg,
     for (int q = 0; q < 4; q++) sF[q * FT + j] = F[q];
 }
 
-template <int FLUX, int ORDER, int LIM, int MODE, bool VISC, int POS = 0>
+// FZ: the stage reads st.freeze and can record or use frozen limiter factors (launched only when a
+// freezing state is set; without it the kernel holds none of that code). Exit ghosts from
+// st.Uexit (the reference iteration's stages) exist in residual mode only.
+template <int FLUX, int ORDER, int LIM, int MODE, bool VISC, int POS = 0, bool FZ = false>
 __global__ void __launch_bounds__(256) CFD_FLUX_ATTR flux_tile_kernel(DevMesh m, DevCfg cfg,
                                                         const double4 *__restrict__ Us,
                                                         const double *__restrict__ grad,
@@ -364,7 +367,8 @@ __global__ void __launch_bounds__(256) CFD_FLUX_ATTR flux_tile_kernel(DevMesh m,
     const int nct = m.tile_cstart[tile + 1] - c0;
     const int fs = m.tile_fstart[tile], nft = m.tile_fstart[tile + 1] - fs;
     const bool glds = (ORDER >= 2) && cfg.grad_lds;
-    const int fz = (ORDER == 2 && st.freeze) ? st.freeze[m.tile_inst ? m.tile_inst[tile] : 0] : 0;
+    const int fz = (FZ && ORDER == 2 && st.freeze) ? st.freeze[m.tile_inst ? m.tile_inst[tile] : 0] : 0;
+    const double4 *Uexit = (MODE == MODE_RESID) ? st.Uexit : nullptr;
     double *sW = smem;                          // [5][T] primitives (rho, u, v, p, a)
     double *sU = sW + 5 * T;                    // [4][T] conserved (first order and WENO)
     double *sF = sU + (ORDER != 2 ? 4 * T : 0); // [4][FT] face fluxes
@@ -426,17 +430,17 @@ __global__ void __launch_bounds__(256) CFD_FLUX_ATTR flux_tile_kernel(DevMesh m,
     // ---- B: one flux per face ----------------------------------------------------------
     if (h0) {
         stageN(k, f0);
-        flux_face<FLUX, ORDER, LIM, VISC, POS>(m, cfg, Us, grad, st.Uexit, sW, sU, sG, sF, glds, T, FT, c0, nct, k, f0, fz, st.psi, fs + k, fsv, st.pflag, sNb);
+        flux_face<FLUX, ORDER, LIM, VISC, POS>(m, cfg, Us, grad, Uexit, sW, sU, sG, sF, glds, T, FT, c0, nct, k, f0, fz, st.psi, fs + k, fsv, st.pflag, sNb);
     }
     if (h1) {
         stageN(k + B, f1);
-        flux_face<FLUX, ORDER, LIM, VISC, POS>(m, cfg, Us, grad, st.Uexit, sW, sU, sG, sF, glds, T, FT, c0, nct, k + B, f1, fz, st.psi, fs + k + B, fsv, st.pflag, sNb);
+        flux_face<FLUX, ORDER, LIM, VISC, POS>(m, cfg, Us, grad, Uexit, sW, sU, sG, sF, glds, T, FT, c0, nct, k + B, f1, fz, st.psi, fs + k + B, fsv, st.pflag, sNb);
     }
     for (int j = k + 2 * B; j < nft; j += B) {
         FaceRec f;
         load_face<true, ORDER >= 2>(m, fs + j, f);
         stageN(j, f);
-        flux_face<FLUX, ORDER, LIM, VISC, POS>(m, cfg, Us, grad, st.Uexit, sW, sU, sG, sF, glds, T, FT, c0, nct, j, f, fz, st.psi, fs + j, fsv, st.pflag, sNb);
+        flux_face<FLUX, ORDER, LIM, VISC, POS>(m, cfg, Us, grad, Uexit, sW, sU, sG, sF, glds, T, FT, c0, nct, j, f, fz, st.psi, fs + j, fsv, st.pflag, sNb);
     }
     __syncthreads();
 
@@ -606,12 +610,19 @@ static inline int grid_of(const DevMesh &m, const DevCfg &c) {
     return c.xcd_remap ? 8 * ((m.ntiles + 7) / 8) : m.ntiles;
 }
 
+// Order-2 stages with limiter freezing (fv_freeze.hip): the FZ variants of the stage kernel.
+hipError_t launch_flux_frozen(const DevMesh &m, const DevCfg &c, int flux, int lim, bool visc,
+                              const double4 *Us, const double *grad, const StageArgs &st, int block,
+                              size_t sm, hipStream_t s);
+
 template <int FLUX, int ORDER, int LIM, bool VISC>
 static void launch_flux_t(const DevMesh &m, const DevCfg &c, int mode, const double4 *Us,
                           const double *grad, const StageArgs &st, int block, size_t sm,
                           hipStream_t s) {
     if (mode == MODE_RESID)
         hipLaunchKernelGGL((flux_tile_kernel<FLUX, ORDER, LIM, MODE_RESID, VISC>), dim3(grid_of(m, c)), dim3(block), sm, s, m, c, Us, grad, st);
+    else if (ORDER == 2 && st.freeze)
+        (void)launch_flux_frozen(m, c, FLUX, LIM, VISC, Us, grad, st, block, sm, s);
     else
         hipLaunchKernelGGL((flux_tile_kernel<FLUX, ORDER, LIM, MODE_STAGE, VISC>), dim3(grid_of(m, c)), dim3(block), sm, s, m, c, Us, grad, st);
 }

@@ -12,7 +12,7 @@ namespace {
 template <int FLUX, int LIM, int PASS>
 void launch_pos_t(const DevMesh &m, const DevCfg &c, const double4 *Us, const double *grad,
                   const StageArgs &st, int block, size_t sm, hipStream_t s) {
-    hipLaunchKernelGGL((flux_tile_kernel<FLUX, 2, LIM, MODE_STAGE, false, PASS>), dim3(grid_of(m, c)),
+    hipLaunchKernelGGL((flux_tile_kernel<FLUX, 2, LIM, MODE_STAGE, false, PASS, true>), dim3(grid_of(m, c)),
                        dim3(block), sm, s, m, c, Us, grad, st);
 }
 
