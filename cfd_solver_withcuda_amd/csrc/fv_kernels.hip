@@ -575,10 +575,21 @@ hipError_t launch_grad(const DevMesh &m, const DevCfg &c, int grad_type, const d
     return hipGetLastError();
 }
 
+// LDS of one flux launch: the face-geometry copy (3 doubles per face) only when the launch forms
+// time steps (the kernel's needN), at least the block reductions' 4 doubles per thread.
+static size_t flux_launch_smem(const DevMesh &m, int order, int grad_lds, int block, bool geom) {
+    size_t d = 5 * (size_t)m.tile + (order != 2 ? 4 * (size_t)m.tile : 0) +
+               (geom ? 7 : 4) * (size_t)m.max_tile_faces;
+    if (order >= 2 && grad_lds) d += 8 * (size_t)m.tile;
+    if (d < 4 * (size_t)block) d = 4 * (size_t)block;
+    return sizeof(double) * d;
+}
+
 hipError_t launch_flux(const DevMesh &m, const DevCfg &c, int flux, int order, int lim, int mode,
                        const double4 *Us, const double *grad, const StageArgs &st, int block,
                        hipStream_t s) {
-    const size_t sm = flux_smem_bytes(m, order, c.grad_lds);
+    const bool geom = (mode == MODE_RESID) || (st.lts && st.stage == 0) || (st.last && st.dt_partial);
+    const size_t sm = flux_launch_smem(m, order, c.grad_lds, block, geom);
     if (order == 3)
         return c.viscous ? launch_flux_weno_visc(m, c, flux, lim, mode, Us, grad, st, block, sm, s)
                          : launch_flux_weno(m, c, flux, lim, mode, Us, grad, st, block, sm, s);
