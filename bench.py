@@ -26,7 +26,8 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 # rocprofv3 summaries of the flux kernel per workload, with calibrated FETCH_SIZE + WRITE_SIZE
-PROFILES = {("bump", "muscl"): "profiles/flux_tile_kernel_bump1400x700.md",
+PROFILES = {("ffs_hllc", "muscl"): "profiles/flux_tile_kernel_ffs4680x1560_hllc.md",
+            ("bump", "muscl"): "profiles/flux_tile_kernel_bump1400x700.md",
             ("ffs", "muscl"): "profiles/flux_tile_kernel_ffs4680x1560.md",
             ("bump", "weno"): "profiles/flux_tile_kernel_bump1400x700_weno5.md",
             ("bump", "viscous"): "profiles/flux_tile_kernel_bump1400x700_viscous.md"}
@@ -470,10 +471,17 @@ def main():
         a = torch.empty(64 * 1024 * 1024, dtype=torch.float64, device="cuda")
         b = torch.empty_like(a)
         a.fill_(1.0)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        b.copy_(a)
+        e0.record()
         for _ in range(5):
             b.copy_(a)
+        e1.record()
         torch.cuda.synchronize()
-        print(json.dumps({"copy_calibration_bytes_read": a.numel() * 8, "copies": 5}))
+        ms = e0.elapsed_time(e1) / 5
+        # the practical HBM ceiling of a streaming kernel: 512 MiB read + 512 MiB written per copy
+        print(json.dumps({"copy_calibration_bytes_read": a.numel() * 8, "copies": 6,
+                          "copy_ms": ms, "copy_GBs": 2 * a.numel() * 8 / (ms / 1e3) / 1e9}))
         return 0
 
     from cfd_solver_withcuda_amd.solver import Mesh, Solver, cons_from_prim, make_config

@@ -155,91 +155,97 @@ __device__ __forceinline__ void ghost_state(int bc, const FState &in, double n1,
     state_from_U(Ug[0], Ug[1], Ug[2], Ug[3], g);
 }
 
-// Calculate_Face_Average_Flux (src/Average_Interface_Flux.cpp:20-82).
+// Calculate_Face_Average_Flux (src/Average_Interface_Flux.cpp:20-82): the mean of the two sides'
+// Euler fluxes through the face, times its length (same operations in the same order).
 __device__ __forceinline__ void avg_flux(const FState &L, const FState &R, double nx, double ny,
                                          double dl, double *A) {
-    double Rho_L = L.W[0], u_L = L.W[1], v_L = L.W[2], P_L = L.W[3];
-    double Rho_R = R.W[0], u_R = R.W[1], v_R = R.W[2], P_R = R.W[3];
-    double Vdotn_L = (u_L * nx + v_L * ny);
-    double Vdotn_R = (u_R * nx + v_R * ny);
-    double Vmag_L = 0.5 * (u_L * u_L + v_L * v_L);
-    double Vmag_R = 0.5 * (u_R * u_R + v_R * v_R);
-    double Vdotn_L_dl = Vdotn_L * dl;
-    double Vdotn_R_dl = Vdotn_R * dl;
-    A[0] = 0.5 * (Rho_L * Vdotn_L_dl + Rho_R * Vdotn_R_dl);
-    A[1] = 0.5 * ((Rho_L * u_L * Vdotn_L_dl + P_L * nx * dl) + (Rho_R * u_R * Vdotn_R_dl + P_R * nx * dl));
-    A[2] = 0.5 * ((Rho_L * v_L * Vdotn_L_dl + P_L * ny * dl) + (Rho_R * v_R * Vdotn_R_dl + P_R * ny * dl));
-    A[3] = 0.5 * (((kGamma1 * P_L + Rho_L * Vmag_L) * Vdotn_L_dl) + ((kGamma1 * P_R + Rho_R * Vmag_R) * Vdotn_R_dl));
+    const double rl = L.W[0], ul = L.W[1], vl = L.W[2], pl = L.W[3];
+    const double rr = R.W[0], ur = R.W[1], vr = R.W[2], pr = R.W[3];
+    const double qnl = (ul * nx + vl * ny);
+    const double qnr = (ur * nx + vr * ny);
+    const double kel = 0.5 * (ul * ul + vl * vl);
+    const double ker = 0.5 * (ur * ur + vr * vr);
+    const double qnl_dl = qnl * dl;
+    const double qnr_dl = qnr * dl;
+    A[0] = 0.5 * (rl * qnl_dl + rr * qnr_dl);
+    A[1] = 0.5 * ((rl * ul * qnl_dl + pl * nx * dl) + (rr * ur * qnr_dl + pr * nx * dl));
+    A[2] = 0.5 * ((rl * vl * qnl_dl + pl * ny * dl) + (rr * vr * qnr_dl + pr * ny * dl));
+    A[3] = 0.5 * (((kGamma1 * pl + rl * kel) * qnl_dl) + ((kGamma1 * pr + rr * ker) * qnr_dl));
 }
 
-// ROE dissipation (src/Roe_Scheme.cpp:6-256), Harten entropy fix delta = 0.1 a~.
+// ROE dissipation (src/Roe_Scheme.cpp:6-256), Harten entropy fix delta = 0.1 a~:
+// D = 1/2 sum_k |lambda_k| alpha_k r_k dl over the four waves of the Roe-averaged state (same
+// operations in the same order as the reference, with this file's own names).
 __device__ __forceinline__ void roe_diss(const FState &L, const FState &R, double nx, double ny,
                                          double dl, double *D) {
     D[0] = D[1] = D[2] = D[3] = 0.0;
-    double normal_magnitude = sqrt(nx * nx + ny * ny);
-    if (normal_magnitude < 1e-12) return;
-    double Rho_L = L.W[0], u_L = L.W[1], v_L = L.W[2], P_L = L.W[3], C_L = L.W[4];
-    if (Rho_L <= 0.0 || P_L <= 0.0 || C_L <= 0.0) return;
-    double Rho_R = R.W[0], u_R = R.W[1], v_R = R.W[2], P_R = R.W[3], C_R = R.W[4];
-    if (Rho_R <= 0.0 || P_R <= 0.0 || C_R <= 0.0) return;
-    double Vmag_L = 0.5 * (u_L * u_L + v_L * v_L);
-    double H_L = ((C_L * C_L) / (kGamma - 1.0)) + Vmag_L;
-    double Vmag_R = 0.5 * (u_R * u_R + v_R * v_R);
-    double H_R = ((C_R * C_R) / (kGamma - 1.0)) + Vmag_R;
-    double sqrt_Rho_L = sqrt(Rho_L);
-    double sqrt_Rho_R = sqrt(Rho_R);
-    double Term1 = 1.0 / (sqrt_Rho_L + sqrt_Rho_R);
-    double Roe_Rho = sqrt(Rho_L * Rho_R);
-    double Roe_u = Term1 * (u_R * sqrt_Rho_R + u_L * sqrt_Rho_L);
-    double Roe_v = Term1 * (v_R * sqrt_Rho_R + v_L * sqrt_Rho_L);
-    double Roe_H = Term1 * (H_R * sqrt_Rho_R + H_L * sqrt_Rho_L);
-    double Roe_Vmag = 0.5 * (Roe_u * Roe_u + Roe_v * Roe_v);
-    double Roe_a = sqrt((kGamma - 1.0) * (Roe_H - Roe_Vmag));
-    if (Roe_a <= 0.0) {
-        Roe_a = 0.5 * (C_L + C_R);
-        if (Roe_a <= 0.0) return;
+    const double nmag = sqrt(nx * nx + ny * ny);
+    if (nmag < 1e-12) return;
+    const double rl = L.W[0], ul = L.W[1], vl = L.W[2], pl = L.W[3], cl = L.W[4];
+    if (rl <= 0.0 || pl <= 0.0 || cl <= 0.0) return;
+    const double rr = R.W[0], ur = R.W[1], vr = R.W[2], pr = R.W[3], cr = R.W[4];
+    if (rr <= 0.0 || pr <= 0.0 || cr <= 0.0) return;
+    const double kel = 0.5 * (ul * ul + vl * vl);
+    const double hl = ((cl * cl) / (kGamma - 1.0)) + kel;
+    const double ker = 0.5 * (ur * ur + vr * vr);
+    const double hr = ((cr * cr) / (kGamma - 1.0)) + ker;
+    const double wl = sqrt(rl);  // sqrt(rho) weights of the Roe average
+    const double wr = sqrt(rr);
+    const double inv_w = 1.0 / (wl + wr);
+    const double rho_t = sqrt(rl * rr);
+    const double u_t = inv_w * (ur * wr + ul * wl);
+    const double v_t = inv_w * (vr * wr + vl * wl);
+    const double h_t = inv_w * (hr * wr + hl * wl);
+    const double ke_t = 0.5 * (u_t * u_t + v_t * v_t);
+    double c_t = sqrt((kGamma - 1.0) * (h_t - ke_t));
+    if (c_t <= 0.0) {
+        c_t = 0.5 * (cl + cr);
+        if (c_t <= 0.0) return;
     }
-    double Un = nx * Roe_u + ny * Roe_v;
-    double Ut = -ny * Roe_u + nx * Roe_v;
-    double du = (u_R - u_L), dv = (v_R - v_L), dP = (P_R - P_L), dRho = (Rho_R - Rho_L);
-    double dUn = nx * du + ny * dv;
-    double dUt = -ny * du + nx * dv;
-    double R12 = Roe_u - nx * Roe_a, R13 = Roe_v - ny * Roe_a, R14 = Roe_H - Roe_a * Un;
-    double R22 = Roe_u, R23 = Roe_v, R24 = Roe_Vmag;
-    double R32 = -ny, R33 = nx, R34 = Ut;
-    double R42 = Roe_u + nx * Roe_a, R43 = Roe_v + ny * Roe_a, R44 = Roe_H + Roe_a * Un;
-    double Term2 = 1.0 / (Roe_a * Roe_a);
-    double a1 = 0.5 * Term2 * (dP - Roe_Rho * Roe_a * dUn);
-    double a2 = dRho - Term2 * dP;
-    double a3 = Roe_Rho * dUt;
-    double a4 = 0.5 * Term2 * (dP + Roe_Rho * Roe_a * dUn);
-    double L1 = fabs(Un - Roe_a), L2 = fabs(Un), L3 = fabs(Un), L4 = fabs(Un + Roe_a);
-    double ef = 0.1 * Roe_a;
-    if (L1 < ef) L1 = 0.5 * (L1 * L1 / ef + ef);
-    if (L4 < ef) L4 = 0.5 * (L4 * L4 / ef + ef);
-    double t1 = L1 * a1, t2 = L2 * a2, t3 = L3 * a3, t4 = L4 * a4;
+    const double qn = nx * u_t + ny * v_t;   // normal and tangential Roe velocities
+    const double qt = -ny * u_t + nx * v_t;
+    const double du = (ur - ul), dv = (vr - vl), dp = (pr - pl), drho = (rr - rl);
+    const double dqn = nx * du + ny * dv;
+    const double dqt = -ny * du + nx * dv;
+    // right eigenvectors (mass component 1, 1, 0, 1): acoustic -, entropy, shear, acoustic +
+    const double e1u = u_t - nx * c_t, e1v = v_t - ny * c_t, e1e = h_t - c_t * qn;
+    const double e2u = u_t, e2v = v_t, e2e = ke_t;
+    const double e3u = -ny, e3v = nx, e3e = qt;
+    const double e4u = u_t + nx * c_t, e4v = v_t + ny * c_t, e4e = h_t + c_t * qn;
+    const double inv_c2 = 1.0 / (c_t * c_t);
+    const double s1 = 0.5 * inv_c2 * (dp - rho_t * c_t * dqn);  // wave strengths
+    const double s2 = drho - inv_c2 * dp;
+    const double s3 = rho_t * dqt;
+    const double s4 = 0.5 * inv_c2 * (dp + rho_t * c_t * dqn);
+    double l1 = fabs(qn - c_t), l2 = fabs(qn), l3 = fabs(qn), l4 = fabs(qn + c_t);
+    const double ef = 0.1 * c_t;
+    if (l1 < ef) l1 = 0.5 * (l1 * l1 / ef + ef);
+    if (l4 < ef) l4 = 0.5 * (l4 * l4 / ef + ef);
+    const double t1 = l1 * s1, t2 = l2 * s2, t3 = l3 * s3, t4 = l4 * s4;
     D[0] = 0.5 * (t1 * 1.0 + t2 * 1.0 + t3 * 0.0 + t4 * 1.0) * dl;
-    D[1] = 0.5 * (t1 * R12 + t2 * R22 + t3 * R32 + t4 * R42) * dl;
-    D[2] = 0.5 * (t1 * R13 + t2 * R23 + t3 * R33 + t4 * R43) * dl;
-    D[3] = 0.5 * (t1 * R14 + t2 * R24 + t3 * R34 + t4 * R44) * dl;
+    D[1] = 0.5 * (t1 * e1u + t2 * e2u + t3 * e3u + t4 * e4u) * dl;
+    D[2] = 0.5 * (t1 * e1v + t2 * e2v + t3 * e3v + t4 * e4v) * dl;
+    D[3] = 0.5 * (t1 * e1e + t2 * e2e + t3 * e3e + t4 * e4e) * dl;
 }
 
+// The larger of three values, first-listed winning ties (Maximum, src/Utility_Files.cpp:5-13).
 __device__ __forceinline__ double max3_ref(double a, double b, double c) {
-    if ((a >= b) && (a >= c)) return a;  // Maximum (src/Utility_Files.cpp:5-13)
+    if ((a >= b) && (a >= c)) return a;
     if ((b >= a) && (b >= c)) return b;
     return c;
 }
 
-// LLF dissipation (src/LLF.cpp:8-83).
+// LLF dissipation (src/LLF.cpp:8-83): 1/2 of the largest |qn -+ c|, |qn| over both sides, times
+// the jump of the conserved state.
 __device__ __forceinline__ void llf_diss(const FState &L, const FState &R, double nx, double ny,
                                          double dl, double *D) {
-    double Vdotn_L = (L.W[1] * nx + L.W[2] * ny);
-    double Vdotn_R = (R.W[1] * nx + R.W[2] * ny);
-    double C_L = L.W[4], C_R = R.W[4];
-    double Max1 = max3_ref(fabs(Vdotn_L - C_L) * dl, fabs(Vdotn_L + C_L) * dl, fabs(Vdotn_L) * dl);
-    double Max2 = max3_ref(fabs(Vdotn_R - C_R) * dl, fabs(Vdotn_R + C_R) * dl, fabs(Vdotn_R) * dl);
-    double mev = (Max1 >= Max2) ? Max1 : Max2;
-    for (int k = 0; k < 4; k++) D[k] = 0.5 * mev * (R.U[k] - L.U[k]);
+    const double qnl = (L.W[1] * nx + L.W[2] * ny);
+    const double qnr = (R.W[1] * nx + R.W[2] * ny);
+    const double cl = L.W[4], cr = R.W[4];
+    const double sl = max3_ref(fabs(qnl - cl) * dl, fabs(qnl + cl) * dl, fabs(qnl) * dl);
+    const double sr = max3_ref(fabs(qnr - cr) * dl, fabs(qnr + cr) * dl, fabs(qnr) * dl);
+    const double smax = (sl >= sr) ? sl : sr;
+    for (int k = 0; k < 4; k++) D[k] = 0.5 * smax * (R.U[k] - L.U[k]);
 }
 
 // HLLC (CUDA_KERNELS/HLLC_LLF_Flux_Cuda_Kernels.cu:25-208), times the face length.

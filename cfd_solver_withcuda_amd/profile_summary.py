@@ -65,6 +65,8 @@ def main(argv=None):
     p.add_argument("--ordinals", default="",
                    help="start:stop of the kernel's dispatches to average (default: all)")
     p.add_argument("--config", default="")
+    p.add_argument("--extra-kernel", action="append", default=[],
+                   help="NAME=ALGORITHMIC_BYTES: a further kernel's traffic (all its dispatches)")
     p.add_argument("--note", action="append", default=[])
     a = p.parse_args(argv)
 
@@ -95,6 +97,16 @@ def main(argv=None):
             out.append(f"- algorithmic bytes per launch {a.algorithmic_bytes / 1e6:.1f} MB; "
                        f"measured / algorithmic = {traffic / a.algorithmic_bytes:.3f}")
         out += ["", f"traffic_bytes_per_launch: {traffic:.0f}", f"traffic_config: {a.config}"]
+        for spec in a.extra_kernel:
+            name, _, alg = spec.rpartition("=")
+            f2, n2 = counter_per_dispatch(a.fetch, name, "FETCH_SIZE")
+            w2, _ = counter_per_dispatch(a.write, name, "WRITE_SIZE")
+            t2 = f2 * 1024 * kf + w2 * 1024 * kw
+            out += ["", f"## HBM traffic of `{name}`", "",
+                    f"- FETCH_SIZE {f2:.0f} KiB, WRITE_SIZE {w2:.0f} KiB per dispatch ({n2} dispatches) -> "
+                    f"{f2 * 1024 * kf / 1e6:.1f} MB read, {w2 * 1024 * kw / 1e6:.1f} MB written",
+                    f"- algorithmic bytes per launch {float(alg) / 1e6:.1f} MB; measured / algorithmic = "
+                    f"{t2 / float(alg):.3f}"]
     if a.note:
         out += [""] + [f"- {n}" for n in a.note]
     sys.stdout.write("\n".join(out) + "\n")
