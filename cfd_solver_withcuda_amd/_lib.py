@@ -31,7 +31,7 @@ class Config(ctypes.Structure):
                 ("cfl", _d), ("bc", _i * 6), ("fs", _d * 4), ("rk", _i), ("lts", _i),
                 ("ref_mode", _i), ("block", _i), ("xcd_remap", _i), ("grad_lds", _i),
                 ("freeze_tol", _d), ("freeze_stall", _i), ("zeta", _d), ("zeta1", _d),
-                ("viscous", _i), ("mu", _d), ("prandtl", _d), ("r_gas", _d), ("positivity", _i)]
+                ("viscous", _i), ("mu", _d), ("prandtl", _d), ("r_gas", _d), ("positivity", _i), ("p_exit", _d)]
 
 
 # cfd_progress_fn: void (*)(int steps_done, int marching, void *user)

@@ -293,7 +293,8 @@ static int check_config(const cfd_mesh_s *m, const cfd_config_t *c) {
     if (c->ref_mode && c->rk == 4) return fail(CFD_E_ARG, "config: the reference iteration has no RK4");
     if (c->block != 64 && c->block != 128 && c->block != 256) return fail(CFD_E_ARG, "config: block must be 64, 128 or 256");
     for (int k = 1; k <= 5; k++)
-        if (c->bc[k] < 1 || c->bc[k] > 8) return fail(CFD_E_ARG, "config: bc types must be 1..8");
+        if (c->bc[k] < 1 || c->bc[k] > 10) return fail(CFD_E_ARG, "config: bc types must be 1..10");
+    if (!(c->p_exit >= 0.0) || !std::isfinite(c->p_exit)) return fail(CFD_E_ARG, "config: p_exit must be finite and >= 0");
     if (int rc = capi_check_viscous(c, "config")) return rc;
     if (c->positivity < 0 || c->positivity > 2) return fail(CFD_E_ARG, "config: positivity must be 0, 1 or 2");
     if (c->positivity && (c->order != 2 || c->ref_mode || c->viscous))
@@ -808,6 +809,7 @@ void capi_fill_devcfg(DevCfg &d, const cfd_config_t &c) {
     d.roe_bdiss = c.roe_bdiss;
     d.xcd_remap = c.xcd_remap;
     d.grad_lds = c.grad_lds;
+    d.p_exit = c.p_exit;
     capi_fill_viscous(d, c);
 }
 }  // namespace cfd

@@ -132,7 +132,9 @@ int check_batch_config(const cfd_config_t *c) {
     if (!c->lts) return capi_fail(CFD_E_ARG, "batch config: batched steady solves use local time stepping (lts = 1)");
     if (c->positivity) return capi_fail(CFD_E_ARG, "batch config: the positivity fallback exists for single solvers only");
     for (int k = 1; k <= 5; k++)
-        if (c->bc[k] < 1 || c->bc[k] > 8) return capi_fail(CFD_E_ARG, "batch config: bc types must be 1..8");
+        if (c->bc[k] < 1 || c->bc[k] > 10) return capi_fail(CFD_E_ARG, "batch config: bc types must be 1..10");
+    if (!(c->p_exit >= 0.0) || !std::isfinite(c->p_exit))
+        return capi_fail(CFD_E_ARG, "batch config: p_exit must be finite and >= 0");
     return capi_check_viscous(c, "batch config");
 }
 
@@ -310,6 +312,7 @@ int build_batch(const std::vector<InstanceInput> &in, const double *fs4, const c
     b->dcfg.roe_bdiss = cfg.roe_bdiss;
     b->dcfg.xcd_remap = cfg.xcd_remap;
     b->dcfg.grad_lds = cfg.grad_lds;
+    b->dcfg.p_exit = cfg.p_exit;  // 0: each instance's prescribed p
     capi_fill_viscous(b->dcfg, cfg);
     const size_t sm = flux_smem_bytes(d, cfg.order, cfg.grad_lds);
     if (sm > 65536 || grad_smem_bytes(d, cfg.gradient) > 65536) {

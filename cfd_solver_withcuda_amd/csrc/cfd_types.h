@@ -14,7 +14,9 @@ enum BcType {
     BC_FREESTREAM = 5,          // all primitives prescribed
     BC_FARFIELD = 6,            // Riemann-invariant far field
     BC_SUBSONIC_INLET_REF = 7,  // the reference's subsonic inlet
-    BC_WALL_NOSLIP = 8          // no-slip adiabatic wall: velocity negated, rho and p copied
+    BC_WALL_NOSLIP = 8,         // no-slip adiabatic wall: velocity negated, rho and p copied
+    BC_SUBSONIC_EXIT = 9,       // back-pressure exit (characteristic, prescribed static pressure)
+    BC_SUBSONIC_INLET = 10      // total-condition inlet (total p, total rho, direction prescribed)
 };
 // Reference face kinds (src/Read_Gmsh_File.cpp:318-323).
 enum FaceKind { KIND_INTERNAL = 0, KIND_LEFT = 1, KIND_RIGHT = 2, KIND_TOP = 3, KIND_BOTTOM = 4, KIND_WALL = 5 };
@@ -69,6 +71,7 @@ struct DevCfg {
     double r_gas;    // gas constant: T = p / (rho r_gas)
     double kfac;     // conductivity / viscosity = cp / Pr = gamma r_gas / ((gamma - 1) Pr)
     double inv_pr;   // 1 / Pr
+    double p_exit;   // static pressure of back-pressure exits (0: the prescribed state's p)
 };
 
 }  // namespace cfd

@@ -47,7 +47,8 @@ __device__ __forceinline__ int block_tile(const DevMesh &m, int remap) {
 }
 
 // Prescribed (freestream / inlet) state of a tile: the configuration's, or on a batched mesh the
-// tile's instance's. Kept in a local array (the kernel parameters stay read-only).
+// tile's instance's, then the exits' static pressure (the configuration's, else that state's p).
+// Kept in a local array (the kernel parameters stay read-only).
 __device__ __forceinline__ void tile_fs(const DevMesh &m, const DevCfg &cfg, int tile, double *fsv) {
     if (m.inst_fs) {
         const double4 f = m.inst_fs[m.tile_inst[tile]];
@@ -58,6 +59,7 @@ __device__ __forceinline__ void tile_fs(const DevMesh &m, const DevCfg &cfg, int
     } else {
         for (int q = 0; q < 4; q++) fsv[q] = cfg.fs[q];
     }
+    fsv[4] = (cfg.p_exit > 0.0) ? cfg.p_exit : fsv[3];
 }
 
 __device__ __forceinline__ void load_state(const double4 *__restrict__ U, int c, FState &s) {
@@ -358,7 +360,7 @@ __global__ void __launch_bounds__(256) CFD_FLUX_ATTR flux_tile_kernel(DevMesh m,
     if constexpr (POS == 2) {
         if (!st.dirty[tile]) return;  // no face of this tile touches a flagged cell
     }
-    double fsv[4];
+    double fsv[5];
     tile_fs(m, cfg, tile, fsv);
     const int T = m.tile;
     const int FT = m.max_tile_faces;

@@ -76,7 +76,7 @@ __global__ void __launch_bounds__(256) grad_tile_kernel(DevMesh m, DevCfg cfg,
     extern __shared__ double smem[];
     const int tile = block_tile(m, cfg.xcd_remap);
     if (tile < 0) return;
-    double fsv[4];
+    double fsv[5];
     tile_fs(m, cfg, tile, fsv);
     const int T = m.tile;
     const int FT = m.max_tile_faces;
@@ -217,7 +217,8 @@ __global__ void __launch_bounds__(256) ghost_kernel(int n, const int *__restrict
     const int bc = cfg.bc[gkind[k]];
     FState in, g;
     load_state((bc == BC_EXIT && Uexit) ? Uexit : Us, c, in);
-    ghost_state(bc, in, nv.x, nv.y, cfg.fs, g);
+    const double fsv[5] = {cfg.fs[0], cfg.fs[1], cfg.fs[2], cfg.fs[3], (cfg.p_exit > 0.0) ? cfg.p_exit : cfg.fs[3]};
+    ghost_state(bc, in, nv.x, nv.y, fsv, g);
     Ug[k] = make_double4(g.U[0], g.U[1], g.U[2], g.U[3]);
 }
 

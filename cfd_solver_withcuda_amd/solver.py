@@ -21,7 +21,7 @@ FLUX = dict(roe=0, hllc=1, llf=2, ausm=3)
 LIMITER = dict(none=0, minmod=1, vanleer=2, superbee=3)
 GRADIENT = dict(gg=0, wlsq=1)
 BC = dict(inlet_ref=1, exit=2, wall=3, symmetry=4, freestream=5, farfield=6, subsonic_inlet_ref=7,
-          noslip_wall=8)
+          noslip_wall=8, subsonic_exit=9, subsonic_inlet=10)
 VISCOUS = dict(off=0, constant=1, sutherland=2)
 
 # ghost types of the reference's face kinds (LEFT->Inlet, RIGHT->Exit, rest->Wall,
@@ -32,7 +32,7 @@ REF_BCS = ("inlet_ref", "exit", "wall", "wall", "wall")
 def make_config(flux="roe", order=1, limiter="vanleer", gradient="gg", roe_bdiss=0, cfl=0.5,
                 bc=REF_BCS, fs=(1.0, 0.0, 0.0, 1.0), rk=3, lts=0, ref_mode=0, block=256,
                 xcd_remap=1, grad_lds=1, freeze_tol=0.0, freeze_stall=0, zeta=0.5, zeta1=0.5,
-                viscous="off", mu=0.0, prandtl=0.72, r_gas=287.0, positivity=0) -> Config:
+                viscous="off", mu=0.0, prandtl=0.72, r_gas=287.0, positivity=0, p_exit=0.0) -> Config:
     """Scheme settings (cfd_config_t). viscous: "off" (Euler), "constant" (viscosity mu) or
     "sutherland" (Sutherland's law in T = p / (rho r_gas)); laminar viscous terms need order 2 or 3."""
     c = Config()
@@ -64,6 +64,7 @@ def make_config(flux="roe", order=1, limiter="vanleer", gradient="gg", roe_bdiss
     c.prandtl = float(prandtl)
     c.r_gas = float(r_gas)
     c.positivity = int(positivity)
+    c.p_exit = float(p_exit)
     return c
 
 

@@ -47,7 +47,10 @@ typedef struct {
                       1 inlet (rho,P given, interior velocity), 2 exit (copy), 3 slip wall,
                       4 symmetry, 5 freestream, 6 characteristic far field,
                       7 the reference's subsonic inlet, 8 no-slip adiabatic wall (velocity
-                      negated, rho and P copied) */
+                      negated, rho and P copied), 9 back-pressure exit (characteristic, static
+                      pressure p_exit; supersonic outflow copies), 10 total-condition subsonic
+                      inlet (total pressure, total density and direction of fs; static pressure
+                      from the interior) */
     double fs[4];  /* prescribed primitive state (rho, u, v, p) */
     int rk;        /* 1 forward Euler (reference mode), 3 TVD-RK3, 4 classical RK4 */
     int lts;       /* local time stepping */
@@ -82,6 +85,8 @@ typedef struct {
      * it on, the device buffer of the state alternates between two arrays from step to step (query
      * cfd_state_device after each step). */
     int positivity;
+    /* Static pressure of back-pressure exits (ghost type 9); 0 = the prescribed state's p. */
+    double p_exit;
 } cfd_config_t;
 
 const char *cfd_last_error(void);

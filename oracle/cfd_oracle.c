@@ -468,6 +468,20 @@ static void cons_from_prim(double rho, double u, double v, double p, double *U) 
     U[3] = ((p / GM1) + rho * vmag);
 }
 
+/* Density and pressure of a conserved state, with the arithmetic of Calculate_Primitive_Variables
+ * (src/Basic_Functions.cpp:85-142). */
+static void rho_p_of(const double *U, double *rho, double *p) {
+    double Rho = U[0];
+    double inv = 1.0 / Rho;
+    double v1 = U[1] * inv;
+    double v2 = U[2] * inv;
+    if (fabs(v1) <= 1e-10) v1 = 0.0;
+    if (fabs(v2) <= 1e-10) v2 = 0.0;
+    double vmag = (v1 * v1 + v2 * v2);
+    *rho = Rho;
+    *p = GM1 * (U[3] - 0.5 * Rho * vmag);
+}
+
 /* Ghost state from the interior state across a boundary face with outward unit normal n. */
 static void ghost_fstate(int bc, const FState *in, double n1, double n2, const double *fs,
                          FState *g) {
@@ -501,6 +515,49 @@ static void ghost_fstate(int bc, const FState *in, double n1, double n2, const d
         Ug[1] = in->W[0] * (-u);
         Ug[2] = in->W[0] * (-v);
         Ug[3] = ((in->W[3] / GM1) + in->W[0] * vmag);
+        state_from_U(Ug, g);
+        return;
+    }
+    case OBC_SUBSONIC_EXIT: { /* back-pressure exit (Subsonic_Exit_Boundary_Condition,
+                               * src/Exit_Boundary_Conditions.cpp:8-66) with the interior cell's
+                               * density and sound speed as the reference state and the prescribed
+                               * pressure fs[4]; supersonic outflow copies the interior state */
+        const double r = in->W[0], u = in->W[1], v = in->W[2], p = in->W[3], c = in->W[4];
+        if (u * n1 + v * n2 >= c) {
+            *g = *in;
+            return;
+        }
+        const double pb = fs[4];
+        const double t = (p - pb) / (r * c * c);
+        const double rf = r - t * r;
+        const double uf = u + n1 * t * c;
+        const double vf = v + n2 * t * c;
+        double Uf[4], Uc[4];
+        cons_from_prim(rf, uf, vf, pb, Uf);
+        cons_from_prim(r, u, v, p, Uc);
+        for (int q = 0; q < 4; q++) Ug[q] = 2.0 * Uf[q] - Uc[q];
+        double rg, pg;
+        rho_p_of(Ug, &rg, &pg);
+        if (!(rg > 0.0) || !(pg > 0.0))
+            for (int q = 0; q < 4; q++) Ug[q] = Uf[q]; /* non-physical extrapolation: the face state */
+        state_from_U(Ug, g);
+        return;
+    }
+    case OBC_SUBSONIC_INLET: { /* total pressure and density from the prescribed state's
+                                * isentropic stagnation values, direction of its velocity, static
+                                * pressure from the interior (capped at the total pressure) */
+        const double rinf = fs[0], uinf = fs[1], vinf = fs[2], pinf = fs[3];
+        const double q2 = uinf * uinf + vinf * vinf;
+        const double f = 1.0 + 0.5 * GM1 * (q2 / (GAMMA * pinf / rinf));
+        const double p0 = pinf * pow(f, GAMMA / GM1);
+        const double r0 = rinf * pow(f, 1.0 / GM1);
+        const double qinf = sqrt(q2);
+        const double dx = (qinf > 0.0) ? uinf / qinf : -n1;
+        const double dy = (qinf > 0.0) ? vinf / qinf : -n2;
+        const double p = fmin(in->W[3], p0);
+        const double rb = r0 * pow(p / p0, 1.0 / GAMMA);
+        const double qb = sqrt(fmax((2.0 / GM1) * (GAMMA * p0 / r0 - GAMMA * p / rb), 0.0));
+        cons_from_prim(rb, qb * dx, qb * dy, p, Ug);
         state_from_U(Ug, g);
         return;
     }
@@ -564,19 +621,19 @@ static void ghost_fstate(int bc, const FState *in, double n1, double n2, const d
     }
 }
 
-void on_ghost_state(int bc, const double *Win5, double nx, double ny, const double *fs4,
+void on_ghost_state(int bc, const double *Win5, double nx, double ny, const double *fs5,
                     double *Wg5) {
     FState in, g;
     for (int k = 0; k < 5; k++) in.W[k] = Win5[k];
     cons_from_prim(Win5[0], Win5[1], Win5[2], Win5[3], in.U);
-    ghost_fstate(bc, &in, nx, ny, fs4, &g);
+    ghost_fstate(bc, &in, nx, ny, fs5, &g);
     for (int k = 0; k < 5; k++) Wg5[k] = g.W[k];
 }
 
-void on_ghost_cons(int bc, const double *U4, double nx, double ny, const double *fs4, double *Ug4) {
+void on_ghost_cons(int bc, const double *U4, double nx, double ny, const double *fs5, double *Ug4) {
     FState in, g;
     state_from_U(U4, &in);
-    ghost_fstate(bc, &in, nx, ny, fs4, &g);
+    ghost_fstate(bc, &in, nx, ny, fs5, &g);
     for (int k = 0; k < 4; k++) Ug4[k] = g.U[k];
 }
 
@@ -871,11 +928,19 @@ double or_limiter_slope(int limiter, double a, double b) {
 /* ------------------------------------------------------------------------------------------
  * Gradients of (rho, u, v, p)
  * ---------------------------------------------------------------------------------------- */
+/* The prescribed state (rho, u, v, p) and the exits' static pressure (p_exit, else p). */
+static void fs5_of(const ONatCfg *cfg, double *fs5) {
+    for (int k = 0; k < 4; k++) fs5[k] = cfg->fs[k];
+    fs5[4] = (cfg->p_exit > 0.0) ? cfg->p_exit : cfg->fs[3];
+}
+
 static int bc_of(const ONatCfg *cfg, int kind) { return (kind >= 1 && kind <= 5) ? cfg->bc[kind] : OBC_WALL; }
 
 /* W5 per physical cell; grad[8*nc]: (drho/dx, drho/dy, du/dx, du/dy, dv/dx, dv/dy, dp/dx, dp/dy) */
 void on_gradients(const OMesh *m, const ONatCfg *cfg, const double *W5, double *grad) {
     const int nc = m->nc;
+    double fs5[5];
+    fs5_of(cfg, fs5);
     if (cfg->gradient == OGRAD_GG) {
         /* Calculate_Gradient_At_Cell_Center (src/Grid_Refine_Functions.cpp:221-246) for each of
          * (rho,u,v,p); ghost faces use the cell's own value */
@@ -929,7 +994,7 @@ void on_gradients(const OMesh *m, const ONatCfg *cfg, const double *W5, double *
             } else {
                 FState g;
                 ghost_fstate(bc_of(cfg, m->kind[4 * c + f]), &in, m->nrm[8 * c + 2 * f],
-                             m->nrm[8 * c + 2 * f + 1], cfg->fs, &g);
+                             m->nrm[8 * c + 2 * f + 1], fs5, &g);
                 for (int k = 0; k < 4; k++) Wn[k] = g.W[k];
             }
             for (int k = 0; k < 4; k++) {
@@ -1143,6 +1208,8 @@ void on_residual(const OMesh *m, const ONatCfg *cfg, const double *U, const doub
 
 static void residual_fo(const OMesh *m, const ONatCfg *cfg, const double *U, const double *U_exit_src,
                         double *R, double *dt, double *grad, const unsigned char *fo) {
+    double fs5[5];
+    fs5_of(cfg, fs5);
     const int nc = m->nc;
     FState *S = (FState *)malloc(sizeof(FState) * nc);
 #pragma omp parallel for schedule(static)
@@ -1165,7 +1232,7 @@ static void residual_fo(const OMesh *m, const ONatCfg *cfg, const double *U, con
         FState SL = S[L], SR;
         FState srcL = S[L];
         if (boundary && bc == OBC_EXIT && U_exit_src) state_from_U(&U_exit_src[4 * L], &srcL);
-        if (boundary) ghost_fstate(bc, &srcL, nx, ny, cfg->fs, &SR);
+        if (boundary) ghost_fstate(bc, &srcL, nx, ny, fs5, &SR);
         else SR = S[Rc];
         int st[4];
         if (cfg->order == 3 && !boundary && line_stencil(m, L, Rc, st)) {
@@ -1200,7 +1267,7 @@ static void residual_fo(const OMesh *m, const ONatCfg *cfg, const double *U, con
             if (boundary) {
                 FState src = fL;
                 if (bc == OBC_EXIT && U_exit_src) src = srcL;
-                ghost_fstate(bc, &src, nx, ny, cfg->fs, &fR);
+                ghost_fstate(bc, &src, nx, ny, fs5, &fR);
             } else {
                 reconstruct_side(cfg, &SR, &SL, &grad[8 * Rc], dx, dy, -1.0, phiL ? phiL + 4 : NULL, &fR);
             }
@@ -1291,19 +1358,6 @@ void or_rk_stage(int rk, int stage, int n, const double *Un, const double *Us, d
         }
 }
 
-/* Density and pressure of a conserved state, with the arithmetic of Calculate_Primitive_Variables
- * (src/Basic_Functions.cpp:85-142). */
-static void rho_p_of(const double *U, double *rho, double *p) {
-    double Rho = U[0];
-    double inv = 1.0 / Rho;
-    double v1 = U[1] * inv;
-    double v2 = U[2] * inv;
-    if (fabs(v1) <= 1e-10) v1 = 0.0;
-    if (fabs(v2) <= 1e-10) v2 = 0.0;
-    double vmag = (v1 * v1 + v2 * v2);
-    *rho = Rho;
-    *p = GM1 * (U[3] - 0.5 * Rho * vmag);
-}
 
 static int g_fallback_cells = 0;
 

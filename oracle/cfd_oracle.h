@@ -50,7 +50,11 @@ enum {
     OBC_FREESTREAM = 5, /* all four primitives prescribed */
     OBC_FARFIELD = 6,   /* characteristic (Riemann-invariant) far field */
     OBC_SUBSONIC_INLET_REF = 7, /* reference subsonic inlet, incl. its energy expression */
-    OBC_WALL_NOSLIP = 8 /* no-slip adiabatic wall: velocity negated, rho and p copied */
+    OBC_WALL_NOSLIP = 8, /* no-slip adiabatic wall: velocity negated, rho and p copied */
+    OBC_SUBSONIC_EXIT = 9, /* back-pressure exit (Subsonic_Exit_Boundary_Condition's characteristic
+                              face state, ghost 2 U_face - U_cell) */
+    OBC_SUBSONIC_INLET = 10 /* total-condition inlet: total p and rho and direction of fs, static p
+                               from the interior */
 };
 enum { OFLUX_ROE = 0, OFLUX_HLLC = 1, OFLUX_LLF = 2, OFLUX_AUSM = 3 };
 enum { OLIM_NONE = 0, OLIM_MINMOD = 1, OLIM_VANLEER = 2, OLIM_SUPERBEE = 3 };
@@ -77,16 +81,17 @@ typedef struct {
                         or pressure outside the relaxed range of its own and its face neighbours'
                         stage-input values, is recomputed with first-order states on all its faces;
                         2: the positivity test alone (no range test) */
+    double p_exit;   /* static pressure of back-pressure exits (OBC_SUBSONIC_EXIT); 0: fs[3] */
 } ONatCfg;
 
 /* ---- primitive / ghost helpers --------------------------------------------------------- */
 void or_prim(const double *U4, double *prim11);
 /* WENO5-JS of one variable (src/WENO2D.cpp:9-79); shift 0 right edge of c, 1 left edge. */
 double or_weno5(double a, double b, double c, double d, double e, int shift);
-void on_ghost_state(int bc, const double *Win5, double nx, double ny, const double *fs4,
+void on_ghost_state(int bc, const double *Win5, double nx, double ny, const double *fs5,
                     double *Wg5);
 /* conserved ghost state of a cell with conserved state U4 across a face with unit normal n */
-void on_ghost_cons(int bc, const double *U4, double nx, double ny, const double *fs4, double *Ug4);
+void on_ghost_cons(int bc, const double *U4, double nx, double ny, const double *fs5, double *Ug4);
 double or_limiter_slope(int limiter, double dminus, double dplus);
 
 /* ---- face-loop residual (the spec of the GPU kernels) ------------------------------------

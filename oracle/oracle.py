@@ -24,7 +24,7 @@ _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
 
 OBC = dict(inlet_ref=1, exit=2, wall=3, symmetry=4, freestream=5, farfield=6, subsonic_inlet_ref=7,
-           noslip_wall=8)
+           noslip_wall=8, subsonic_exit=9, subsonic_inlet=10)
 OVISC = dict(off=0, constant=1, sutherland=2)
 OFLUX = dict(roe=0, hllc=1, llf=2, ausm=3)
 OLIM = dict(none=0, minmod=1, vanleer=2, superbee=3)
@@ -44,12 +44,12 @@ class ONatCfg(ctypes.Structure):
                 ("gradient", ctypes.c_int), ("roe_bdiss", ctypes.c_int), ("cfl", ctypes.c_double),
                 ("bc", ctypes.c_int * 6), ("fs", ctypes.c_double * 4), ("freeze", ctypes.c_int),
                 ("phi", ctypes.c_void_p), ("viscous", ctypes.c_int), ("mu", ctypes.c_double),
-                ("prandtl", ctypes.c_double), ("r_gas", ctypes.c_double), ("positivity", ctypes.c_int)]
+                ("prandtl", ctypes.c_double), ("r_gas", ctypes.c_double), ("positivity", ctypes.c_int), ("p_exit", ctypes.c_double)]
 
 
 def make_cfg(flux="roe", order=1, limiter="vanleer", gradient="gg", roe_bdiss=0, cfl=0.5,
              bc=("inlet_ref", "exit", "wall", "wall", "wall"), fs=(1.0, 0.0, 0.0, 1.0), viscous="off",
-             mu=0.0, prandtl=0.72, r_gas=287.0, positivity=0) -> ONatCfg:
+             mu=0.0, prandtl=0.72, r_gas=287.0, positivity=0, p_exit=0.0) -> ONatCfg:
     c = ONatCfg()
     c.flux = OFLUX[flux]
     c.order = order
@@ -67,6 +67,7 @@ def make_cfg(flux="roe", order=1, limiter="vanleer", gradient="gg", roe_bdiss=0,
     c.prandtl = prandtl
     c.r_gas = r_gas
     c.positivity = int(positivity)
+    c.p_exit = float(p_exit)
     return c
 
 
@@ -209,16 +210,22 @@ class Oracle:
         self.lib.on_face_flux(ctypes.byref(cfg), boundary, _d(WL), _d(WR), nx, ny, dl, _d(F))
         return F
 
-    def ghost_state(self, bc: str, W5, nx, ny, fs) -> np.ndarray:
+    @staticmethod
+    def _fs5(fs, p_exit):
+        """(rho, u, v, p) and the exits' static pressure (p_exit, else p)."""
+        f = [float(x) for x in fs][:4]
+        return np.ascontiguousarray(f + [p_exit if p_exit else f[3]], np.float64)
+
+    def ghost_state(self, bc: str, W5, nx, ny, fs, p_exit=0.0) -> np.ndarray:
         W = np.ascontiguousarray(W5, np.float64)
-        f = np.ascontiguousarray(fs, np.float64)
+        f = self._fs5(fs, p_exit)
         out = np.zeros(5)
         self.lib.on_ghost_state(OBC[bc], _d(W), nx, ny, _d(f), _d(out))
         return out
 
-    def ghost_cons(self, bc: str, U4, nx, ny, fs) -> np.ndarray:
+    def ghost_cons(self, bc: str, U4, nx, ny, fs, p_exit=0.0) -> np.ndarray:
         U = np.ascontiguousarray(U4, np.float64)
-        f = np.ascontiguousarray(fs, np.float64)
+        f = self._fs5(fs, p_exit)
         out = np.zeros(4)
         self.lib.on_ghost_cons(OBC[bc], _d(U), nx, ny, _d(f), _d(out))
         return out

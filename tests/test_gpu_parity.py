@@ -154,6 +154,50 @@ def test_ghost_states_vs_oracle(orc, bump, bcs):
             assert np.array_equal(got, want), (c, f, bcs[kind - 1], got, want)
 
 
+FS_SUB = (1.4, 0.5, 0.05, 1.0)  # a Mach 0.5 prescribed state
+
+
+def subsonic_state(ctr, n):
+    x, y = ctr[:n, 0], ctr[:n, 1]
+    return cons_from_prim(1.4 * (1 + 0.05 * np.sin(3 * x)), 0.5 * (1 + 0.1 * np.cos(4 * y)),
+                          0.05 * np.sin(2 * x), 1.0 + 0.08 * np.cos(5 * x + y))
+
+
+@pytest.mark.parametrize("p_exit", [0.0, 0.93])
+def test_subsonic_boundaries_vs_oracle(orc, bump, p_exit):
+    """The back-pressure exit (9) and the total-condition inlet (10): ghost states, residuals at
+    first and second order and LTS-RK3 steps against the oracle. The inlet calls pow(), so those
+    comparisons are to 1e-13 relative; the exit ghosts are bit-exact."""
+    print(f"case subsonic boundaries p_exit={p_exit}", flush=True)
+    bcs = ("subsonic_inlet", "subsonic_exit", "wall", "wall", "wall")
+    om = orc.mesh(bump.points, bump.cells)
+    U = subsonic_state(om.arrays().ctr, om.nphys)
+    s = Solver(Mesh(bump.points, bump.cells, tile=64), make_config(bc=bcs, fs=FS_SUB, p_exit=p_exit))
+    a = s.mesh.arrays()
+    s.set_state(U)
+    Ug = s.ghost_states()
+    trip = np.concatenate([a.inlet.reshape(-1, 3), a.exit.reshape(-1, 3), a.wall.reshape(-1, 3)])
+    for c, f, g in trip:
+        kind = a.kind[c, f]
+        want = orc.ghost_cons(bcs[kind - 1], U[c], a.nrm[c, 2 * f], a.nrm[c, 2 * f + 1], FS_SUB, p_exit=p_exit)
+        got = Ug[g - s.mesh.nphys]
+        if bcs[kind - 1] == "subsonic_inlet":
+            assert np.allclose(got, want, rtol=1e-13, atol=0), (c, f, got, want)
+        else:
+            assert np.array_equal(got, want), (c, f, bcs[kind - 1], got, want)
+    for kw in (dict(flux="roe", order=1), dict(flux="hllc", order=2, limiter="vanleer", gradient="wlsq")):
+        Ro, dto, _ = om.residual(make_cfg(roe_bdiss=1, bc=bcs, fs=FS_SUB, p_exit=p_exit, cfl=0.7, **kw), U)
+        s.configure(make_config(roe_bdiss=1, bc=bcs, fs=FS_SUB, p_exit=p_exit, cfl=0.7, rk=3, lts=1, **kw))
+        s.set_state(U)
+        R, dt, _ = s.residual()
+        assert rel(R, Ro) <= 1e-13 and np.array_equal(dt, dto), (kw, rel(R, Ro))
+        s.step(5)
+        Uo = np.ascontiguousarray(U.copy())
+        for _ in range(5):
+            om.step(make_cfg(roe_bdiss=1, bc=bcs, fs=FS_SUB, p_exit=p_exit, cfl=0.7, **kw), Uo, rk=3, lts=1)
+        assert rel(s.get_state(), Uo) <= 1e-12, (kw, rel(s.get_state(), Uo))
+
+
 @pytest.mark.parametrize("flux,order,rk,lts", [("roe", 1, 3, 0), ("hllc", 2, 3, 1), ("ausm", 2, 4, 0),
                                               ("llf", 2, 4, 1), ("hllc", 2, 3, 0)])
 def test_native_steps_vs_oracle(orc, bump, flux, order, rk, lts):
