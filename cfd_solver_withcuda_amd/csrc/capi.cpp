@@ -284,7 +284,7 @@ static int check_config(const cfd_mesh_s *m, const cfd_config_t *c) {
     if (c->order < 1 || c->order > 3) return fail(CFD_E_ARG, "config: order must be 1, 2 or 3 (WENO5)");
     if (c->order == 3 && (c->freeze_tol > 0.0 || c->freeze_stall > 0))
         return fail(CFD_E_ARG, "config: limiter freezing needs order 2");
-    if (c->limiter < 0 || c->limiter > 3) return fail(CFD_E_ARG, "config: limiter must be 0..3");
+    if (c->limiter < 0 || c->limiter > 4) return fail(CFD_E_ARG, "config: limiter must be 0..4");
     if (c->gradient < 0 || c->gradient > 1) return fail(CFD_E_ARG, "config: gradient must be 0 or 1");
     if (!(c->cfl > 0.0)) return fail(CFD_E_ARG, "config: cfl must be positive");
     if (!(c->freeze_tol >= 0.0) || c->freeze_stall < 0) return fail(CFD_E_ARG, "config: freeze_tol and freeze_stall must be >= 0");

@@ -122,7 +122,7 @@ int check_batch_config(const cfd_config_t *c) {
     if (c->order < 1 || c->order > 3) return capi_fail(CFD_E_ARG, "batch config: order must be 1, 2 or 3 (WENO5)");
     if (c->order == 3 && (c->freeze_tol > 0.0 || c->freeze_stall > 0))
         return capi_fail(CFD_E_ARG, "batch config: limiter freezing needs order 2");
-    if (c->limiter < 0 || c->limiter > 3) return capi_fail(CFD_E_ARG, "batch config: limiter must be 0..3");
+    if (c->limiter < 0 || c->limiter > 4) return capi_fail(CFD_E_ARG, "batch config: limiter must be 0..4");
     if (c->gradient < 0 || c->gradient > 1) return capi_fail(CFD_E_ARG, "batch config: gradient must be 0 or 1");
     if (!(c->cfl > 0.0)) return capi_fail(CFD_E_ARG, "batch config: cfl must be positive");
     if (c->rk != 3 && c->rk != 4) return capi_fail(CFD_E_ARG, "batch config: rk must be 3 or 4");

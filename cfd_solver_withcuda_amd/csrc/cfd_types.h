@@ -21,7 +21,7 @@ enum BcType {
 // Reference face kinds (src/Read_Gmsh_File.cpp:318-323).
 enum FaceKind { KIND_INTERNAL = 0, KIND_LEFT = 1, KIND_RIGHT = 2, KIND_TOP = 3, KIND_BOTTOM = 4, KIND_WALL = 5 };
 enum FluxType { FLUX_ROE = 0, FLUX_HLLC = 1, FLUX_LLF = 2, FLUX_AUSM = 3 };
-enum LimiterType { LIM_NONE = 0, LIM_MINMOD = 1, LIM_VANLEER = 2, LIM_SUPERBEE = 3 };
+enum LimiterType { LIM_NONE = 0, LIM_MINMOD = 1, LIM_VANLEER = 2, LIM_SUPERBEE = 3, LIM_VANALBADA = 4 };
 enum GradType { GRAD_GG = 0, GRAD_WLSQ = 1 };
 
 // Device view of a tiled mesh. Cells are renumbered so that tiles are contiguous ranges of

@@ -15,6 +15,9 @@ constexpr double kGamma = 1.4;
 constexpr double kGm1 = (kGamma - 1.0);
 constexpr double kGamma1 = (kGamma / kGm1);  // include/definitions.h:65 (gamma / gamma_M_1)
 constexpr double kInvGm1 = 1.0 / kGm1;
+// Regularisation of the van Albada limiter (squared difference of the non-dimensional primitives
+// below which slopes are not limited).
+constexpr double kVanAlbadaEps = 1e-4;
 
 // Face state: primitives (rho, u, v, p, a) and conserved (rho, rho u, rho v, rho E).
 struct FState {
@@ -416,6 +419,11 @@ __device__ __forceinline__ double limited_slope(double a, double b) {
     } else if constexpr (LIM == LIM_VANLEER) {
         if (a * b <= 0.0) return 0.0;
         return 2.0 * a * b / (a + b);
+    } else if constexpr (LIM == LIM_VANALBADA) {
+        // van Albada, phi(r) = (r^2 + r) / (r^2 + 1) (Limiter_Cuda_Kernels.cu, vanalbada_limiter_device),
+        // in the differentiable form (a (b^2 + e) + b (a^2 + e)) / (a^2 + b^2 + 2 e): differences
+        // well above sqrt(e) are limited as phi(r) b, smaller ones take the central slope
+        return (a * (b * b + kVanAlbadaEps) + b * (a * a + kVanAlbadaEps)) / (a * a + b * b + 2.0 * kVanAlbadaEps);
     } else {
         if (a * b <= 0.0) return 0.0;
         double aa = fabs(a), bb = fabs(b);

@@ -643,6 +643,7 @@ static hipError_t launch_flux_family(const DevMesh &m, const DevCfg &c, int flux
             case LIM_NONE: launch_flux_t<F, ORDER, LIM_NONE, VISC>(m, c, mode, Us, grad, st, block, sm, s); break;
             case LIM_MINMOD: launch_flux_t<F, ORDER, LIM_MINMOD, VISC>(m, c, mode, Us, grad, st, block, sm, s); break;
             case LIM_VANLEER: launch_flux_t<F, ORDER, LIM_VANLEER, VISC>(m, c, mode, Us, grad, st, block, sm, s); break;
+            case LIM_VANALBADA: launch_flux_t<F, ORDER, LIM_VANALBADA, VISC>(m, c, mode, Us, grad, st, block, sm, s); break;
             default: launch_flux_t<F, ORDER, LIM_SUPERBEE, VISC>(m, c, mode, Us, grad, st, block, sm, s); break;
             }
         }

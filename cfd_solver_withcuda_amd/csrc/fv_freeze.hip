@@ -24,6 +24,7 @@ void launch_fz(const DevMesh &m, const DevCfg &c, int flux, int lim, const doubl
         case LIM_NONE: launch_fz_t<F, LIM_NONE, VISC>(m, c, Us, grad, st, block, sm, s); break;
         case LIM_MINMOD: launch_fz_t<F, LIM_MINMOD, VISC>(m, c, Us, grad, st, block, sm, s); break;
         case LIM_VANLEER: launch_fz_t<F, LIM_VANLEER, VISC>(m, c, Us, grad, st, block, sm, s); break;
+        case LIM_VANALBADA: launch_fz_t<F, LIM_VANALBADA, VISC>(m, c, Us, grad, st, block, sm, s); break;
         default: launch_fz_t<F, LIM_SUPERBEE, VISC>(m, c, Us, grad, st, block, sm, s); break;
         }
     };

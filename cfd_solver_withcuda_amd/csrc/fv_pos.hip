@@ -25,6 +25,7 @@ hipError_t launch_pos_pass(const DevMesh &m, const DevCfg &c, int flux, int lim,
         case LIM_NONE: launch_pos_t<F, LIM_NONE, PASS>(m, c, Us, grad, st, block, sm, s); break;
         case LIM_MINMOD: launch_pos_t<F, LIM_MINMOD, PASS>(m, c, Us, grad, st, block, sm, s); break;
         case LIM_VANLEER: launch_pos_t<F, LIM_VANLEER, PASS>(m, c, Us, grad, st, block, sm, s); break;
+        case LIM_VANALBADA: launch_pos_t<F, LIM_VANALBADA, PASS>(m, c, Us, grad, st, block, sm, s); break;
         default: launch_pos_t<F, LIM_SUPERBEE, PASS>(m, c, Us, grad, st, block, sm, s); break;
         }
     };

@@ -18,7 +18,7 @@ import numpy as np
 from ._lib import CfdError, Config, check, dptr, iptr, lib, uptr
 
 FLUX = dict(roe=0, hllc=1, llf=2, ausm=3)
-LIMITER = dict(none=0, minmod=1, vanleer=2, superbee=3)
+LIMITER = dict(none=0, minmod=1, vanleer=2, superbee=3, vanalbada=4)
 GRADIENT = dict(gg=0, wlsq=1)
 BC = dict(inlet_ref=1, exit=2, wall=3, symmetry=4, freestream=5, farfield=6, subsonic_inlet_ref=7,
           noslip_wall=8, subsonic_exit=9, subsonic_inlet=10)

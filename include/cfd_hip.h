@@ -39,7 +39,8 @@ typedef struct {
     int order;     /* 1 first order, 2 MUSCL, 3 WENO5 (WENO_Reconstruction, src/WENO2D.cpp:9-79,
                       on the conserved variables along lines of quadrilaterals through opposite
                       faces; MUSCL on faces without a complete five-cell line; native iteration) */
-    int limiter;   /* 0 none, 1 minmod, 2 Van Leer, 3 superbee (MUSCL faces) */
+    int limiter;   /* 0 none, 1 minmod, 2 Van Leer, 3 superbee, 4 van Albada (differentiable form,
+                      slopes of differences below 1e-2 not limited; MUSCL faces) */
     int gradient;  /* 0 Green-Gauss, 1 weighted least squares (MUSCL faces) */
     int roe_bdiss; /* Roe dissipation on boundary faces (0 = the reference's behaviour) */
     double cfl;

@@ -920,6 +920,12 @@ double or_limiter_slope(int limiter, double a, double b) {
         double s = fmax(fmin(2.0 * aa, bb), fmin(aa, 2.0 * bb));
         return (b > 0.0) ? s : -s;
     }
+    case OLIM_VANALBADA: {
+        /* van Albada phi(r) = (r^2 + r) / (r^2 + 1) (CUDA_KERNELS/Limiter_Cuda_Kernels.cu:146-154)
+         * in the differentiable form with e = 1e-4 (no cut-off at r <= 0) */
+        const double e = 1e-4;
+        return (a * (b * b + e) + b * (a * a + e)) / (a * a + b * b + 2.0 * e);
+    }
     default:
         return 0.0;
     }

@@ -57,7 +57,7 @@ enum {
                                from the interior */
 };
 enum { OFLUX_ROE = 0, OFLUX_HLLC = 1, OFLUX_LLF = 2, OFLUX_AUSM = 3 };
-enum { OLIM_NONE = 0, OLIM_MINMOD = 1, OLIM_VANLEER = 2, OLIM_SUPERBEE = 3 };
+enum { OLIM_NONE = 0, OLIM_MINMOD = 1, OLIM_VANLEER = 2, OLIM_SUPERBEE = 3, OLIM_VANALBADA = 4 };
 enum { OGRAD_GG = 0, OGRAD_WLSQ = 1 };
 
 typedef struct {

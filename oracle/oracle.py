@@ -27,7 +27,7 @@ OBC = dict(inlet_ref=1, exit=2, wall=3, symmetry=4, freestream=5, farfield=6, su
            noslip_wall=8, subsonic_exit=9, subsonic_inlet=10)
 OVISC = dict(off=0, constant=1, sutherland=2)
 OFLUX = dict(roe=0, hllc=1, llf=2, ausm=3)
-OLIM = dict(none=0, minmod=1, vanleer=2, superbee=3)
+OLIM = dict(none=0, minmod=1, vanleer=2, superbee=3, vanalbada=4)
 OGRAD = dict(gg=0, wlsq=1)
 
 

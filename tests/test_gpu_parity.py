@@ -47,7 +47,7 @@ BCS_NATIVE = ("freestream", "exit", "wall", "wall", "wall")
 CASES = []
 for flux in ("roe", "hllc", "llf", "ausm"):
     CASES.append((flux, 1, "none", "gg"))
-    for lim in ("minmod", "vanleer", "superbee", "none"):
+    for lim in ("minmod", "vanleer", "superbee", "vanalbada", "none"):
         for grad in ("gg", "wlsq"):
             CASES.append((flux, 2, lim, grad))
 
@@ -323,7 +323,8 @@ def test_freestream_preserved(bump):
     assert np.abs(R).max() < 1e-12
 
 
-@pytest.mark.parametrize("flux,lim,grad", [("hllc", "vanleer", "wlsq"), ("ausm", "minmod", "gg")])
+@pytest.mark.parametrize("flux,lim,grad", [("hllc", "vanleer", "wlsq"), ("ausm", "minmod", "gg"),
+                                           ("hllc", "vanalbada", "wlsq")])
 def test_limiter_freezing_vs_oracle(orc, bump, flux, lim, grad):
     """Record-then-frozen limiter factors (cfd_set_freeze): states bit-exact with the oracle's
     freezing over 3 plain, 1 recording and 4 frozen LTS RK3 steps."""

@@ -268,6 +268,47 @@ def test_limiter_symmetry_and_tvd_region(orc, lim):
     assert orc.limiter_slope("none", 3.0, 1.0) == 2.0  # unlimited: central difference
 
 
+def test_vanalbada_closed_forms(orc):
+    """The differentiable van Albada slope (a (b^2 + e) + b (a^2 + e)) / (a^2 + b^2 + 2 e), e = 1e-4:
+    phi(r) b of the reference kernel (phi = (r^2 + r) / (r^2 + 1)) for differences well above
+    sqrt(e), the central slope for differences well below it, symmetric, odd, and exact for equal
+    one-sided differences (linear data)."""
+    phi = lambda r: (r * r + r) / (r * r + 1.0)
+    e = 1e-4
+    for r in (0.25, 0.5, 1.0, 2.0, 3.0, 10.0):
+        for b in (1.0, -2.5, 40.0):
+            a = r * b
+            got = orc.limiter_slope("vanalbada", a, b)
+            # the regularisation moves the slope by e (a + b) (a - b)^2 / ((a^2 + b^2)(a^2 + b^2 + 2 e))
+            bound = e * abs(a + b) * (a - b) ** 2 / ((a * a + b * b) * (a * a + b * b + 2 * e))
+            assert abs(got - phi(r) * b) <= 1.01 * bound + 1e-15 * abs(b), (r, b, got)
+    for a, b in ((1e-5, 3e-5), (-2e-6, 7e-6), (4e-7, -1e-7)):
+        assert orc.limiter_slope("vanalbada", a, b) == pytest.approx(0.5 * (a + b), rel=1e-5)
+    rng = np.random.default_rng(2)
+    for _ in range(300):
+        a, b = rng.normal(size=2) * 10.0 ** rng.uniform(-4, 2, size=2)
+        s = orc.limiter_slope("vanalbada", a, b)
+        assert s == orc.limiter_slope("vanalbada", b, a)
+        assert -s == orc.limiter_slope("vanalbada", -a, -b)
+        assert orc.limiter_slope("vanalbada", a, a) == pytest.approx(a, rel=1e-15)
+
+
+def test_vanalbada_march_converges_unfrozen_where_vanleer_stalls(orc):
+    """The same Mach 2.5, 10 degree ramp as below: with the differentiable van Albada limiter the
+    plain (unfrozen) march reaches 1e-8, while Van Leer stalls far above it."""
+    m = meshgen.ramp_mesh(48, 32, 10.0, seed=11)
+    om = orc.mesh(m.points, m.cells)
+    fs = (1.4, 2.5, 0.0, 1.0)
+    bc = ("freestream", "exit", "exit", "wall", "wall")
+    U = np.ascontiguousarray(np.tile(cons_from_prim(*fs), (om.nphys, 1)))
+    cfg = make_cfg(flux="hllc", order=2, limiter="vanalbada", gradient="wlsq", cfl=0.8, bc=bc, fs=fs)
+    for k in range(8000):
+        err = om.step(cfg, U, rk=3, lts=1)[0][0]
+        if err <= 1e-8:
+            break
+    assert err <= 1e-8, (k, err)
+
+
 # ---------------------------------------------------------------------------------------------
 # gradients and reconstruction
 # ---------------------------------------------------------------------------------------------
@@ -343,7 +384,7 @@ def test_wlsq_brute_force_normal_equations(orc):
         assert np.allclose(grad[c].reshape(4, 2), sol.T, rtol=1e-10, atol=1e-12), c
 
 
-@pytest.mark.parametrize("lim", ("none", "minmod", "vanleer", "superbee"))
+@pytest.mark.parametrize("lim", ("none", "minmod", "vanleer", "superbee", "vanalbada"))
 def test_muscl_exact_for_linear_fields(orc, lim):
     """With exact (WLSQ, linear field) gradients both sides reconstruct the same face state, so
     the upwind dissipation vanishes: Roe, LLF and HLLC residuals agree on cells two layers away
