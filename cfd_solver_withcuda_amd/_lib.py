@@ -76,6 +76,7 @@ def lib() -> ctypes.CDLL:
     _sig(L, "cfd_batch_create", _i, [_i, _ip, _dp, _ip, _ip, _dp, cfgp, _i, ctypes.POINTER(_vp)])
     _sig(L, "cfd_batch_destroy", None, [_vp])
     _sig(L, "cfd_batch_info", _i, [_vp, _ip, _dp])
+    _sig(L, "cfd_batch_tuning", _i, [_vp, _ip])
     _sig(L, "cfd_batch_set_state", _i, [_vp, _dp])
     _sig(L, "cfd_batch_get_state", _i, [_vp, _dp])
     _sig(L, "cfd_batch_solve", _i, [_vp, _d, _i, _i, _ip, _dp, _ip, _dp, _dp, _vp])

@@ -54,6 +54,9 @@ class Batch:
         keys = ("ninst", "cells", "tiles", "face_entries", "tile", "face_budget", "block", "max_tile_faces")
         d = {k: int(v) for k, v in zip(keys, o)}
         d["build_s"], d["autotune_s"] = float(secs[0]), float(secs[1])
+        src = ctypes.c_int()
+        check(lib().cfd_batch_tuning(self.h, ctypes.byref(src)), "cfd_batch_tuning")
+        d["shape_source"] = ("given", "autotuned", "autotune cache")[src.value]
         return d
 
     @property

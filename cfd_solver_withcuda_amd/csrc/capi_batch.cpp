@@ -15,7 +15,11 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <fstream>
+#include <sstream>
 #include <string>
 #include <thread>
 #include <vector>
@@ -97,6 +101,7 @@ struct cfd_batch_s {
     void *progress_user = nullptr;
     int progress_every = 0;
     double build_s = 0.0, tune_s = 0.0;
+    int tune_src = 0;  // launch shape: 0 given, 1 timed by the autotuner, 2 from the autotune cache
     ~cfd_batch_s() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
@@ -420,6 +425,121 @@ int batch_time_steps(cfd_batch_s *b, int n, double *ms) {
     return 0;
 }
 
+// Steps the autotuner marches each candidate from freestream before timing it.
+constexpr int kTuneDevelop = 200;
+
+// `n` graph-launched steps of the current state (tolerance -1: nothing converges), then a wait.
+int batch_march(cfd_batch_s *b, int n) {
+    int rc = batch_reset(b);
+    if (rc) return rc;
+    rc = batch_graph(b, -1.0);
+    if (rc) return rc;
+    for (int k = 0; k < n; k++) BHIP(hipGraphLaunch(b->exec, b->stream));
+    BHIP(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+// ---- autotune cache --------------------------------------------------------------------------
+// One JSON object in a file, one entry per line: "key": [tile, face budget, block, ms per cell].
+// The key names the GPU (architecture, compute units) and the batch's statistics (instances, total
+// and largest cell counts in powers of two) and scheme, so a batch of the same shape on the same
+// GPU reuses the measured choice. Path: CFD_AUTOTUNE_CACHE, else $HOME/.cache/cfdhip_autotune.json;
+// an unreadable or unwritable file only disables the cache.
+std::string tune_cache_path() {
+    if (const char *p = std::getenv("CFD_AUTOTUNE_CACHE")) return p;
+    if (const char *h = std::getenv("HOME")) return std::string(h) + "/.cache/cfdhip_autotune.json";
+    return "";
+}
+
+int ilog2(long v) {
+    int k = 0;
+    while (v > 1) {
+        v >>= 1;
+        k++;
+    }
+    return k;
+}
+
+std::string tune_key(const cfd_config_t &c, const std::vector<InstanceInput> &in) {
+    int dev = 0;
+    hipDeviceProp_t p{};
+    std::string arch = "unknown";
+    int cus = 0;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&p, dev) == hipSuccess) {
+        arch = p.gcnArchName;
+        cus = p.multiProcessorCount;
+    }
+    long total = 0, largest = 0;
+    for (const auto &x : in) {
+        total += x.ncells;
+        largest = std::max<long>(largest, x.ncells);
+    }
+    std::ostringstream k;
+    k << arch << ":" << cus << "|inst2^" << ilog2((long)in.size()) << "|cells2^" << ilog2(total) << "|max2^"
+      << ilog2(largest) << "|order" << c.order << "|flux" << c.flux << "|lim" << c.limiter << "|grad"
+      << c.gradient << "|visc" << c.viscous << "|rk" << c.rk;
+    std::string s = k.str();
+    for (char &ch : s)
+        if (ch == '"' || ch == '\\') ch = '_';
+    return s;
+}
+
+// entries of the cache file: key -> "[tile, budget, block, per_cell]"
+std::vector<std::pair<std::string, std::string>> tune_cache_load(const std::string &path) {
+    std::vector<std::pair<std::string, std::string>> out;
+    if (path.empty()) return out;
+    std::ifstream f(path);
+    std::string line;
+    while (std::getline(f, line)) {
+        const size_t a = line.find('"'), b = line.find("\": [");
+        const size_t e = line.find(']', b == std::string::npos ? 0 : b);
+        if (a == std::string::npos || b == std::string::npos || e == std::string::npos || b < a) continue;
+        out.emplace_back(line.substr(a + 1, b - a - 1), line.substr(b + 3, e - b - 2));
+    }
+    return out;
+}
+
+bool tune_cache_get(const std::string &path, const std::string &key, int *tile, int *budget, int *block) {
+    for (const auto &kv : tune_cache_load(path)) {
+        if (kv.first != key) continue;
+        int t = 0, bu = 0, bl = 0;
+        if (std::sscanf(kv.second.c_str(), "[%d, %d, %d", &t, &bu, &bl) == 3 && t > 0 && bu > 0 &&
+            (bl == 64 || bl == 128 || bl == 256) && t <= bl) {
+            *tile = t;
+            *budget = bu;
+            *block = bl;
+            return true;
+        }
+    }
+    return false;
+}
+
+void tune_cache_put(const std::string &path, const std::string &key, int tile, int budget, int block,
+                    double per_cell) {
+    if (path.empty()) return;
+    auto entries = tune_cache_load(path);
+    char val[128];
+    std::snprintf(val, sizeof(val), "[%d, %d, %d, %.6e]", tile, budget, block, per_cell);
+    bool found = false;
+    for (auto &kv : entries)
+        if (kv.first == key) {
+            kv.second = val;
+            found = true;
+        }
+    if (!found) entries.emplace_back(key, val);
+    const std::string tmp = path + ".tmp" + std::to_string((long)std::chrono::steady_clock::now().time_since_epoch().count());
+    {
+        std::ofstream f(tmp);
+        if (!f) return;
+        f << "{\n";
+        for (size_t i = 0; i < entries.size(); i++)
+            f << "\"" << entries[i].first << "\": " << entries[i].second << (i + 1 < entries.size() ? ",\n" : "\n");
+        f << "}\n";
+        if (!f) return;
+    }
+    (void)std::rename(tmp.c_str(), path.c_str());
+}
+
 int set_freestream(cfd_batch_s *b, const double *fs4) {
     std::vector<double4> U(b->nc);
     const double gm1 = 1.4 - 1.0;  // formed as the reference forms gamma - 1 (include/definitions.h:62-64)
@@ -453,7 +573,12 @@ int cfd_batch_create(int ninst, const int *pt_off, const double *pts3, const int
     const double t0 = now_s();
     int tile = 0, budget = 0, block = cfg->block;
     double tune_s = 0.0;
-    if (block == 0) {
+    int tune_src = 0;
+    const std::string cache = (block == 0) ? tune_cache_path() : std::string();
+    const std::string key = (block == 0) ? tune_key(*cfg, in) : std::string();
+    if (block == 0 && tune_cache_get(cache, key, &tile, &budget, &block)) {
+        tune_src = 2;
+    } else if (block == 0) {
         // Autotune: time each (tile, face budget, block) candidate on a sample of the instances
         // (the largest ones, up to ~1.5M cells) and keep the fastest per cell.
         std::vector<int> order(ninst);
@@ -477,7 +602,10 @@ int cfd_batch_create(int ninst, const int *pt_off, const double *pts3, const int
             rc = build_batch(sample, sfs.data(), sc, cd[0], cd[1], cd[2], nthreads, &sb);
             if (rc) continue;  // e.g. a candidate that does not fit in LDS
             double ms = 0.0;
+            // timed on a developing flow, not on the uniform start (where every limiter skips its
+            // division and every face takes the same flux branch)
             rc = set_freestream(sb, sfs.data());
+            if (!rc) rc = batch_march(sb, kTuneDevelop);
             if (!rc) rc = batch_time_steps(sb, 5, &ms);
             delete sb;
             if (rc) return rc;
@@ -491,6 +619,8 @@ int cfd_batch_create(int ninst, const int *pt_off, const double *pts3, const int
         }
         if (block == 0) return capi_fail(CFD_E_ARG, "cfd_batch_create: no autotune candidate fits");
         tune_s = now_s() - t0;
+        tune_src = 1;
+        tune_cache_put(cache, key, tile, budget, block, best);
     } else {
         if (block != 64 && block != 128 && block != 256)
             return capi_fail(CFD_E_ARG, "cfd_batch_create: block must be 0 (autotune), 64, 128 or 256");
@@ -503,6 +633,7 @@ int cfd_batch_create(int ninst, const int *pt_off, const double *pts3, const int
     rc = build_batch(in, fs4, c, tile, budget, block, nthreads, &b);
     if (rc) return rc;
     b->tune_s = tune_s;
+    b->tune_src = tune_src;
     b->build_s = now_s() - t0 - tune_s;
     rc = set_freestream(b, fs4);
     if (rc) {
@@ -525,6 +656,12 @@ int cfd_batch_info(const cfd_batch_t *b, int *out8, double *secs2) {
         secs2[0] = b->build_s;
         secs2[1] = b->tune_s;
     }
+    return 0;
+}
+
+int cfd_batch_tuning(const cfd_batch_t *b, int *source) {
+    if (!b || !source) return capi_fail(CFD_E_ARG, "cfd_batch_tuning: null argument");
+    *source = b->tune_src;
     return 0;
 }
 

@@ -8,9 +8,14 @@
      surrogate's field (warm, with per-instance fallback to freestream when the warm start's
      first-step residual exceeds the freestream start's). Both marches run to Error[0] <= tol.
 
-Warm-started instances record their limiter factors on the first step and march frozen (their
-shocks are already in place); a frozen march whose error grows tenfold returns to the evaluated
-limiter. The plain FP64 march decides the answer; the surrogate only moves its starting point.
+The scheme is HLLC + MUSCL with the differentiable van Albada limiter + WLSQ gradients: its
+steady solutions are reached by the plain march without limiter freezing, so both marches converge
+to the same discrete solution (with a switching limiter such as Van Leer, steady marches stall and
+need frozen limiter factors, and a march frozen at a different state converges to that frozen
+scheme's own fixed point). With `freeze_stall` > 0 (Van Leer), warm-started instances record their
+limiter factors on the first step instead. The plain FP64 march decides the answer; the surrogate
+only moves its starting point. `field_tol` continues both converged batches, untimed, to a tighter
+tolerance and compares the fields there too.
 """
 from __future__ import annotations
 
@@ -23,8 +28,8 @@ from .batch import Batch
 from .solver import cons_from_prim, make_config
 from .surrogate import MLP, features, predicted_state, prim_from_cons
 
-SCHEME = dict(flux="hllc", order=2, limiter="vanleer", gradient="wlsq", cfl=1.2, rk=3, lts=1,
-              roe_bdiss=1, bc=cases.BUMP_BCS, freeze_stall=300)
+SCHEME = dict(flux="hllc", order=2, limiter="vanalbada", gradient="wlsq", cfl=1.2, rk=3, lts=1,
+              roe_bdiss=1, bc=cases.BUMP_BCS, freeze_stall=0)
 
 
 def freestream_states(cs):
@@ -32,15 +37,16 @@ def freestream_states(cs):
             for c in cs]
 
 
-def scheme(cfl=None):
+def scheme(cfl=None, **over):
     kw = dict(SCHEME)
     if cfl is not None:
         kw["cfl"] = cfl
+    kw.update({k: v for k, v in over.items() if v is not None})
     return kw
 
 
-def solve_batch(cs, tol, max_steps, check_every, block=128, start=None, cfl=None):
-    b = Batch([c.mesh for c in cs], np.array([c.fs for c in cs]), make_config(block=block, **scheme(cfl)))
+def solve_batch(cs, tol, max_steps, check_every, block=128, start=None, cfl=None, **over):
+    b = Batch([c.mesh for c in cs], np.array([c.fs for c in cs]), make_config(block=block, **scheme(cfl, **over)))
     if start is not None:
         b.set_state(start)
     r = b.solve(tol=tol, max_steps=max_steps, check_every=check_every)
@@ -57,11 +63,16 @@ def training_set(cs, states):
 
 def run(n_train=64, n_eval=32, train_cells=30000, eval_cells=60000, tol=1e-8, train_tol=1e-8,
         max_steps=60000, check_every=100, mlp_steps=3000, mlp_batch=16384, seed=5, cfl=None,
-        freeze_warm=True, warm_cfl=None, history=False, log=print):
+        freeze_warm=True, warm_cfl=None, history=False, log=print, limiter=None, freeze_stall=None,
+        field_tol=None):
     out = {}
+    over = dict(limiter=limiter, freeze_stall=freeze_stall)
+    sch = scheme(cfl, **over)
+    freeze_warm = freeze_warm and sch["freeze_stall"] > 0
+    out["scheme"] = {k: sch[k] for k in ("flux", "limiter", "gradient", "cfl", "freeze_stall")}
     t0 = time.perf_counter()
     tr = cases.bump_family(n_train, seed=seed, cells=train_cells, stream=200)
-    b_tr, rt = solve_batch(tr, train_tol, max_steps, check_every, cfl=cfl)
+    b_tr, rt = solve_batch(tr, train_tol, max_steps, check_every, cfl=cfl, **over)
     states = b_tr.get_state()
     # converged solutions, and the finite end states of the ones still marching at max_steps
     ok = rt["status"] >= 0
@@ -84,7 +95,7 @@ def run(n_train=64, n_eval=32, train_cells=30000, eval_cells=60000, tol=1e-8, tr
     ev = cases.bump_family(n_eval, seed=seed + 100, cells=eval_cells, stream=300)
     fs_states = freestream_states(ev)
     # plain: freestream start
-    b = Batch([c.mesh for c in ev], np.array([c.fs for c in ev]), make_config(block=128, **scheme(cfl)))
+    b = Batch([c.mesh for c in ev], np.array([c.fs for c in ev]), make_config(block=128, **sch))
     b.set_state(fs_states)
     rp = b.solve(tol=tol, max_steps=max_steps, check_every=check_every)
     plain_states = b.get_state()
@@ -115,7 +126,8 @@ def run(n_train=64, n_eval=32, train_cells=30000, eval_cells=60000, tol=1e-8, tr
     rw = b.solve(tol=tol, max_steps=max_steps, check_every=check_every)
     warm_wall = time.perf_counter() - tw
     warm_states = b.get_state()
-    b.set_freeze(None)
+    if freeze_warm:
+        b.set_freeze(None)
     if history:
         out["warm_history"] = b.history().tolist()
     out["warm"] = dict(wall_s=warm_wall, march_s=rw["wall_s"], inference_s=t_inf, fallback_check_s=t_check,
@@ -137,6 +149,23 @@ def run(n_train=64, n_eval=32, train_cells=30000, eval_cells=60000, tol=1e-8, tr
                                    max=float(np.max(s_err)) if s_err else None)
     out["converged_fields_max_rel_diff"] = float(np.max(d_conv)) if d_conv else None
     out["converged_fields_rel_l2_diff"] = dict(mean=float(np.mean(d_l2)), max=float(np.max(d_l2))) if d_l2 else None
+    if field_tol is not None and both.any():
+        # both converged batches marched on (untimed) to field_tol: the distance between the two
+        # fixed points, with less of the remaining march error in it
+        fin = {}
+        for name, states in (("plain", plain_states), ("warm", warm_states)):
+            b.set_state(states)
+            r2 = b.solve(tol=field_tol, max_steps=max_steps, check_every=check_every)
+            fin[name] = (b.get_state(), r2["status"])
+        d_max, d_l2t = [], []
+        for k in range(len(ev)):
+            if both[k] and fin["plain"][1][k] == 0 and fin["warm"][1][k] == 0:
+                Up, Uw = fin["plain"][0][k], fin["warm"][0][k]
+                d_max.append(np.abs(Uw - Up).max() / np.abs(Up).max())
+                d_l2t.append(np.linalg.norm(Uw - Up) / np.linalg.norm(Up))
+        out["fields_at_field_tol"] = dict(tol=field_tol, instances=len(d_max),
+                                          max_rel_diff=float(np.max(d_max)) if d_max else None,
+                                          rel_l2_diff_max=float(np.max(d_l2t)) if d_l2t else None)
     out["speedup"] = out["plain"]["wall_s"] / out["warm"]["wall_s"]
     # the instances both marches converged: time until the last of them converged, and steps
     if both.any():

@@ -197,6 +197,12 @@ void cfd_batch_destroy(cfd_batch_t *b);
 /* out8 = ninst, cells, tiles, face entries, tile, face budget, block, max faces per tile;
  * secs2 = mesh build seconds, autotune seconds (either may be NULL) */
 int cfd_batch_info(const cfd_batch_t *b, int *out8, double *secs2);
+/* Where the launch shape came from: *source = 0 given (cfg->block), 1 timed by the autotuner
+ * (each candidate marched 200 steps from freestream, then 5 timed steps), 2 read from the autotune
+ * cache (a JSON file, CFD_AUTOTUNE_CACHE or $HOME/.cache/cfdhip_autotune.json, keyed by the GPU's
+ * architecture and compute units and the batch's instance count, total and largest cell counts
+ * in powers of two, and scheme). */
+int cfd_batch_tuning(const cfd_batch_t *b, int *source);
 /* U[4*total cells] conserved, instances concatenated in order, each in its caller's cell order */
 int cfd_batch_set_state(cfd_batch_t *b, const double *U);
 int cfd_batch_get_state(cfd_batch_t *b, double *U);

@@ -160,3 +160,45 @@ def test_batch_history_cfl_and_first_step_freezing():
         assert np.array_equal(Ub[i], s.get_state()), i
     with pytest.raises(Exception):
         b.set_freeze([2, 0, 0])
+
+
+def test_uneven_batch_of_32_bit_exact_vs_single_solvers_and_oracle(tmp_path, monkeypatch):
+    """32 compression ramps of uneven sizes (about 1k to 100k cells, log-uniform) in one batch with
+    an autotuned launch shape: after 4 steps every instance equals the same mesh solved alone, bit
+    for bit, and the two smallest equal the oracle. A second batch of the same shape takes its launch
+    shape from the autotune cache."""
+    from cfd_solver_withcuda_amd import cases
+
+    monkeypatch.setenv("CFD_AUTOTUNE_CACHE", str(tmp_path / "autotune.json"))
+    cs = cases.ramp_batch(32, seed=41, min_cells=1000, max_cells=100000)
+    sizes = np.array([len(c.mesh.cells) for c in cs])
+    assert sizes.max() > 60000 and sizes.min() < 3000
+    print(f"case uneven batch of 32, {sizes.sum()} cells", flush=True)
+    fs = np.array([c.fs for c in cs])
+    b = Batch([c.mesh for c in cs], fs, make_config(block=0, **kw()))
+    info = b.info()
+    assert info["shape_source"] == "autotuned"
+    r = b.solve(tol=-1.0, max_steps=4, check_every=4)
+    assert np.all(r["steps"] == 4)
+    Ub = b.get_state()
+    tile, budget, block = info["tile"], info["face_budget"], info["block"]
+    for i, c in enumerate(cs):
+        s = Solver(Mesh(c.mesh.points, c.mesh.cells, tile=tile, face_budget=budget),
+                   make_config(block=block, fs=c.fs, **kw()))
+        s.set_state(freestream_state(len(c.mesh.cells), c.fs))
+        s.step(4)
+        assert np.array_equal(Ub[i], s.get_state()), i
+    orc = Oracle()
+    for i in np.argsort(sizes)[:2]:
+        c = cs[i]
+        om = orc.mesh(c.mesh.points, c.mesh.cells)
+        U = np.ascontiguousarray(freestream_state(len(c.mesh.cells), c.fs))
+        for _ in range(4):
+            om.step(make_cfg(flux="ausm", order=2, limiter="vanleer", gradient="wlsq", cfl=0.8, bc=RAMP_BCS,
+                             fs=c.fs), U, rk=3, lts=1)
+        assert np.array_equal(Ub[i], U), i
+    b2 = Batch([c.mesh for c in cs], fs, make_config(block=0, **kw()))
+    i2 = b2.info()
+    assert i2["shape_source"] == "autotune cache"
+    assert (i2["tile"], i2["face_budget"], i2["block"]) == (tile, budget, block)
+    assert '"' in (tmp_path / "autotune.json").read_text()
