@@ -8,14 +8,15 @@
      surrogate's field (warm, with per-instance fallback to freestream when the warm start's
      first-step residual exceeds the freestream start's). Both marches run to Error[0] <= tol.
 
-The scheme is HLLC + MUSCL with the differentiable van Albada limiter + WLSQ gradients: its
-steady solutions are reached by the plain march without limiter freezing, so both marches converge
-to the same discrete solution (with a switching limiter such as Van Leer, steady marches stall and
-need frozen limiter factors, and a march frozen at a different state converges to that frozen
-scheme's own fixed point). With `freeze_stall` > 0 (Van Leer), warm-started instances record their
-limiter factors on the first step instead. The plain FP64 march decides the answer; the surrogate
-only moves its starting point. `field_tol` continues both converged batches, untimed, to a tighter
-tolerance and compares the fields there too.
+The scheme is HLLC + MUSCL/Van Leer + WLSQ gradients with limiter freezing on a 300-step stall
+(without freezing, Van Leer marches on this family stall above 1e-8). Warm-started instances record
+their limiter factors on the first step (their shocks are already in place); a frozen march whose
+error grows tenfold returns to the evaluated limiter. A march frozen at one state converges to the
+fixed point of that frozen scheme, so two marches frozen at different states end at slightly
+different fields. With `limiter="vanalbada", freeze_stall=0` (the differentiable limiter, no
+freezing) both marches share one discrete solution wherever the plain march converges. The plain
+FP64 march decides the answer; the surrogate only moves its starting point. `field_tol` continues
+both converged batches, untimed, to a tighter tolerance and compares the fields there too.
 """
 from __future__ import annotations
 
@@ -28,8 +29,8 @@ from .batch import Batch
 from .solver import cons_from_prim, make_config
 from .surrogate import MLP, features, predicted_state, prim_from_cons
 
-SCHEME = dict(flux="hllc", order=2, limiter="vanalbada", gradient="wlsq", cfl=1.2, rk=3, lts=1,
-              roe_bdiss=1, bc=cases.BUMP_BCS, freeze_stall=0)
+SCHEME = dict(flux="hllc", order=2, limiter="vanleer", gradient="wlsq", cfl=1.2, rk=3, lts=1,
+              roe_bdiss=1, bc=cases.BUMP_BCS, freeze_stall=300)
 
 
 def freestream_states(cs):
