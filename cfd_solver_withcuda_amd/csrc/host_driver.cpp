@@ -41,10 +41,10 @@ void need_mesh(const char *where) {
     if (!g_solver) throw std::runtime_error(std::string(where) + ": no mesh (call Read_VTK_Grid or Form_Cells)");
 }
 
-cfd_config_t config(int order, int rk, int ref_mode) {
+cfd_config_t config(int order, int rk, int ref_mode, int type) {
     cfd_config_t c;
     std::memset(&c, 0, sizeof(c));
-    switch (Dissipation_Type) {
+    switch (type) {
     case 1: c.flux = 2; break;    // LLF
     case 3: c.flux = 0; break;    // ROE (Harten entropy fix, no dissipation on ghost faces)
     case 101: c.flux = 1; break;  // HLLC
@@ -84,8 +84,8 @@ cfd_config_t config(int order, int rk, int ref_mode) {
 // else second or first order.
 int scheme_order() { return Is_WENO ? 3 : (Is_Second_Order ? 2 : 1); }
 
-void configure(int order, int rk, int ref_mode) {
-    cfd_config_t c = config(order, rk, ref_mode);
+void configure(int order, int rk, int ref_mode, int type = Dissipation_Type) {
+    cfd_config_t c = config(order, rk, ref_mode, type);
     check(cfd_solver_configure(g_solver, &c), "configure");
 }
 
@@ -120,26 +120,153 @@ void one_iteration(int rk, const char *where) {
     g_have_err = true;
 }
 
-void residual(int order, const char *where) {
+// Net flux and local time step of U_Cells' physical rows with dissipation `type` at `order`.
+void eval_residual(int order, int type, const char *where, std::vector<double> &R, std::vector<double> &dt) {
     need_mesh(where);
     // the reference's second order is Roe on limited differences (ROE_2O); other fluxes use MUSCL
-    configure(order, 3, (order == 2 && Dissipation_Type == 3) ? 1 : 0);
+    configure(order, 3, (order == 2 && type == 3) ? 1 : 0, type);
     sync_state();
     const int n = No_Physical_Cells;
-    std::vector<double> R(4 * (size_t)n), dt(n);
+    R.resize(4 * (size_t)n);
+    dt.resize(n);
     check(cfd_residual(g_solver, R.data(), dt.data(), nullptr, nullptr), where);
+}
+
+void residual(int order, const char *where) {
+    std::vector<double> R, dt;
+    eval_residual(order, Dissipation_Type, where, R, dt);
+    const int n = No_Physical_Cells;
     for (int i = 0; i < n; i++) {
         for (int k = 0; k < 4; k++) Cells_Net_Flux[i][k] = R[4 * (size_t)i + k];
         Cells_del_t[i] = dt[i];
     }
 }
 
+// ---- the dissipation plugin point -------------------------------------------------------------
+// A sweep is one pass of per-cell calls in ascending cell order (the reference's loop,
+// src/Net_Flux.cpp:65-94): its first call evaluates every cell on the device; the others read
+// the result. A call whose cell is not above the previous call's, or that names another scheme,
+// starts a new sweep.
+struct Scheme {
+    int type;   // Dissipation_Type
+    int order;  // 1, or 2 (ROE_2O for type 3, MUSCL otherwise)
+    bool operator==(const Scheme &o) const { return type == o.type && order == o.order; }
+};
+
+struct Registered {
+    Dissipation_Function_t fn;
+    Scheme scheme;
+};
+std::vector<Registered> g_registered;
+
+struct Sweep {
+    bool valid = false;
+    Scheme scheme{0, 0};
+    int last_cell = -1;
+    long generation = 0;  // evaluation counter when this sweep evaluated
+    std::vector<double> R, dt;
+};
+Sweep g_flux_sweep, g_dt_sweep;
+long g_generation = 0;  // counts the sweep evaluations
+
+bool resolve(Dissipation_Function_t fn, Scheme &out) {
+    if (fn == &LLF) out = {1, 1};
+    else if (fn == &ROE) out = {3, 1};
+    else if (fn == &ROE_2O) out = {3, 2};
+    else if (fn == &HLLC) out = {101, 1};
+    else if (fn == &AUSM) out = {102, 1};
+    else {
+        for (const Registered &r : g_registered)
+            if (r.fn == fn) {
+                out = r.scheme;
+                return true;
+            }
+        return false;
+    }
+    return true;
+}
+
+[[noreturn]] void stand_in(const char *name) {
+    throw std::runtime_error(std::string(name) + ": the device evaluates dissipation for all faces at once; pass " +
+                             name + " to Calculate_Flux_For_All_Faces");
+}
+
+void check_cell(int cell, const char *where) {
+    need_mesh(where);
+    if (cell < 0 || cell >= No_Physical_Cells)
+        throw std::runtime_error(std::string(where) + ": cell " + std::to_string(cell) + " is not a physical cell");
+}
+
 }  // namespace
+
+void LLF(const int &, int &, const int &) { stand_in("LLF"); }
+void ROE(const int &, int &, const int &) { stand_in("ROE"); }
+void ROE_2O(const int &, int &, const int &) { stand_in("ROE_2O"); }
+void HLLC(const int &, int &, const int &) { stand_in("HLLC"); }
+void AUSM(const int &, int &, const int &) { stand_in("AUSM"); }
+
+void Register_Dissipation_Function(Dissipation_Function_t fn, int dissipation_type, bool second_order) {
+    if (!fn) throw std::runtime_error("Register_Dissipation_Function: null function");
+    if (dissipation_type != 1 && dissipation_type != 3 && dissipation_type != 101 && dissipation_type != 102)
+        throw std::runtime_error("Register_Dissipation_Function: dissipation type must be 1 (LLF), 3 (ROE), "
+                                 "101 (HLLC) or 102 (AUSM)");
+    Scheme sc{dissipation_type, second_order ? 2 : 1};
+    Scheme known;
+    if (resolve(fn, known)) {
+        for (Registered &r : g_registered)
+            if (r.fn == fn) {
+                r.scheme = sc;
+                return;
+            }
+        throw std::runtime_error("Register_Dissipation_Function: the library's own functions are fixed");
+    }
+    g_registered.push_back({fn, sc});
+}
+
+void Calculate_Flux_For_All_Faces(int &Current_Cell_No, Dissipation_Function_t Dissipation_Function) {
+    const char *where = "Calculate_Flux_For_All_Faces";
+    check_cell(Current_Cell_No, where);
+    Scheme sc;
+    if (!Dissipation_Function || !resolve(Dissipation_Function, sc))
+        throw std::runtime_error("Calculate_Flux_For_All_Faces: unknown dissipation function (pass LLF, ROE, "
+                                 "ROE_2O, HLLC or AUSM, or register it with Register_Dissipation_Function)");
+    Sweep &w = g_flux_sweep;
+    if (!w.valid || !(w.scheme == sc) || Current_Cell_No <= w.last_cell) {
+        eval_residual(sc.order, sc.type, where, w.R, w.dt);
+        w.valid = true;
+        w.scheme = sc;
+        w.generation = ++g_generation;
+    }
+    w.last_cell = Current_Cell_No;
+    // the reference adds each face's (average - dissipative) flux into the cell's net flux
+    for (int k = 0; k < 4; k++) Cells_Net_Flux[Current_Cell_No][k] += w.R[4 * (size_t)Current_Cell_No + k];
+}
+
+void Evaluate_Time_Step(int &Cell_No) {
+    const char *where = "Evaluate_Time_Step";
+    check_cell(Cell_No, where);
+    Sweep &t = g_dt_sweep;
+    if (!t.valid || Cell_No <= t.last_cell) {
+        // a new sweep: the time steps of the flux sweep evaluated since the last one began, else
+        // an evaluation of its own (the local time step depends on the state only)
+        if (g_flux_sweep.valid && g_flux_sweep.generation > t.generation) {
+            t.dt = g_flux_sweep.dt;
+            t.generation = g_flux_sweep.generation;
+        } else {
+            std::vector<double> R;
+            eval_residual(1, 3, where, R, t.dt);
+            t.generation = ++g_generation;
+        }
+        t.valid = true;
+    }
+    t.last_cell = Cell_No;
+    Cells_del_t[Cell_No] = t.dt[Cell_No];
+}
 
 bool Form_Cells(int npts, const double *pts3, int ncells, const int *cells4) {
     Release_Solver();
     if (cfd_mesh_create(npts, pts3, ncells, cells4, 128, 256, 1, &g_mesh)) return false;
-    cfd_config_t c = config(1, 3, 1);
+    cfd_config_t c = config(1, 3, 1, Dissipation_Type);
     if (cfd_solver_create(g_mesh, &c, &g_solver)) {
         cfd_mesh_destroy(g_mesh);
         g_mesh = nullptr;
@@ -156,6 +283,8 @@ bool Form_Cells(int npts, const double *pts3, int ncells, const int *cells4) {
     Error.assign(4, 0.0);
     g_dev.clear();
     g_pending = g_have_err = false;
+    g_flux_sweep = Sweep();
+    g_dt_sweep = Sweep();
     return true;
 }
 

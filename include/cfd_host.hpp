@@ -5,6 +5,8 @@
 //   Apply_Boundary_Conditions()         src/Boundary_Conditions.cpp:12-61     (include/Boundary_Conditions.h:11)
 //   Evaluate_Cell_Net_Flux_1O/2O()      src/Net_Flux.cpp:46-128               (include/Flux.h:11-12)
 //   Evaluate_Cell_Net_Flux_WENO()       src/WENO2D.cpp:395-456
+//   Calculate_Flux_For_All_Faces(c, fn) src/Net_Flux.cpp:7-44                 (include/Flux.h:14)
+//   Evaluate_Time_Step(c)               src/Time_Step.cpp:331-359             (include/Timestep.h:10)
 //   Explicit_Method()                   src/Numerical_Method.cpp:26-103       (include/Solver.h:11)
 //   Runge_Kutta_Method()                src/Numerical_Method.cpp:342-457      (include/Solver.h:12)
 //   Estimate_Error(), Update()          src/Error_Estimate_Update.cpp:10-44   (include/Error_Update.h:10-11)
@@ -65,6 +67,33 @@ void Evaluate_Cell_Net_Flux_2O();
 // WENO5 net flux and time step (src/WENO2D.cpp:395-456): along lines of quadrilaterals, MUSCL
 // (Limiter_Type, Gradient_Type) on the other faces, where the reference leaves zero.
 void Evaluate_Cell_Net_Flux_WENO();
+
+// ---- the dissipation plugin point (include/Flux.h:14-24) -----------------------------------
+// The reference's net flux takes the dissipation as a function called per face; the device
+// evaluates all faces at once, so a function pointer here names a device scheme:
+//   LLF (Dissipation_Type 1), ROE (3, Harten's fix, no dissipation on ghost faces), ROE_2O (Roe
+//   on MinMod-limited differences, Limiter_Zeta/Limiter_Zeta1) keep the reference's names;
+//   HLLC (101) and AUSM (102) are this library's. Calling one of them directly throws.
+// Register_Dissipation_Function maps a caller's own function (for instance the reference's LLF
+// kept under another name) to a scheme: second_order selects ROE_2O for type 3 and MUSCL
+// (Limiter_Type, Gradient_Type) for the others.
+// Calculate_Flux_For_All_Faces(c, fn) adds cell c's net flux sum(average - dissipative) over its
+// faces into Cells_Net_Flux[c], as the reference does (the caller zeroes the row first). Calls
+// in ascending cell order form a sweep: its first call evaluates every cell of U_Cells on the
+// device; a call whose cell is not above the previous call's, or that names another scheme,
+// starts a new sweep. The per-face globals Average_Convective_Flux / Dissipative_Flux are not
+// kept. Evaluate_Time_Step(c) sets Cells_del_t[c] the same way (from the latest sweep's
+// evaluation). An unknown function or a cell outside [0, No_Physical_Cells) throws.
+typedef void (*Dissipation_Function_t)(const int &, int &, const int &);
+void LLF(const int &, int &, const int &);
+void ROE(const int &, int &, const int &);
+void ROE_2O(const int &, int &, const int &);
+void HLLC(const int &, int &, const int &);
+void AUSM(const int &, int &, const int &);
+void Register_Dissipation_Function(Dissipation_Function_t fn, int dissipation_type, bool second_order);
+void Calculate_Flux_For_All_Faces(int &Current_Cell_No, Dissipation_Function_t Dissipation_Function);
+void Evaluate_Time_Step(int &Cell_No);
+
 void Explicit_Method();
 void Runge_Kutta_Method();
 void Estimate_Error();

@@ -106,3 +106,23 @@ def test_mesh_rejects_bad_input():
         Mesh(pts, np.array([[0, 1, 2, -1]], np.int32), device=False)  # zero-length faces
     with pytest.raises(CfdError):
         Mesh(np.eye(3), np.array([[0, 1, 7, -1]], np.int32), device=False)  # node out of range
+
+
+def test_host_driver_exports_declared_names():
+    """libcfdhost.so defines every function include/cfd_host.hpp declares (the reference's driver
+    names, including the dissipation plugin point)."""
+    import shutil
+    import subprocess
+
+    host = os.path.join(os.path.dirname(LIB_PATH), "libcfdhost.so")
+    assert os.path.exists(host)
+    text = open(os.path.join(ROOT, "include", "cfd_host.hpp")).read()
+    text = re.sub(r"//.*", "", text)
+    names = set(re.findall(r"^(?:void|bool)\s+(\w+)\s*\(", text, flags=re.M))
+    assert {"Calculate_Flux_For_All_Faces", "Evaluate_Time_Step", "ROE_2O", "Runge_Kutta_Method"} <= names
+    nm = shutil.which("nm")
+    if nm is None:
+        pytest.skip("nm not available")
+    out = subprocess.run([nm, "-DC", "--defined-only", host], capture_output=True, text=True, check=True).stdout
+    defined = {m.group(1) for m in re.finditer(r"\s[TW]\s+(\w+)\(", out)}
+    assert not names - defined, sorted(names - defined)

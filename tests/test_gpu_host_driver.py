@@ -81,3 +81,32 @@ def test_host_driver_weno_vs_oracle(tmp_path):
     for _ in range(nsteps):
         om.step(cfg, Uo, rk=3, lts=0, ref=True)
     assert np.array_equal(U, Uo), float(np.abs(U - Uo).max())
+
+
+PLUGIN_EXE = os.path.join(ROOT, "tests", "host", "_build", "plugin_driver")
+
+
+def test_dissipation_plugin_point_sweeps(tmp_path):
+    """The reference's per-cell loop over Calculate_Flux_For_All_Faces(cell, fn) and
+    Evaluate_Time_Step(cell) (src/Net_Flux.cpp:65-94) through the host driver's plugin point:
+    every scheme's sweep (ROE, LLF, HLLC, AUSM, ROE_2O, a registered caller function) equals the bulk
+    Evaluate_Cell_Net_Flux_1O/2O bit for bit, a state change between sweeps is seen, and unknown
+    functions, direct stand-in calls and bad cells throw (checked inside the program). The ROE sweep
+    of the Sod state equals the reference's own net flux and del_t."""
+    assert os.path.exists(PLUGIN_EXE), "build first: python -m cfd_solver_withcuda_amd.build"
+    g = np.load(os.path.join(ROOT, "tests", "golden", "sod_roe.npz"), allow_pickle=False)
+    m = meshgen.shock_tube_mesh(100, 10)
+    vtk = tmp_path / "sod.vtk"
+    meshgen.write_vtk(str(vtk), m)
+    u0 = tmp_path / "U0.bin"
+    np.ascontiguousarray(g["U0"], np.float64).tofile(u0)
+    out = tmp_path / "out.bin"
+    r = subprocess.run([PLUGIN_EXE, str(vtk), str(u0), str(out)], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    assert "0 failures" in r.stdout
+    n = len(m.cells)
+    d = np.fromfile(out, np.float64)
+    R, dt = d[:4 * n].reshape(n, 4), d[4 * n:]
+    assert np.allclose(R, g["R_roe"], rtol=0, atol=1e-13 * np.abs(g["R_roe"]).max())
+    assert np.allclose(dt, g["dt_roe"], rtol=1e-14, atol=0)
