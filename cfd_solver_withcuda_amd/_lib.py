@@ -83,6 +83,7 @@ def lib() -> ctypes.CDLL:
     _sig(L, "cfd_batch_time_steps", _i, [_vp, _i, ctypes.POINTER(ctypes.c_float)])
     _sig(L, "cfd_batch_set_freeze", _i, [_vp, _ip])
     _sig(L, "cfd_batch_set_cfl", _i, [_vp, _d])
+    _sig(L, "cfd_batch_set_launch", _i, [_vp, _i])
     _sig(L, "cfd_batch_set_progress", _i, [_vp, PROGRESS_FN, _vp, _i])
     _sig(L, "cfd_batch_history", _i, [_vp, _ip, _dp])
     _fp = ctypes.POINTER(ctypes.c_float)

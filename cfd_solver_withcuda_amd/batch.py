@@ -116,6 +116,13 @@ class Batch:
         """Courant number of later solves."""
         check(lib().cfd_batch_set_cfl(self.h, float(cfl)), "cfd_batch_set_cfl")
 
+    def set_launch(self, mode):
+        """Launch of the gradient and flux/stage passes in later solves: "tiles" (one workgroup per
+        tile) or "persistent" (workgroups that walk the tiles still marching; order 1 or 2,
+        inviscid). Results are identical."""
+        m = {"tiles": 0, "persistent": 1}[mode] if isinstance(mode, str) else int(mode)
+        check(lib().cfd_batch_set_launch(self.h, m), "cfd_batch_set_launch")
+
     def history(self) -> np.ndarray:
         """Error[0] per check of the last solve, [checks, ninst]."""
         n = ctypes.c_int()

@@ -22,7 +22,7 @@ LIB = os.path.join(PKG, "libcfdhip.so")
 ARCH = os.environ.get("CFD_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["fv_kernels.hip", "fv_visc.hip", "fv_weno.hip", "fv_weno_visc.hip", "ref_kernels.hip",
-               "mlp_kernels.hip", "capi_noalloc.hip", "fv_pos.hip", "fv_freeze.hip"]
+               "mlp_kernels.hip", "capi_noalloc.hip", "fv_pos.hip", "fv_freeze.hip", "fv_persist.hip"]
 CPP_SOURCES = ["mesh_host.cpp", "capi.cpp", "capi_batch.cpp", "capi_mlp.cpp"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-function",

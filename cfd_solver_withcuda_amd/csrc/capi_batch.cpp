@@ -94,6 +94,7 @@ struct cfd_batch_s {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     double graph_tol = -1.0;
+    int graph_tiles = 0;         // tiles the captured step's launch grids cover
     std::vector<int> fz_start;   // per instance freeze mode each solve starts with (empty: 0)
     std::vector<double> hist;    // Error[0] per instance at each check of the last solve
     int hist_checks = 0;
@@ -344,9 +345,13 @@ int batch_reset(cfd_batch_s *b) {
     return 0;
 }
 
-// The launches of one step on b->stream (captured into the graph).
-int batch_step_launches(cfd_batch_s *b, double tol) {
+// The launches of one step on b->stream (captured into the graph). The tile passes' grids cover
+// `grid_tiles` tiles (at least the tiles still marching: the compacted list never grows during a
+// solve), so that a batch whose instances have mostly converged launches few empty workgroups.
+int batch_step_launches(cfd_batch_s *b, double tol, int grid_tiles) {
     const cfd_config_t &c = b->cfg;
+    DevMesh dm = b->dev;
+    dm.ntiles = grid_tiles;  // the batch kernels read the marching count from the list, not ntiles
     const int nst = (c.rk == 4) ? 4 : 3;
     const double4 *ins3[3] = {b->Un, b->Ua, b->Ub};
     double4 *outs3[3] = {b->Ua, b->Ub, b->Un};
@@ -355,7 +360,7 @@ int batch_step_launches(cfd_batch_s *b, double tol) {
     for (int k = 0; k < nst; k++) {
         const double4 *in = (nst == 4) ? ins4[k] : ins3[k];
         double4 *out = (nst == 4) ? outs4[k] : outs3[k];
-        if (c.order >= 2) BHIP(launch_grad(b->dev, b->dcfg, c.gradient, in, b->grad, b->block, b->stream));
+        if (c.order >= 2) BHIP(launch_grad(dm, b->dcfg, c.gradient, in, b->grad, b->block, b->stream));
         StageArgs a{};
         a.rk = nst;
         a.stage = k;
@@ -369,7 +374,7 @@ int batch_step_launches(cfd_batch_s *b, double tol) {
         a.err_partial = a.last ? b->err_partial : nullptr;
         a.freeze = b->d_fz;
         a.psi = b->psi;
-        BHIP(launch_flux(b->dev, b->dcfg, c.flux, c.order, c.limiter, MODE_STAGE, in, b->grad, a, b->block, b->stream));
+        BHIP(launch_flux(dm, b->dcfg, c.flux, c.order, c.limiter, MODE_STAGE, in, b->grad, a, b->block, b->stream));
     }
     BHIP(launch_batch_status(b->ninst, b->d_inst_tile, b->err_partial, tol, b->d_status, b->d_steps, b->d_err4,
                              b->d_fz, b->d_fz_best, b->d_fz_since, c.freeze_tol, c.freeze_stall, b->stream));
@@ -378,8 +383,9 @@ int batch_step_launches(cfd_batch_s *b, double tol) {
     return 0;
 }
 
-int batch_graph(cfd_batch_s *b, double tol) {
-    if (b->exec && b->graph_tol == tol) return 0;
+int batch_graph(cfd_batch_s *b, double tol, int grid_tiles = -1) {
+    if (grid_tiles < 0) grid_tiles = b->ntiles;
+    if (b->exec && b->graph_tol == tol && b->graph_tiles == grid_tiles) return 0;
     if (b->exec) {
         BHIP(hipGraphExecDestroy(b->exec));
         b->exec = nullptr;
@@ -389,7 +395,7 @@ int batch_graph(cfd_batch_s *b, double tol) {
         b->graph = nullptr;
     }
     BHIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    int rc = batch_step_launches(b, tol);
+    int rc = batch_step_launches(b, tol, grid_tiles);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(b->stream, &g);
     if (rc) {
@@ -400,6 +406,7 @@ int batch_graph(cfd_batch_s *b, double tol) {
     b->graph = g;
     BHIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
     b->graph_tol = tol;
+    b->graph_tiles = grid_tiles;
     return 0;
 }
 
@@ -728,6 +735,14 @@ int cfd_batch_solve(cfd_batch_t *b, double tol, int max_steps, int check_every, 
         }
         if (b->progress && b->progress_every > 0 && b->hist_checks % b->progress_every == 0)
             b->progress(done, marching, b->progress_user);
+        if (!all && !b->dcfg.persist) {
+            // once the marching instances hold half the tiles the launch grids cover or fewer,
+            // capture the step again with grids sized to them
+            int active = 0;
+            for (int i = 0; i < ni; i++)
+                if (st[i] == 1) active += b->inst_tile[i + 1] - b->inst_tile[i];
+            if (2 * active <= b->graph_tiles && (rc = batch_graph(b, tol, active))) return rc;
+        }
     }
     const double wall = now_s() - t0;
     if (steps) BHIP(hipMemcpy(steps, b->d_steps, ni * sizeof(int), hipMemcpyDeviceToHost));
@@ -766,6 +781,17 @@ int cfd_batch_set_cfl(cfd_batch_t *b, double cfl) {
     b->cfg.cfl = cfl;
     b->dcfg.cfl = cfl;
     b->graph_tol = std::nan("");  // the captured step holds the old value: capture again
+    return 0;
+}
+
+int cfd_batch_set_launch(cfd_batch_t *b, int mode) {
+    if (!b || (mode != 0 && mode != 1)) return capi_fail(CFD_E_ARG, "cfd_batch_set_launch: mode must be 0 or 1");
+    if (mode == 1 && (b->cfg.order > 2 || b->cfg.viscous))
+        return capi_fail(CFD_E_ARG, "cfd_batch_set_launch: persistent workgroups need order 1 or 2, inviscid");
+    if (b->dcfg.persist != mode) {
+        b->dcfg.persist = mode;
+        b->graph_tol = std::nan("");  // the captured step holds the old launches: capture again
+    }
     return 0;
 }
 

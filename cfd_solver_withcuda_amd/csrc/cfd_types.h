@@ -72,6 +72,7 @@ struct DevCfg {
     double kfac;     // conductivity / viscosity = cp / Pr = gamma r_gas / ((gamma - 1) Pr)
     double inv_pr;   // 1 / Pr
     double p_exit;   // static pressure of back-pressure exits (0: the prescribed state's p)
+    int persist;     // tile passes: 0 one workgroup per tile, 1 persistent workgroups (launch only)
 };
 
 }  // namespace cfd

@@ -37,11 +37,19 @@ __device__ __forceinline__ int tile_of_block(int b, int ntiles, int remap) {
     return (b % 8) * per + b / 8;
 }
 
-// Tile of this workgroup: its XCD-aware position in the tile sequence, or in the list of tiles
-// still marching for a batched mesh. -1 when the workgroup has no tile.
-__device__ __forceinline__ int block_tile(const DevMesh &m, int remap) {
-    const int n = m.tile_list ? *m.tile_count : m.ntiles;
-    const int b = tile_of_block(blockIdx.x, n, remap);
+// Tiles to process: all, or for a batched mesh the list of tiles still marching.
+__device__ __forceinline__ int tiles_to_run(const DevMesh &m) { return m.tile_list ? *m.tile_count : m.ntiles; }
+
+// Virtual workgroups covering n tiles (a multiple of 8 with the XCD-aware order).
+__device__ __forceinline__ int tile_span(int n, int remap) { return remap ? 8 * ((n + 7) / 8) : n; }
+
+// Tile of virtual workgroup v: its XCD-aware position in the tile sequence, or in the list of
+// tiles still marching. -1 when v has no tile. The XCD-aware order is a bijection of [0, span)
+// only: a batched launch keeps the grid of all tiles while fewer still march, and workgroups
+// beyond the span must not map onto a tile again.
+__device__ __forceinline__ int virtual_tile(const DevMesh &m, int v, int n, int remap) {
+    if (v >= tile_span(n, remap)) return -1;
+    const int b = tile_of_block(v, n, remap);
     if (b >= n) return -1;
     return m.tile_list ? m.tile_list[b] : b;
 }
@@ -349,16 +357,27 @@ __device__ __forceinline__ void flux_face(const DevMesh &m, const DevCfg &cfg,
 // FZ: the stage reads st.freeze and can record or use frozen limiter factors (launched only when a
 // freezing state is set; without it the kernel holds none of that code). Exit ghosts from
 // st.Uexit (the reference iteration's stages) exist in residual mode only.
-template <int FLUX, int ORDER, int LIM, int MODE, bool VISC, int POS = 0, bool FZ = false>
+//
+// PERSIST: the launch has fewer workgroups than tiles (what the GPU holds at once) and each
+// workgroup walks the virtual workgroup indices blockIdx.x, blockIdx.x + gridDim.x, ... up to the
+// tiles still marching, so a batch whose instances have mostly converged launches no empty
+// workgroups. With a grid that is a multiple of 8 a workgroup stays on one XCD and keeps that
+// XCD's run of tiles. Without it the loop runs once (one tile per workgroup).
+template <int FLUX, int ORDER, int LIM, int MODE, bool VISC, int POS = 0, bool FZ = false, bool PERSIST = false>
 __global__ void __launch_bounds__(256) CFD_FLUX_ATTR flux_tile_kernel(DevMesh m, DevCfg cfg,
                                                         const double4 *__restrict__ Us,
                                                         const double *__restrict__ grad,
                                                         StageArgs st) {
     extern __shared__ double smem[];
-    const int tile = block_tile(m, cfg.xcd_remap);
-    if (tile < 0) return;
+    const int ntr = tiles_to_run(m);
+    const int span = PERSIST ? tile_span(ntr, cfg.xcd_remap) : 0;
+    int v = blockIdx.x;
+    do {
+    const int tile = virtual_tile(m, v, ntr, cfg.xcd_remap);
+    if (tile < 0) continue;
+    if (PERSIST && v != (int)blockIdx.x) __syncthreads();  // the previous tile's LDS reads are done
     if constexpr (POS == 2) {
-        if (!st.dirty[tile]) return;  // no face of this tile touches a flagged cell
+        if (!st.dirty[tile]) continue;  // no face of this tile touches a flagged cell
     }
     double fsv[5];
     tile_fs(m, cfg, tile, fsv);
@@ -606,11 +625,26 @@ __global__ void __launch_bounds__(256) CFD_FLUX_ATTR flux_tile_kernel(DevMesh m,
     if constexpr (POS == 2) {
         if (threadIdx.x == 0) st.dirty[tile] = 0;  // read by every thread before the first barrier
     }
+    } while (PERSIST && (v += gridDim.x) < span);
 }
 
 static inline int grid_of(const DevMesh &m, const DevCfg &c) {
     return c.xcd_remap ? 8 * ((m.ntiles + 7) / 8) : m.ntiles;
 }
+
+// Grid of a persistent launch: the workgroups the GPU holds at once for this kernel (occupancy
+// at this block size and LDS, times the compute units), a multiple of 8, at most one per tile.
+int persist_grid(const void *kernel, const DevMesh &m, const DevCfg &c, int block, size_t sm);
+template <typename K>
+static inline int grid_persist(K kernel, const DevMesh &m, const DevCfg &c, int block, size_t sm) {
+    return persist_grid(reinterpret_cast<const void *>(kernel), m, c, block, sm);
+}
+
+// Persistent stage passes (fv_persist.hip): order 1 or 2, inviscid, with or without limiter
+// freezing.
+hipError_t launch_flux_persist(const DevMesh &m, const DevCfg &c, int flux, int order, int lim,
+                               const double4 *Us, const double *grad, const StageArgs &st, int block,
+                               size_t sm, hipStream_t s);
 
 // Order-2 stages with limiter freezing (fv_freeze.hip): the FZ variants of the stage kernel.
 hipError_t launch_flux_frozen(const DevMesh &m, const DevCfg &c, int flux, int lim, bool visc,

@@ -12,6 +12,8 @@
 // stage is one gradient pass plus one flux pass.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "fv_flux.h"
 
 namespace cfd {
@@ -69,13 +71,19 @@ __device__ __forceinline__ void grad_face(const DevMesh &m, const DevCfg &cfg,
     }
 }
 
-template <int GRAD>
+// PERSIST: persistent workgroups walking the tiles (as flux_tile_kernel).
+template <int GRAD, bool PERSIST = false>
 __global__ void __launch_bounds__(256) grad_tile_kernel(DevMesh m, DevCfg cfg,
                                                         const double4 *__restrict__ Us,
                                                         double *__restrict__ grad) {
     extern __shared__ double smem[];
-    const int tile = block_tile(m, cfg.xcd_remap);
-    if (tile < 0) return;
+    const int ntr = tiles_to_run(m);
+    const int span = PERSIST ? tile_span(ntr, cfg.xcd_remap) : 0;
+    int v = blockIdx.x;
+    do {
+    const int tile = virtual_tile(m, v, ntr, cfg.xcd_remap);
+    if (tile < 0) continue;
+    if (PERSIST && v != (int)blockIdx.x) __syncthreads();  // the previous tile's LDS reads are done
     double fsv[5];
     tile_fs(m, cfg, tile, fsv);
     const int T = m.tile;
@@ -163,6 +171,7 @@ __global__ void __launch_bounds__(256) grad_tile_kernel(DevMesh m, DevCfg cfg,
         out[2] = make_double2(g[4], g[5]);
         out[3] = make_double2(g[6], g[7]);
     }
+    } while (PERSIST && (v += gridDim.x) < span);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -569,11 +578,31 @@ size_t flux_smem_bytes(const DevMesh &m, int order, int grad_lds) {
 hipError_t launch_grad(const DevMesh &m, const DevCfg &c, int grad_type, const double4 *Us,
                        double *grad, int block, hipStream_t s) {
     const size_t sm = grad_smem_bytes(m, grad_type);
-    if (grad_type == GRAD_GG)
+    if (c.persist) {
+        if (grad_type == GRAD_GG) {
+            auto k = grad_tile_kernel<GRAD_GG, true>;
+            hipLaunchKernelGGL(k, dim3(grid_persist(k, m, c, block, sm)), dim3(block), sm, s, m, c, Us, grad);
+        } else {
+            auto k = grad_tile_kernel<GRAD_WLSQ, true>;
+            hipLaunchKernelGGL(k, dim3(grid_persist(k, m, c, block, sm)), dim3(block), sm, s, m, c, Us, grad);
+        }
+    } else if (grad_type == GRAD_GG) {
         hipLaunchKernelGGL(grad_tile_kernel<GRAD_GG>, dim3(grid_of(m, c)), dim3(block), sm, s, m, c, Us, grad);
-    else
+    } else {
         hipLaunchKernelGGL(grad_tile_kernel<GRAD_WLSQ>, dim3(grid_of(m, c)), dim3(block), sm, s, m, c, Us, grad);
+    }
     return hipGetLastError();
+}
+
+int persist_grid(const void *kernel, const DevMesh &m, const DevCfg &c, int block, size_t sm) {
+    const int full = grid_of(m, c);
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, sm) != hipSuccess || per_cu < 1)
+        return full;  // no occupancy figure: one workgroup per tile
+    const int g = std::max(8, (per_cu * cus) / 8 * 8);
+    return std::min(g, full);
 }
 
 // LDS of one flux launch: the face-geometry copy (3 doubles per face) only when the launch forms
@@ -591,6 +620,10 @@ hipError_t launch_flux(const DevMesh &m, const DevCfg &c, int flux, int order, i
                        hipStream_t s) {
     const bool geom = (mode == MODE_RESID) || (st.lts && st.stage == 0) || (st.last && st.dt_partial);
     const size_t sm = flux_launch_smem(m, order, c.grad_lds, block, geom);
+    if (c.persist) {
+        if (mode != MODE_STAGE || order > 2 || c.viscous) return hipErrorInvalidValue;
+        return launch_flux_persist(m, c, flux, order, lim, Us, grad, st, block, sm, s);
+    }
     if (order == 3)
         return c.viscous ? launch_flux_weno_visc(m, c, flux, lim, mode, Us, grad, st, block, sm, s)
                          : launch_flux_weno(m, c, flux, lim, mode, Us, grad, st, block, sm, s);

@@ -225,6 +225,11 @@ typedef void (*cfd_progress_fn)(int steps_done, int marching, void *user);
 int cfd_batch_set_progress(cfd_batch_t *b, cfd_progress_fn fn, void *user, int every_checks);
 /* Courant number of later solves (the local time steps scale with it). */
 int cfd_batch_set_cfl(cfd_batch_t *b, double cfl);
+/* How later solves launch the gradient and flux/stage passes (the captured step is rebuilt):
+ * 0 (default) one workgroup per tile of the batch, those of converged instances exiting at once;
+ * 1 persistent workgroups, as many as the GPU holds at once, each walking the tiles still
+ * marching (order 1 or 2, inviscid). Results are identical. */
+int cfd_batch_set_launch(cfd_batch_t *b, int mode);
 /* Error[0] of every instance at each check of the last solve: *nchecks checks, e0[check*ninst + i]
  * (e0 may be NULL to query the count). The batched counterpart of the reference's per-iteration
  * Error rows (src/output_files.cpp:558-575). */

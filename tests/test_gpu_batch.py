@@ -202,3 +202,66 @@ def test_uneven_batch_of_32_bit_exact_vs_single_solvers_and_oracle(tmp_path, mon
     assert i2["shape_source"] == "autotune cache"
     assert (i2["tile"], i2["face_budget"], i2["block"]) == (tile, budget, block)
     assert '"' in (tmp_path / "autotune.json").read_text()
+
+
+def _solve_both(meshes, fs, cfg, tol, max_steps, check_every, freeze=None):
+    out = []
+    for mode in ("tiles", "persistent"):
+        b = Batch(meshes, fs, make_config(**cfg))
+        b.set_launch(mode)
+        if freeze is not None:
+            b.set_freeze(freeze)
+        r = b.solve(tol=tol, max_steps=max_steps, check_every=check_every)
+        out.append((r, b.get_state(), b.history()))
+    return out
+
+
+def test_persistent_launch_equals_tile_launch_as_instances_converge():
+    """Persistent workgroups (cfd_batch_set_launch 1) give the same march as one workgroup per
+    tile, bit for bit: 32 uneven ramps (more tiles than the persistent grid holds, so workgroups
+    walk several tiles), first order to Error[0] <= 1e-4 so that instances drop out at different
+    steps, and second order HLLC / Van Leer with first-step freezing on half the instances."""
+    from cfd_solver_withcuda_amd import cases
+
+    cs = cases.ramp_batch(32, seed=41, min_cells=1000, max_cells=100000)
+    meshes, fs = [c.mesh for c in cs], np.array([c.fs for c in cs])
+    print(f"case persistent vs tiles, 32 ramps, {sum(len(m.cells) for m in meshes)} cells, order 1", flush=True)
+    (r0, U0, h0), (r1, U1, h1) = _solve_both(meshes, fs, dict(block=128, **kw(order=1)), 1e-4, 3000, 50)
+    assert len(set(r0["steps"].tolist())) > 4  # instances converge at different steps
+    assert np.array_equal(r0["steps"], r1["steps"]) and np.array_equal(r0["status"], r1["status"])
+    assert np.array_equal(r0["err"], r1["err"]) and np.array_equal(h0, h1)
+    for i in range(len(cs)):
+        assert np.array_equal(U0[i], U1[i]), i
+    # the instances that march longest, while most tiles have dropped out, equal the same meshes
+    # solved alone (workgroups beyond the marching tiles must not take a tile a second time)
+    for i in np.argsort(r0["steps"])[-3:]:
+        c = cs[i]
+        s = Solver(Mesh(c.mesh.points, c.mesh.cells, tile=128, face_budget=256),
+                   make_config(block=128, fs=c.fs, **kw(order=1)))
+        s.set_state(freestream_state(len(c.mesh.cells), c.fs))
+        s.step(int(r0["steps"][i]))
+        assert np.array_equal(U0[i], s.get_state()), i
+    print("case persistent vs tiles, order 2 HLLC / Van Leer, freezing", flush=True)
+    cfg = dict(block=64, flux="hllc", order=2, limiter="vanleer", gradient="wlsq", cfl=0.8, rk=3, lts=1,
+               bc=RAMP_BCS, freeze_stall=50)
+    (r0, U0, _), (r1, U1, _) = _solve_both(meshes, fs, cfg, -1.0, 60, 20, freeze=[k % 2 for k in range(32)])
+    assert np.array_equal(r0["err"], r1["err"])
+    for i in range(len(cs)):
+        assert np.array_equal(U0[i], U1[i]), i
+
+
+def test_persistent_launch_rk4_and_limits():
+    """RK4 stages under persistent workgroups equal the tile launch; order 3 and viscous batches
+    refuse persistent workgroups, and unknown modes are refused."""
+    meshes, fs = instances()
+    cfg = dict(block=128, flux="roe", order=2, limiter="minmod", gradient="gg", cfl=0.6, rk=4, lts=1, bc=RAMP_BCS)
+    (r0, U0, _), (r1, U1, _) = _solve_both(meshes, fs, cfg, -1.0, 8, 4)
+    for i in range(len(meshes)):
+        assert np.array_equal(U0[i], U1[i]), i
+    b = Batch(meshes, fs, make_config(block=128, **kw()))
+    with pytest.raises(Exception):
+        b.set_launch(2)
+    b3 = Batch(meshes[:1], fs[:1], make_config(block=128, **dict(kw(), order=3, flux="llf")))
+    with pytest.raises(Exception):
+        b3.set_launch("persistent")
+    b3.set_launch("tiles")
