@@ -65,8 +65,10 @@ def parse():
                         "minmod + Green-Gauss; ffs_march: that configuration's full 2000-step march from the "
                         "impulsive start (positivity fallback on); bump: the 1.27M-cell mixed bump (HLLC + "
                         "MUSCL/Van Leer + WLSQ, LTS TVD-RK3); steady: one bump solve to 1e-8")
-    p.add_argument("--positivity", type=int, default=0,
-                   help="1: time the step mesh's stages with the positivity fallback on (two-pass stages)")
+    p.add_argument("--positivity", type=int, default=None,
+                   help="step-mesh workloads: 1 (default) times the stages with the positivity fallback on "
+                        "(detecting and recomputing passes; without it the second-order march on the developed "
+                        "step flow turns non-physical at the corner within a few steps), 0 the plain stages")
     p.add_argument("--march-steps", type=int, default=2000, help="ffs_march: steps from the impulsive start")
     p.add_argument("--with-bump", type=int, default=1,
                    help="step-mesh workloads: also measure the 1.27M-cell bump (sub-object bump)")
@@ -341,7 +343,7 @@ def steady_bench(args):
     cfl = args.cfl if args.cfl is not None else 1.2
     cfg = dict(flux="hllc", order=2, limiter="vanleer", gradient="wlsq", cfl=cfl, rk=3, lts=1,
                bc=("freestream", "exit", "wall", "wall", "wall"), fs=fs, freeze_stall=300,
-               positivity=args.positivity)
+               positivity=args.positivity or 0)
     mesh = Mesh(m.points, m.cells, tile=128, renumber=1, face_budget=256)
     s = Solver(mesh, make_config(roe_bdiss=1, block=128, xcd_remap=1, **cfg))
     s.set_state(np.tile(cons_from_prim(np.array([fs[0]]), fs[1], fs[2], fs[3]), (mesh.nphys, 1)))
@@ -534,18 +536,19 @@ def fv_bench(args, wl, scheme, gradient, sub=False):
     # limiter sees zero differences and skips its division, and every face takes the same flux
     # branch, which would flatter the kernels. The step mesh's impulsive start turns the
     # second-order march non-physical at the step corner within 25 steps, so it is developed with
-    # the positivity fallback on; the timed steps run the plain scheme (--positivity 1 times the
-    # fallback's two-pass stages instead).
+    # the positivity fallback on, and the timed steps keep it on (the plain stages turn the developed
+    # flow non-physical at the corner within a few steps; --positivity 0 times them anyway).
     develop = args.develop if args.develop >= 0 else {"bump": 2000, "ffs": 1000, "ffs_hllc": 1000}[wl]
     ffs = wl.startswith("ffs") and scheme == "muscl"
+    pos = ffs and (args.positivity is None or args.positivity == 1)
     t_dev = time.perf_counter()
     if develop:
         if ffs:
             s.configure(mk(positivity=1, **cfg))
         s.step(develop, stream=stream)
-        if ffs and not args.positivity:
+        if ffs and not pos:
             s.configure(mk(**cfg))
-    elif args.positivity and ffs:
+    elif pos:
         s.configure(mk(positivity=1, **cfg))
     U_dev = s.get_state()
     t_dev = time.perf_counter() - t_dev
@@ -575,9 +578,10 @@ def fv_bench(args, wl, scheme, gradient, sub=False):
     t_step = ms_steps / 1e3 / args.steps
     launch_cfg = (f"workload={wl} tile={args.tile} face_budget={args.face_budget} "
                   f"block={args.block} grad_lds={args.grad_lds}" + (f" gradient={gradient}" if gradient else "")
-                  + (f" scheme={scheme}" if scheme != "muscl" else ""))
+                  + (f" scheme={scheme}" if scheme != "muscl" else "") + (" positivity=1" if pos else ""))
     value = F * nres / t_step
-    bflux = flux_pass_bytes(C, F, cfg)
+    # with the fallback the stage pass is its detecting pass, which also writes one flag byte per cell
+    bflux = flux_pass_bytes(C, F, cfg) + (C if pos else 0)
     achieved = bflux / (ms_flux / 1e3) / 1e9
     bres = resid_pass_bytes(C, F, cfg)
     out = {
@@ -593,13 +597,14 @@ def fv_bench(args, wl, scheme, gradient, sub=False):
         "vs_baseline": None,
         "dtype": "f64",
         "data": f"synthetic (seeded generated mesh; the state marched {develop} steps from freestream, untimed, "
-                f"before the warmup{', with the positivity fallback' if ffs and develop else ''})",
+                f"before the warmup{', with the positivity fallback' if ffs and develop else ''}"
+                f"{'; the timed steps with the positivity fallback' if pos else ''})",
         "value_is": "FP64 face-flux evals/s (unique faces x residual evaluations per s) of whole "
                     f"{'RK4' if nres == 4 else 'TVD-RK3'} steps; the time-to-residual part is the "
                     "time_to_residual object",
         "config": {"workload": desc, "cells": C, "faces": F, "tile": args.tile,
                    "face_budget": args.face_budget, "block": args.block, "grad_lds": args.grad_lds,
-                   "residual_evals_per_step": nres, "positivity_fallback": bool(args.positivity and ffs),
+                   "residual_evals_per_step": nres, "positivity_fallback": pos,
                    "launch": launch_cfg},
         "cell_updates_per_s": C / t_step,
         "wall_s_timed": wall,
@@ -607,7 +612,9 @@ def fv_bench(args, wl, scheme, gradient, sub=False):
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic((wl, scheme), launch_cfg),
-                     "kernel": "flux_tile_kernel (fused face flux + gather + RK stage), first stage",
+                     "kernel": ("flux_tile_kernel, the positivity fallback's detecting pass (fused face flux + "
+                                "gather + RK stage + admissibility flags), first stage" if pos else
+                                "flux_tile_kernel (fused face flux + gather + RK stage), first stage"),
                      "algorithmic_bytes_per_launch": bflux, "avg_launch_ms": ms_flux,
                      "traffic_source": PROFILES.get((wl, scheme))},
         "grad_pass": {"avg_launch_ms": ms_grad,

@@ -68,6 +68,53 @@ __device__ __forceinline__ void cons_from_prim(double rho, double u, double v, d
     U[3] = ((p / kGm1) + rho * vmag);
 }
 
+// Conserved ghost state of the subsonic outflow and inflow boundaries (types 9 and 10) from the
+// interior primitives (r, u, v, p, sound speed c), the outward normal and the prescribed state
+// (f0..f3 = rho, u, v, p; f4 = exit static pressure). Called rather than inlined: its power
+// functions would otherwise raise the register count of every flux and gradient kernel (the
+// step mesh's stage kernel from 108 to 156 VGPRs, 4 to 3 waves per SIMD), where these
+// boundaries are a small share of the faces.
+__device__ __noinline__ double4 ghost_subsonic(int bc, double r, double u, double v, double p, double c,
+                                                     double n1, double n2, double f0, double f1, double f2,
+                                                     double f3, double f4) {
+    double Ug[4];
+    if (bc == BC_SUBSONIC_EXIT) {
+        // back-pressure exit (Subsonic_Exit_Boundary_Condition, src/Exit_Boundary_Conditions.cpp:8-66):
+        // the face state takes the prescribed static pressure and the outgoing characteristics of
+        // the interior (reference density and sound speed: the interior cell's), and the ghost is
+        // 2 U_face - U_cell
+        const double pb = f4;
+        const double t = (p - pb) / (r * c * c);
+        const double rf = r - t * r;
+        const double uf = u + n1 * t * c;
+        const double vf = v + n2 * t * c;
+        double Uf[4], Uc[4];
+        cons_from_prim(rf, uf, vf, pb, Uf);
+        cons_from_prim(r, u, v, p, Uc);
+        for (int q = 0; q < 4; q++) Ug[q] = 2.0 * Uf[q] - Uc[q];
+        double rg, pg;
+        rho_p_of(Ug, rg, pg);
+        if (!(rg > 0.0) || !(pg > 0.0))
+            for (int q = 0; q < 4; q++) Ug[q] = Uf[q];  // a non-physical extrapolation: the face state
+    } else {
+        // total-condition inlet: total pressure, total density and flow direction of the
+        // prescribed state, static pressure from the interior (isentropic; the reference's
+        // subsonic inlet prescribes rho and p instead)
+        const double q2 = f1 * f1 + f2 * f2;
+        const double f = 1.0 + 0.5 * kGm1 * (q2 / (kGamma * f3 / f0));
+        const double p0 = f3 * pow(f, kGamma / kGm1);
+        const double r0 = f0 * pow(f, 1.0 / kGm1);
+        const double qinf = sqrt(q2);
+        const double dx = (qinf > 0.0) ? f1 / qinf : -n1;
+        const double dy = (qinf > 0.0) ? f2 / qinf : -n2;
+        const double pi = fmin(p, p0);
+        const double rb = r0 * pow(pi / p0, 1.0 / kGamma);
+        const double qb = sqrt(fmax((2.0 / kGm1) * (kGamma * p0 / r0 - kGamma * pi / rb), 0.0));
+        cons_from_prim(rb, qb * dx, qb * dy, pi, Ug);
+    }
+    return make_double4(Ug[0], Ug[1], Ug[2], Ug[3]);
+}
+
 // Ghost state across a boundary face with outward unit normal (n1, n2); fs = the prescribed state
 // (rho, u, v, p) and the exit's static pressure.
 __device__ __forceinline__ void ghost_state(int bc, const FState &in, double n1, double n2,
@@ -151,46 +198,18 @@ __device__ __forceinline__ void ghost_state(int bc, const FState &in, double n1,
         cons_from_prim(rb, ub, vb, pb, Ug);
         break;
     }
-    case BC_SUBSONIC_EXIT: {  // back-pressure exit (Subsonic_Exit_Boundary_Condition,
-        // src/Exit_Boundary_Conditions.cpp:8-66): the face state takes the prescribed static
-        // pressure fs[4] and the outgoing characteristics of the interior (reference density and
-        // sound speed: the interior cell's), and the ghost is 2 U_face - U_cell; supersonic
-        // outflow copies the interior state
-        const double r = in.W[0], u = in.W[1], v = in.W[2], p = in.W[3], c = in.W[4];
-        if (u * n1 + v * n2 >= c) {
-            g = in;
+    case BC_SUBSONIC_EXIT:
+    case BC_SUBSONIC_INLET: {
+        if (bc == BC_SUBSONIC_EXIT && in.W[1] * n1 + in.W[2] * n2 >= in.W[4]) {
+            g = in;  // supersonic outflow copies the interior state
             return;
         }
-        const double pb = fs[4];
-        const double t = (p - pb) / (r * c * c);
-        const double rf = r - t * r;
-        const double uf = u + n1 * t * c;
-        const double vf = v + n2 * t * c;
-        double Uf[4], Uc[4];
-        cons_from_prim(rf, uf, vf, pb, Uf);
-        cons_from_prim(r, u, v, p, Uc);
-        for (int q = 0; q < 4; q++) Ug[q] = 2.0 * Uf[q] - Uc[q];
-        double rg, pg;
-        rho_p_of(Ug, rg, pg);
-        if (!(rg > 0.0) || !(pg > 0.0))
-            for (int q = 0; q < 4; q++) Ug[q] = Uf[q];  // a non-physical extrapolation: the face state
-        break;
-    }
-    case BC_SUBSONIC_INLET: {  // total-condition inlet: total pressure, total density and flow
-        // direction of the prescribed state fs[0..3], static pressure from the interior
-        // (isentropic; the reference's subsonic inlet prescribes rho and p instead)
-        const double rinf = fs[0], uinf = fs[1], vinf = fs[2], pinf = fs[3];
-        const double q2 = uinf * uinf + vinf * vinf;
-        const double f = 1.0 + 0.5 * kGm1 * (q2 / (kGamma * pinf / rinf));
-        const double p0 = pinf * pow(f, kGamma / kGm1);
-        const double r0 = rinf * pow(f, 1.0 / kGm1);
-        const double qinf = sqrt(q2);
-        const double dx = (qinf > 0.0) ? uinf / qinf : -n1;
-        const double dy = (qinf > 0.0) ? vinf / qinf : -n2;
-        const double p = fmin(in.W[3], p0);
-        const double rb = r0 * pow(p / p0, 1.0 / kGamma);
-        const double qb = sqrt(fmax((2.0 / kGm1) * (kGamma * p0 / r0 - kGamma * p / rb), 0.0));
-        cons_from_prim(rb, qb * dx, qb * dy, p, Ug);
+        const double4 G = ghost_subsonic(bc, in.W[0], in.W[1], in.W[2], in.W[3], in.W[4], n1, n2, fs[0], fs[1],
+                                         fs[2], fs[3], fs[4]);
+        Ug[0] = G.x;
+        Ug[1] = G.y;
+        Ug[2] = G.z;
+        Ug[3] = G.w;
         break;
     }
     case BC_EXIT:  // Supersonic_Exit_Boundary_Condition (src/Exit_Boundary_Conditions.cpp:68-81)

@@ -24,6 +24,16 @@
 #else
 #define CFD_FLUX_ATTR
 #endif
+//   CFD_POS_WAVES_PER_EU  minimum waves per SIMD of the positivity fallback's detecting pass (0: the
+//                         compiler's choice)
+#ifndef CFD_POS_WAVES_PER_EU
+#define CFD_POS_WAVES_PER_EU 0
+#endif
+#if CFD_POS_WAVES_PER_EU > 0
+#define CFD_POS_ATTR __attribute__((amdgpu_waves_per_eu(POS == 1 ? CFD_POS_WAVES_PER_EU : 1)))
+#else
+#define CFD_POS_ATTR
+#endif
 
 namespace cfd {
 
@@ -364,7 +374,7 @@ __device__ __forceinline__ void flux_face(const DevMesh &m, const DevCfg &cfg,
 // workgroups. With a grid that is a multiple of 8 a workgroup stays on one XCD and keeps that
 // XCD's run of tiles. Without it the loop runs once (one tile per workgroup).
 template <int FLUX, int ORDER, int LIM, int MODE, bool VISC, int POS = 0, bool FZ = false, bool PERSIST = false>
-__global__ void __launch_bounds__(256) CFD_FLUX_ATTR flux_tile_kernel(DevMesh m, DevCfg cfg,
+__global__ void __launch_bounds__(256) CFD_FLUX_ATTR CFD_POS_ATTR flux_tile_kernel(DevMesh m, DevCfg cfg,
                                                         const double4 *__restrict__ Us,
                                                         const double *__restrict__ grad,
                                                         StageArgs st) {
@@ -395,10 +405,10 @@ __global__ void __launch_bounds__(256) CFD_FLUX_ATTR flux_tile_kernel(DevMesh m,
     double *sF = sU + (ORDER != 2 ? 4 * T : 0); // [4][FT] face fluxes
     double *sG = sF + 4 * FT;                   // [8][T] (MUSCL, LDS-staged gradients)
     double *sN = sG + (glds ? 8 * T : 0);       // [3][FT] face normal and length (time steps)
-    double *sNb = sN + 3 * FT;                  // [4][FT] rho, p of both cells (positivity detection)
     // phase C forms time steps from the face geometry: staged here from the face records of
-    // phase B instead of read again from HBM
+    // phase B instead of read again from HBM (the launch sizes LDS for it only where it is used)
     const bool needN = (MODE == MODE_RESID) || (st.lts && st.stage == 0) || (st.last && st.dt_partial);
+    double *sNb = needN ? sN + 3 * FT : sN;     // [4][FT] rho, p of both cells (positivity detection)
     auto stageN = [&](int j, const FaceRec &f) {
         if (needN) {
             sN[j] = f.nx;

@@ -40,6 +40,8 @@ struct StageArgs {
 
 size_t grad_smem_bytes(const DevMesh &m, int grad_type);
 size_t flux_smem_bytes(const DevMesh &m, int order, int grad_lds);
+// LDS of one flux launch (geom: the launch forms time steps and stages the face geometry).
+size_t flux_launch_smem(const DevMesh &m, int order, int grad_lds, int block, bool geom);
 // Positivity fallback (order 2, MODE_STAGE, inviscid; fv_pos.hip): pass 1 = the stage with
 // detection, pass 2 = the recomputation of marked tiles. Inputs and outputs of a stage must not
 // alias (pass 2 reads the stage's inputs again).

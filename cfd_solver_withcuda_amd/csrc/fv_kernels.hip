@@ -607,7 +607,7 @@ int persist_grid(const void *kernel, const DevMesh &m, const DevCfg &c, int bloc
 
 // LDS of one flux launch: the face-geometry copy (3 doubles per face) only when the launch forms
 // time steps (the kernel's needN), at least the block reductions' 4 doubles per thread.
-static size_t flux_launch_smem(const DevMesh &m, int order, int grad_lds, int block, bool geom) {
+size_t flux_launch_smem(const DevMesh &m, int order, int grad_lds, int block, bool geom) {
     size_t d = 5 * (size_t)m.tile + (order != 2 ? 4 * (size_t)m.tile : 0) +
                (geom ? 7 : 4) * (size_t)m.max_tile_faces;
     if (order >= 2 && grad_lds) d += 8 * (size_t)m.tile;

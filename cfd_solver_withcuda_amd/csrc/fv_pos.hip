@@ -12,8 +12,13 @@ namespace {
 template <int FLUX, int LIM, int PASS>
 void launch_pos_t(const DevMesh &m, const DevCfg &c, const double4 *Us, const double *grad,
                   const StageArgs &st, int block, size_t sm, hipStream_t s) {
-    hipLaunchKernelGGL((flux_tile_kernel<FLUX, 2, LIM, MODE_STAGE, false, PASS, true>), dim3(grid_of(m, c)),
-                       dim3(block), sm, s, m, c, Us, grad, st);
+    // the freezing code only where a freezing state is set: it costs registers
+    if (st.freeze)
+        hipLaunchKernelGGL((flux_tile_kernel<FLUX, 2, LIM, MODE_STAGE, false, PASS, true>), dim3(grid_of(m, c)),
+                           dim3(block), sm, s, m, c, Us, grad, st);
+    else
+        hipLaunchKernelGGL((flux_tile_kernel<FLUX, 2, LIM, MODE_STAGE, false, PASS, false>), dim3(grid_of(m, c)),
+                           dim3(block), sm, s, m, c, Us, grad, st);
 }
 
 template <int PASS>
@@ -43,7 +48,8 @@ hipError_t launch_pos_pass(const DevMesh &m, const DevCfg &c, int flux, int lim,
 hipError_t launch_flux_pos(const DevMesh &m, const DevCfg &c, int flux, int lim, int pass,
                            const double4 *Us, const double *grad, const StageArgs &st, int block,
                            hipStream_t s) {
-    size_t sm = flux_smem_bytes(m, 2, c.grad_lds);
+    const bool geom = (st.lts && st.stage == 0) || (st.last && st.dt_partial);
+    size_t sm = flux_launch_smem(m, 2, c.grad_lds, block, geom);
     if (pass == 1) {
         // the detection's per-face densities and pressures follow the face geometry in LDS
         sm += sizeof(double) * 4 * (size_t)m.max_tile_faces;

@@ -45,6 +45,18 @@ def counter_per_dispatch(path, kernel, counter, ordinals=None):
     return sum(vals) / len(vals), len(vals)
 
 
+def durations(path, kernel):
+    """Durations (us) of the dispatches whose kernel name contains `kernel`, in dispatch order,
+    from a rocprofv3 kernel-trace CSV."""
+    rows = []
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            if kernel in r["Kernel_Name"]:
+                rows.append((int(r["Dispatch_Id"]), (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3))
+    rows.sort()
+    return [d for _, d in rows]
+
+
 def short(name):
     return name.replace("HIP_vector_type<double, 4u>", "double4").replace("cfd::", "")
 
@@ -65,6 +77,11 @@ def main(argv=None):
     p.add_argument("--ordinals", default="",
                    help="start:stop of the kernel's dispatches to average (default: all)")
     p.add_argument("--config", default="")
+    p.add_argument("--trace", help="kernel-trace CSV of the --stats run: durations of single dispatches")
+    p.add_argument("--trace-ordinals", default="",
+                   help="start:stop of the kernel's dispatches in the trace that bench.py times alone")
+    p.add_argument("--stages", default="",
+                   help="start:stop:stride: per-stage average duration over those dispatches")
     p.add_argument("--extra-kernel", action="append", default=[],
                    help="NAME=ALGORITHMIC_BYTES: a further kernel's traffic (all its dispatches)")
     p.add_argument("--note", action="append", default=[])
@@ -77,6 +94,18 @@ def main(argv=None):
     for r in read_stats(a.stats):
         out.append(f"| `{short(r['name'])}` | {r['calls']} | {r['avg_ns'] / 1e3:.1f} | "
                    f"{r['min_ns'] / 1e3:.1f} | {r['max_ns'] / 1e3:.1f} | {r['pct']:.2f} |")
+    if a.trace and a.trace_ordinals:
+        d = durations(a.trace, a.kernel)
+        t0, t1 = (int(x) for x in a.trace_ordinals.split(":"))
+        sel = d[t0:t1]
+        out += ["", f"## Launches of `{a.kernel}` timed alone (rocprofv3 kernel trace, dispatches {t0}-{t1 - 1})", "",
+                f"- {len(sel)} launches, average {sum(sel) / len(sel):.1f} us "
+                f"({', '.join(f'{x:.1f}' for x in sel)})"]
+        if a.stages:
+            s0, s1, k = (int(x) for x in a.stages.split(":"))
+            means = [sum(d[s0 + j:s1:k]) / len(d[s0 + j:s1:k]) for j in range(k)]
+            out.append(f"- average per stage over dispatches {s0}-{s1 - 1}: "
+                       + ", ".join(f"stage {j} {m:.1f} us" for j, m in enumerate(means)))
     if a.fetch and a.write and a.copy_fetch and a.copy_write:
         ords = tuple(int(x) for x in a.ordinals.split(":")) if a.ordinals else None
         f_kib, nf = counter_per_dispatch(a.fetch, a.kernel, "FETCH_SIZE", ords)

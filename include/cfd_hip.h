@@ -175,7 +175,8 @@ int cfd_run_steady(cfd_solver_t *s, double tol, int max_steps, int check_every, 
                    double *err4, void *stream);
 
 /* Times `reps` back-to-back launches on `stream` with HIP events: which = 0 the gradient pass,
- * 1 the fused flux/stage pass (first stage, output to scratch; the state is unchanged),
+ * 1 the fused flux/stage pass (first stage, output to scratch; the state is unchanged; with the
+ * positivity fallback its detecting pass),
  * 2 whole steps (advances the state), 3 the flux pass in residual mode (R and the local time
  * step out, as cfd_residual). *ms = total elapsed milliseconds. */
 int cfd_time_pass(cfd_solver_t *s, int which, int reps, float *ms, void *stream);

@@ -124,3 +124,30 @@ def test_fallback_configuration_is_checked():
                        rk=4, lts=0, block=128, positivity=1), **bad)
         with pytest.raises(Exception):
             Solver(mesh, make_config(**kw))
+
+
+def test_fallback_with_limiter_freezing_matches_oracle(orc):
+    """The fallback's passes with a freezing state set (the kernel variants that carry the frozen
+    limiter factors) against the oracle: 4 plain, 1 recording and 5 frozen LTS RK3 steps of the
+    step mesh from the impulsive start, HLLC + MUSCL / Van Leer + WLSQ."""
+    print("case positivity + freezing step mesh", flush=True)
+    m, fs, U0 = step_mesh()
+    kw = dict(flux="hllc", order=2, limiter="vanleer", gradient="wlsq", roe_bdiss=1, cfl=0.4, bc=STEP_BCS, fs=fs)
+    om = orc.mesh(m.points, m.cells)
+    s = Solver(Mesh(m.points, m.cells, tile=128, face_budget=256),
+               make_config(rk=3, lts=1, block=128, positivity=1, **kw))
+    s.set_state(U0)
+    Uo = U0.copy()
+    ocfg = make_cfg(positivity=1, **kw)
+    phi = np.zeros(8 * om.nfaces, np.float32)
+    ocfg.phi = phi.ctypes.data
+    flagged = 0
+    for mode, n in ((0, 4), (1, 1), (2, 5)):
+        s.set_freeze(mode)
+        s.step(n)
+        ocfg.freeze = mode
+        for _ in range(n):
+            om.step(ocfg, Uo, rk=3, lts=1)
+            flagged += orc.last_fallback_cells()
+        assert np.array_equal(s.get_state(), Uo), (mode, rel(s.get_state(), Uo))
+    assert flagged > 0
