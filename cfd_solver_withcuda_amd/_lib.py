@@ -97,6 +97,7 @@ def lib() -> ctypes.CDLL:
     _sig(L, "cfd_mlp_train", _i, [_vp, _fp, _fp, _i, _i, _i, ctypes.c_float, ctypes.c_float,
                                   ctypes.c_ulonglong, _fp, _i])
     _sig(L, "cfd_mlp_predict", _i, [_vp, _fp, _i, _fp])
+    _sig(L, "cfd_mlp_set_launch", _i, [_vp, _i])
     _sig(L, "cfd_mlp_time_step", _i, [_vp, _i, _i, _fp])
     _sig(L, "cfd_run_steady", _i, [_vp, _d, _i, _i, _ip, _dp, _vp])
     _sig(L, "cfd_time_pass", _i, [_vp, _i, _i, ctypes.POINTER(ctypes.c_float), _vp])

@@ -58,6 +58,10 @@ struct cfd_mlp_s {
     int *idx = nullptr;
     size_t data_cap = 0;
     int adam_t = 0;
+    int graph_mode = 1;             // training steps: 1 one captured graph per step, 0 launch by launch
+    float *sched = nullptr;         // [3 * steps] learning rate and Adam bias corrections per step
+    size_t sched_cap = 0;
+    int *d_step = nullptr;          // step of the captured graph's next replay
     hipStream_t stream = nullptr;
     std::vector<void *> allocs;
     ~cfd_mlp_s() {
@@ -65,6 +69,7 @@ struct cfd_mlp_s {
         for (void *p : allocs) (void)hipFree(p);
         if (feat) (void)hipFree(feat);
         if (targ) (void)hipFree(targ);
+        if (sched) (void)hipFree(sched);
     }
     template <typename T>
     hipError_t alloc(T **p, size_t n) {
@@ -176,7 +181,8 @@ int cfd_mlp_create(int n_in, int width, int depth, int n_out, int max_rows, unsi
               (e = m->alloc(&m->part, (size_t)kSplit * maxMK)) == hipSuccess &&
               (e = m->alloc(&m->gpad, (size_t)maxMK)) == hipSuccess &&
               (e = m->alloc(&m->norm, 2 * (size_t)(n_in + n_out))) == hipSuccess &&
-              (e = m->alloc(&m->lrow, (size_t)R)) == hipSuccess && (e = m->alloc(&m->idx, (size_t)R)) == hipSuccess;
+              (e = m->alloc(&m->lrow, (size_t)R)) == hipSuccess && (e = m->alloc(&m->idx, (size_t)R)) == hipSuccess &&
+              (e = m->alloc(&m->d_step, 1)) == hipSuccess;
     m->Wb.resize(m->L);
     m->Wt.resize(m->L);
     m->X.resize(m->L);
@@ -305,6 +311,76 @@ int cfd_mlp_train(cfd_mlp_t *m, const float *feat, const float *targ, int n, int
     const int Lm = m->L - 1;
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
     std::vector<float> lrow(batch);
+    // cosine decay from lr to lr_final; Adam's bias corrections of its global step count
+    std::vector<float> sch(3 * (size_t)std::max(steps, 1));
+    for (int s = 0; s < steps; s++) {
+        const int t = m->adam_t + s + 1;
+        const double f = steps > 1 ? 0.5 * (1.0 + std::cos(M_PI * s / (double)(steps - 1))) : 1.0;
+        sch[3 * (size_t)s] = (float)(lr_final + (lr - lr_final) * f);
+        sch[3 * (size_t)s + 1] = 1.0f / (1.0f - std::pow(b1, (float)t));
+        sch[3 * (size_t)s + 2] = 1.0f / (1.0f - std::pow(b2, (float)t));
+    }
+    auto log_loss = [&](int s) -> int {
+        if (loss_hist && s % log_every == 0) {
+            MHIP(hipMemcpyAsync(lrow.data(), m->lrow, batch * sizeof(float), hipMemcpyDeviceToHost, m->stream));
+            MHIP(hipStreamSynchronize(m->stream));
+            double acc = 0.0;
+            for (float v : lrow) acc += v;
+            loss_hist[s / log_every] = (float)(acc / batch);
+        }
+        return 0;
+    };
+    if (m->graph_mode && steps > 0) {
+        // one training step captured once and replayed: the step index, learning rate and bias
+        // corrections come from device memory, so every replay is the same launch sequence
+        if ((size_t)steps > m->sched_cap) {
+            if (m->sched) MHIP(hipFree(m->sched));
+            m->sched = nullptr;
+            MHIP(hipMalloc((void **)&m->sched, 3 * (size_t)steps * sizeof(float)));
+            m->sched_cap = steps;
+        }
+        MHIP(hipMemcpy(m->sched, sch.data(), 3 * (size_t)steps * sizeof(float), hipMemcpyHostToDevice));
+        const int zero = 0;
+        MHIP(hipMemcpy(m->d_step, &zero, sizeof(int), hipMemcpyHostToDevice));
+        auto launches = [&]() -> int {
+            MHIP(mlp_sample_dev(seed, m->d_step, batch, n, m->idx, m->stream));
+            MHIP(mlp_encode(m->feat, m->n_in, m->idx, batch, nrm(m, 0), nrm(m, 1), m->Kp[0], m->X[0], m->Xt[0],
+                            batch, m->stream));
+            if (int r = forward(m, batch)) return r;
+            MHIP(mlp_loss(m->Z[Lm], m->Mp[Lm], m->targ, m->n_out, m->idx, batch, nrm(m, 2), nrm(m, 3), m->dZ[Lm],
+                          m->dZt[Lm], batch, m->lrow, m->stream));
+            if (int r = backward(m, batch)) return r;
+            MHIP(mlp_adam_dev(m->W, m->mW, m->vW, m->gW, (int)m->nw, m->sched, m->d_step, b1, b2, eps, m->stream));
+            MHIP(mlp_adam_dev(m->b, m->mb, m->vb, m->gb, (int)m->nb, m->sched, m->d_step, b1, b2, eps, m->stream));
+            if (int r = pack_all(m)) return r;
+            MHIP(mlp_advance(m->d_step, m->stream));
+            return 0;
+        };
+        MHIP(hipStreamSynchronize(m->stream));
+        MHIP(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
+        rc = launches();
+        hipGraph_t g = nullptr;
+        hipError_t ec = hipStreamEndCapture(m->stream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (ec != hipSuccess) return capi_hip_fail(ec, "cfd_mlp_train capture");
+        hipGraphExec_t ge = nullptr;
+        hipError_t ei = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) return capi_hip_fail(ei, "cfd_mlp_train instantiate");
+        for (int s = 0; s < steps && !rc; s++) {
+            hipError_t el = hipGraphLaunch(ge, m->stream);
+            if (el != hipSuccess) rc = capi_hip_fail(el, "cfd_mlp_train graph launch");
+            else rc = log_loss(s);
+        }
+        (void)hipStreamSynchronize(m->stream);
+        (void)hipGraphExecDestroy(ge);
+        if (rc) return rc;
+        m->adam_t += steps;
+        return 0;
+    }
     for (int s = 0; s < steps; s++) {
         MHIP(mlp_sample(seed, s, batch, n, m->idx, m->stream));
         MHIP(mlp_encode(m->feat, m->n_in, m->idx, batch, nrm(m, 0), nrm(m, 1), m->Kp[0], m->X[0], m->Xt[0], batch,
@@ -316,24 +392,20 @@ int cfd_mlp_train(cfd_mlp_t *m, const float *feat, const float *targ, int n, int
         rc = backward(m, batch);
         if (rc) return rc;
         m->adam_t++;
-        // cosine decay from lr to lr_final
-        const double f = steps > 1 ? 0.5 * (1.0 + std::cos(M_PI * s / (double)(steps - 1))) : 1.0;
-        const float lrs = (float)(lr_final + (lr - lr_final) * f);
-        const float c1 = 1.0f / (1.0f - std::pow(b1, (float)m->adam_t));
-        const float c2 = 1.0f / (1.0f - std::pow(b2, (float)m->adam_t));
+        const float lrs = sch[3 * (size_t)s], c1 = sch[3 * (size_t)s + 1], c2 = sch[3 * (size_t)s + 2];
         MHIP(mlp_adam(m->W, m->mW, m->vW, m->gW, (int)m->nw, lrs, b1, b2, eps, c1, c2, m->stream));
         MHIP(mlp_adam(m->b, m->mb, m->vb, m->gb, (int)m->nb, lrs, b1, b2, eps, c1, c2, m->stream));
         rc = pack_all(m);
         if (rc) return rc;
-        if (loss_hist && s % log_every == 0) {
-            MHIP(hipMemcpyAsync(lrow.data(), m->lrow, batch * sizeof(float), hipMemcpyDeviceToHost, m->stream));
-            MHIP(hipStreamSynchronize(m->stream));
-            double acc = 0.0;
-            for (float v : lrow) acc += v;
-            loss_hist[s / log_every] = (float)(acc / batch);
-        }
+        if ((rc = log_loss(s))) return rc;
     }
     MHIP(hipStreamSynchronize(m->stream));
+    return 0;
+}
+
+int cfd_mlp_set_launch(cfd_mlp_t *m, int mode) {
+    if (!m || (mode != 0 && mode != 1)) return capi_fail(CFD_E_ARG, "cfd_mlp_set_launch: mode must be 0 or 1");
+    m->graph_mode = mode;
     return 0;
 }
 

@@ -28,5 +28,11 @@ hipError_t mlp_adam(float *W, float *m1, float *m2, const float *g, int n, float
 hipError_t mlp_pack(const float *W, int M, int K, int Mp, int Kp, uint16_t *Wb, uint16_t *Wt, hipStream_t s);
 hipError_t mlp_unpad(const float *P, int Kp, int M, int K, float *out, hipStream_t s);
 hipError_t mlp_sample(unsigned long long seed, int step, int rows, int n, int *idx, hipStream_t s);
+// Step-indexed forms for a captured training step: the step is *d_step on the device, Adam's
+// learning rate and bias corrections are sched[3 step .. 3 step + 2], mlp_advance adds one to it.
+hipError_t mlp_sample_dev(unsigned long long seed, const int *d_step, int rows, int n, int *idx, hipStream_t s);
+hipError_t mlp_adam_dev(float *W, float *m1, float *m2, const float *g, int n, const float *sched,
+                        const int *d_step, float b1, float b2, float eps, hipStream_t s);
+hipError_t mlp_advance(int *d_step, hipStream_t s);
 
 }  // namespace cfd

@@ -235,7 +235,59 @@ __global__ void __launch_bounds__(256) sample_kernel(unsigned long long seed, in
     idx[r] = (int)(z % (unsigned long long)n);
 }
 
+// The device-side step of a captured training step: the minibatch of step *d_step, and Adam with
+// that step's learning rate and bias corrections sched[3 step .. 3 step + 2]; advance_kernel moves
+// to the next step at the end of the captured step.
+__global__ void __launch_bounds__(256) sample_dev_kernel(unsigned long long seed, const int *__restrict__ d_step,
+                                                         int rows, int n, int *__restrict__ idx) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    const int step = *d_step;
+    unsigned long long z = seed + 0x9E3779B97F4A7C15ull * ((unsigned long long)step * (unsigned long long)rows + r + 1);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    idx[r] = (int)(z % (unsigned long long)n);
+}
+
+__global__ void __launch_bounds__(256) adam_dev_kernel(float *__restrict__ W, float *__restrict__ m1,
+                                                       float *__restrict__ m2, const float *__restrict__ g,
+                                                       int n, const float *__restrict__ sched,
+                                                       const int *__restrict__ d_step, float b1, float b2,
+                                                       float eps) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int step = *d_step;
+    const float lr = sched[3 * step], c1 = sched[3 * step + 1], c2 = sched[3 * step + 2];
+    const float gi = g[i];
+    const float a = b1 * m1[i] + (1.0f - b1) * gi;
+    const float b = b2 * m2[i] + (1.0f - b2) * gi * gi;
+    m1[i] = a;
+    m2[i] = b;
+    W[i] -= lr * (a * c1) / (sqrtf(b * c2) + eps);
+}
+
+__global__ void __launch_bounds__(64) advance_kernel(int *d_step) {
+    if (threadIdx.x == 0) *d_step += 1;
+}
+
 }  // namespace
+
+hipError_t mlp_sample_dev(unsigned long long seed, const int *d_step, int rows, int n, int *idx, hipStream_t s) {
+    hipLaunchKernelGGL(sample_dev_kernel, dim3((rows + 255) / 256), dim3(256), 0, s, seed, d_step, rows, n, idx);
+    return hipGetLastError();
+}
+
+hipError_t mlp_adam_dev(float *W, float *m1, float *m2, const float *g, int n, const float *sched,
+                        const int *d_step, float b1, float b2, float eps, hipStream_t s) {
+    hipLaunchKernelGGL(adam_dev_kernel, dim3((n + 255) / 256), dim3(256), 0, s, W, m1, m2, g, n, sched, d_step, b1, b2, eps);
+    return hipGetLastError();
+}
+
+hipError_t mlp_advance(int *d_step, hipStream_t s) {
+    hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(64), 0, s, d_step);
+    return hipGetLastError();
+}
 
 hipError_t mlp_gemm(int epi, const uint16_t *A, int lda, const uint16_t *B, int ldb, int R, int C,
                     int K, int ksplit, const float *bias, const float *Zprev, float *Zout,

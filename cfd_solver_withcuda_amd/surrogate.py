@@ -78,6 +78,11 @@ class MLP:
                                   fptr(hist), log_every), "cfd_mlp_train")
         return hist
 
+    def set_launch(self, mode):
+        """Training steps: "graph" (one captured step, replayed; default) or "kernels"."""
+        m = {"kernels": 0, "graph": 1}[mode] if isinstance(mode, str) else int(mode)
+        check(lib().cfd_mlp_set_launch(self.h, m), "cfd_mlp_set_launch")
+
     def predict(self, feat):
         feat = np.ascontiguousarray(feat, np.float32)
         out = np.zeros((len(feat), self.n_out), np.float32)

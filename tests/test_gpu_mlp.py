@@ -81,3 +81,28 @@ def test_training_fits_a_smooth_field():
     err = np.abs(pred - test_targ).mean(axis=0) / np.abs(test_targ).mean(axis=0)
     print("relative mean abs error per output", err, flush=True)
     assert np.all(err < 0.05)
+
+
+def test_captured_training_step_equals_kernel_launches():
+    """Training from one captured step replayed (the default) gives the same weights and loss
+    history, bit for bit, as launching each kernel, over two train calls in a row (the second
+    continues Adam's step count, with a smaller batch and a different data set); the host does its
+    own synchronised work between the calls."""
+    feat, targ = data(12000, seed=7)
+    feat2, targ2 = data(5000, seed=8)
+    out = []
+    for mode in ("kernels", "graph"):
+        m = MLP(4, 256, 6, 4, max_rows=4096, seed=21)
+        m.fit_normalization(feat, targ)
+        m.set_launch(mode)
+        h1 = m.train(feat, targ, steps=30, batch=2048, lr=2e-3, lr_final=1e-4, seed=3, log_every=5)
+        W1, b1 = m.get_params()
+        p1 = m.predict(feat[:600])
+        h2 = m.train(feat2, targ2, steps=17, batch=1024, lr=1e-3, lr_final=1e-4, seed=4, log_every=4)
+        W2, b2 = m.get_params()
+        out.append((h1, W1, b1, p1, h2, W2, b2))
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
+    assert not np.array_equal(out[0][1], out[0][5])
+    with pytest.raises(Exception):
+        MLP(4, 32, 1, 4, max_rows=512).set_launch(2)
