@@ -113,7 +113,10 @@ def load_model(path):
     return mlp
 
 
-def evaluate(model, out_file, first=0, count=256, tol=1e-8, max_steps=40000, field_tol=None):
+def evaluate(model, out_file, first=0, count=256, tol=1e-8, max_steps=60000, freeze_warm=True, snap=0.02):
+    """freeze_warm: warm-started instances record their limiter factors on the first step and march
+    frozen (else they freeze on a stall, as the plain march does). snap: cells predicted within
+    that relative distance of the freestream state start at it exactly (predicted_state)."""
     sch = scheme()
     mlp = load_model(model)
     t0 = time.perf_counter()
@@ -128,7 +131,7 @@ def evaluate(model, out_file, first=0, count=256, tol=1e-8, max_steps=40000, fie
     plain = b.get_state()
     log(f"plain: {rp['wall_s']:.1f} s, {(rp['status'] == 0).sum()} converged")
     tw = time.perf_counter()
-    warm = [predicted_state(mlp, c.mesh.points, c.mesh.cells, c.mach, c.param) for c in ev]
+    warm = [predicted_state(mlp, c.mesh.points, c.mesh.cells, c.mach, c.param, fs=c.fs, snap=snap) for c in ev]
     t_inf = time.perf_counter() - tw
     b.set_state(warm)
     e_w = b.solve(tol=-1.0, max_steps=1, check_every=1)["err"][:, 0]
@@ -137,7 +140,7 @@ def evaluate(model, out_file, first=0, count=256, tol=1e-8, max_steps=40000, fie
     use = np.isfinite(e_w) & (e_w <= e_f)
     b.set_state([w if u else f for w, f, u in zip(warm, fs_states, use)])
     if sch["freeze_stall"] > 0:
-        b.set_freeze(use.astype(np.int32))
+        b.set_freeze(use.astype(np.int32) if freeze_warm else None)
     t_check = time.perf_counter() - tw - t_inf
     rw = b.solve(tol=tol, max_steps=max_steps, check_every=100, progress=progress, progress_every=50)
     warm_wall = time.perf_counter() - tw
@@ -146,8 +149,9 @@ def evaluate(model, out_file, first=0, count=256, tol=1e-8, max_steps=40000, fie
     both = (rp["status"] == 0) & (rw["status"] == 0)
     s_err, d_max, d_l2 = [], [], []
     for k in range(len(ev)):
-        Wp = prim_from_cons(plain[k])
-        s_err.append(float(np.linalg.norm(prim_from_cons(warm[k]) - Wp) / np.linalg.norm(Wp)))
+        if rp["status"][k] == 0:  # surrogate error against the converged plain field
+            Wp = prim_from_cons(plain[k])
+            s_err.append(float(np.linalg.norm(prim_from_cons(warm[k]) - Wp) / np.linalg.norm(Wp)))
         if both[k]:
             d_max.append(float(np.abs(warm_states[k] - plain[k]).max() / np.abs(plain[k]).max()))
             d_l2.append(float(np.linalg.norm(warm_states[k] - plain[k]) / np.linalg.norm(plain[k])))
@@ -159,7 +163,9 @@ def evaluate(model, out_file, first=0, count=256, tol=1e-8, max_steps=40000, fie
                          converged=int((rw["status"] == 0).sum()), steps_max=int(rw["steps"].max()),
                          steps_sum=int(rw["steps"].sum()), fallbacks=int((~use).sum())),
                speedup=rp["wall_s"] / warm_wall,
-               surrogate_rel_l2=dict(mean=float(np.mean(s_err)), max=float(np.max(s_err))),
+               freeze_warm=bool(freeze_warm), snap=snap,
+               surrogate_rel_l2=dict(instances=len(s_err), mean=float(np.mean(s_err)) if s_err else None,
+                                     max=float(np.max(s_err)) if s_err else None),
                converged_fields=dict(instances=int(both.sum()), max_rel_diff=max(d_max) if d_max else None,
                                      rel_l2_mean=float(np.mean(d_l2)) if d_l2 else None,
                                      rel_l2_max=max(d_l2) if d_l2 else None))
@@ -169,8 +175,10 @@ def evaluate(model, out_file, first=0, count=256, tol=1e-8, max_steps=40000, fie
                                      plain_steps=int(rp["steps"][both].sum()), warm_steps=int(rw["steps"][both].sum()))
         out["both_converged"]["speedup"] = out["both_converged"]["plain_s"] / out["both_converged"]["warm_s"]
     out["instances"] = [dict(mach=round(c.mach, 4), thickness=round(c.param, 4), cells=len(c.mesh.cells),
-                             plain=[int(a), int(s1)], warm=[int(w), int(s2)], used_surrogate=bool(u))
-                        for c, a, s1, w, s2, u in zip(ev, rp["status"], rp["steps"], rw["status"], rw["steps"], use)]
+                             plain=[int(a), int(s1), float(e1)], warm=[int(w), int(s2), float(e2)],
+                             used_surrogate=bool(u))
+                        for c, a, s1, e1, w, s2, e2, u in zip(ev, rp["status"], rp["steps"], rp["err"][:, 0],
+                                                             rw["status"], rw["steps"], rw["err"][:, 0], use)]
     with open(out_file, "w") as f:
         json.dump(out, f)
     log(json.dumps({k: v for k, v in out.items() if k != "instances"}))
@@ -192,12 +200,15 @@ def main(argv=None):
     e.add_argument("--out", required=True)
     e.add_argument("--first", type=int, default=0)
     e.add_argument("--count", type=int, default=256)
-    e.add_argument("--max-steps", type=int, default=40000)
+    e.add_argument("--max-steps", type=int, default=60000)
+    e.add_argument("--freeze-warm", type=int, default=1)
+    e.add_argument("--snap", type=float, default=0.02)
     a = p.parse_args(argv)
     if a.cmd == "train":
         train(a.out, n_cases=a.cases, cells=a.cells, chunk=a.chunk, mlp_steps=a.mlp_steps, max_steps=a.max_steps)
     else:
-        evaluate(a.model, a.out, first=a.first, count=a.count, max_steps=a.max_steps)
+        evaluate(a.model, a.out, first=a.first, count=a.count, max_steps=a.max_steps, freeze_warm=bool(a.freeze_warm),
+                 snap=a.snap)
     return 0
 
 

@@ -129,8 +129,26 @@ __global__ void __launch_bounds__(256) row_sum_kernel(const uint16_t *__restrict
                                                       float *__restrict__ out) {
     __shared__ float red[256];
     const int r = blockIdx.x, t = threadIdx.x;
-    float s = 0.f;
-    for (int j = t; j < n; j += 256) s += bf2f(X[(size_t)r * ld + j]);
+    // eight values per thread and step (one 16-byte load), four steps in flight; n is a multiple
+    // of 512, the partial sums combine in a fixed order
+    const uint16_t *row = X + (size_t)r * ld;
+    float s4[4] = {0.f, 0.f, 0.f, 0.f};
+    int j = 8 * t;
+    for (; j + 3 * 2048 < n; j += 4 * 2048)
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const uint4 v = *reinterpret_cast<const uint4 *>(row + j + u * 2048);
+            const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int q = 0; q < 4; q++) s4[u] += bf2f((uint16_t)(w[q] & 0xffff)) + bf2f((uint16_t)(w[q] >> 16));
+        }
+    for (; j < n; j += 2048) {
+        const uint4 v = *reinterpret_cast<const uint4 *>(row + j);
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int q = 0; q < 4; q++) s4[0] += bf2f((uint16_t)(w[q] & 0xffff)) + bf2f((uint16_t)(w[q] >> 16));
+    }
+    const float s = (s4[0] + s4[1]) + (s4[2] + s4[3]);
     red[t] = s;
     __syncthreads();
     for (int h = 128; h > 0; h >>= 1) {

@@ -125,12 +125,21 @@ def cons_from_prim_arr(W):
     return np.column_stack([rho, rho * u, rho * v, p / (GAMMA - 1.0) + 0.5 * rho * (u * u + v * v)])
 
 
-def predicted_state(mlp: MLP, points, cells, mach, thickness, floor=1e-3):
+def predicted_state(mlp: MLP, points, cells, mach, thickness, floor=1e-3, fs=None, snap=0.0):
     """Conserved FP64 state from the surrogate's primitive prediction (density and pressure
-    floored so the state is admissible)."""
+    floored so the state is admissible). With `fs` (rho, u, v, p) and snap > 0, cells whose
+    prediction lies within `snap` of that state (relative to rho, |u| and p of it) take it exactly:
+    ahead of the leading shock a supersonic flow is that state, and the network's small errors
+    there would otherwise have to be convected out of the domain."""
     W = mlp.predict(features(points, cells, mach, thickness)).astype(np.float64)
     W[:, 0] = np.maximum(W[:, 0], floor)
     W[:, 3] = np.maximum(W[:, 3], floor)
+    if fs is not None and snap > 0.0:
+        r, u, v, p = (float(x) for x in fs)
+        q = max(np.hypot(u, v), 1e-300)
+        near = ((np.abs(W[:, 0] - r) <= snap * r) & (np.abs(W[:, 1] - u) <= snap * q) &
+                (np.abs(W[:, 2] - v) <= snap * q) & (np.abs(W[:, 3] - p) <= snap * p))
+        W[near] = (r, u, v, p)
     return cons_from_prim_arr(W)
 
 

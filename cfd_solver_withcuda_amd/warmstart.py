@@ -65,7 +65,7 @@ def training_set(cs, states):
 def run(n_train=64, n_eval=32, train_cells=30000, eval_cells=60000, tol=1e-8, train_tol=1e-8,
         max_steps=60000, check_every=100, mlp_steps=3000, mlp_batch=16384, seed=5, cfl=None,
         freeze_warm=True, warm_cfl=None, history=False, log=print, limiter=None, freeze_stall=None,
-        field_tol=None):
+        field_tol=None, snap=0.02):
     out = {}
     over = dict(limiter=limiter, freeze_stall=freeze_stall)
     sch = scheme(cfl, **over)
@@ -107,7 +107,7 @@ def run(n_train=64, n_eval=32, train_cells=30000, eval_cells=60000, tol=1e-8, tr
     log(f"plain: {out['plain']}")
     # warm: inference, fallback test on the first step's residual, march
     tw = time.perf_counter()
-    warm = [predicted_state(mlp, c.mesh.points, c.mesh.cells, c.mach, c.param) for c in ev]
+    warm = [predicted_state(mlp, c.mesh.points, c.mesh.cells, c.mach, c.param, fs=c.fs, snap=snap) for c in ev]
     t_inf = time.perf_counter() - tw
     b.set_state(warm)
     e_w = b.solve(tol=-1.0, max_steps=1, check_every=1)["err"][:, 0]
