@@ -252,11 +252,12 @@ __device__ __forceinline__ void flux_face(const DevMesh &m, const DevCfg &cfg,
     if constexpr (POS == 1) {
         // positivity fallback, detection: density and pressure of the face's two cells (the
         // cell's own values on a boundary face, whose other side is a ghost) for the gather's
-        // neighbourhood range
-        sNb[j] = SL.W[0];
-        sNb[FT + j] = SL.W[3];
-        sNb[2 * FT + j] = boundary ? SL.W[0] : SR.W[0];
-        sNb[3 * FT + j] = boundary ? SL.W[3] : SR.W[3];
+        // neighbourhood range, rounded to FP32 (the range's margin is 1e-3 relative)
+        float *nb = reinterpret_cast<float *>(sNb);
+        nb[j] = (float)SL.W[0];
+        nb[FT + j] = (float)SL.W[3];
+        nb[2 * FT + j] = (float)(boundary ? SL.W[0] : SR.W[0]);
+        nb[3 * FT + j] = (float)(boundary ? SL.W[3] : SR.W[3]);
     }
     double F[4];
     if constexpr (ORDER == 3) {
@@ -408,7 +409,7 @@ __global__ void __launch_bounds__(256) CFD_FLUX_ATTR CFD_POS_ATTR flux_tile_kern
     // phase C forms time steps from the face geometry: staged here from the face records of
     // phase B instead of read again from HBM (the launch sizes LDS for it only where it is used)
     const bool needN = (MODE == MODE_RESID) || (st.lts && st.stage == 0) || (st.last && st.dt_partial);
-    double *sNb = needN ? sN + 3 * FT : sN;     // [4][FT] rho, p of both cells (positivity detection)
+    double *sNb = needN ? sN + 3 * FT : sN;     // float [4][FT] rho, p of both cells (positivity detection)
     auto stageN = [&](int j, const FaceRec &f) {
         if (needN) {
             sN[j] = f.nx;
@@ -555,14 +556,15 @@ __global__ void __launch_bounds__(256) CFD_FLUX_ATTR CFD_POS_ATTR flux_tile_kern
                 // inputs (the same test as the oracle's stage_fallback_flags)
                 double rn, pn;
                 rho_p_of(Uo, rn, pn);
-                double rlo = sW[k], rhi = rlo, plo = sW[3 * T + k], phi = plo;
+                const float *nb = reinterpret_cast<const float *>(sNb);
+                double rlo = (double)(float)sW[k], rhi = rlo, plo = (double)(float)sW[3 * T + k], phi = plo;
                 for (int f = 0; f < 4; f++) {
                     const uint16_t sl = sls[f];
                     if (sl == 0xFFFF) break;
                     const int j = sl >> 1;
                     const bool isR = sl & 1;
-                    const double rj = isR ? sNb[j] : sNb[2 * FT + j];
-                    const double pj = isR ? sNb[FT + j] : sNb[3 * FT + j];
+                    const double rj = (double)(isR ? nb[j] : nb[2 * FT + j]);
+                    const double pj = (double)(isR ? nb[FT + j] : nb[3 * FT + j]);
                     rlo = fmin(rlo, rj);
                     rhi = fmax(rhi, rj);
                     plo = fmin(plo, pj);

@@ -51,8 +51,9 @@ hipError_t launch_flux_pos(const DevMesh &m, const DevCfg &c, int flux, int lim,
     const bool geom = (st.lts && st.stage == 0) || (st.last && st.dt_partial);
     size_t sm = flux_launch_smem(m, 2, c.grad_lds, block, geom);
     if (pass == 1) {
-        // the detection's per-face densities and pressures follow the face geometry in LDS
-        sm += sizeof(double) * 4 * (size_t)m.max_tile_faces;
+        // the detection's per-face densities and pressures (4 floats a face) follow the face
+        // geometry in LDS
+        sm += sizeof(float) * 4 * (size_t)m.max_tile_faces;
         return launch_pos_pass<1>(m, c, flux, lim, Us, grad, st, block, sm, s);
     }
     return launch_pos_pass<2>(m, c, flux, lim, Us, grad, st, block, sm, s);

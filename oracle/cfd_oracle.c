@@ -1392,12 +1392,17 @@ static int stage_fallback_flags(const OMesh *m, const double *Uin, const double 
         double r0, p0, rn, pn;
         rho_p_of(&Uin[4 * c], &r0, &p0);
         rho_p_of(&Unew[4 * c], &rn, &pn);
+        /* the range's bounds from the stage inputs rounded to FP32 (the margin is 1e-3 relative) */
+        r0 = (double)(float)r0;
+        p0 = (double)(float)p0;
         double rlo = r0, rhi = r0, plo = p0, phi = p0;
         for (int f = 0; f < m->nf[c]; f++) {
             int j = m->nbr[4 * c + f];
             if (j < 0 || j >= nc) continue;
             double rj, pj;
             rho_p_of(&Uin[4 * j], &rj, &pj);
+            rj = (double)(float)rj;
+            pj = (double)(float)pj;
             rlo = fmin(rlo, rj);
             rhi = fmax(rhi, rj);
             plo = fmin(plo, pj);
