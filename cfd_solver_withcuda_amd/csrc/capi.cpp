@@ -90,6 +90,7 @@ struct cfd_solver_s {
     double4 *Uc = nullptr, *acc2 = nullptr;
     uint8_t *pflag = nullptr;
     int *dirty = nullptr;
+    unsigned pos_seq = 0;  // detecting launches so far: picks the marked-tile count of the next one
     double *dt = nullptr, *grad = nullptr, *hbuf = nullptr, *err_partial = nullptr, *dt_partial = nullptr;
     double *scal = nullptr;  // [0],[1] global dt slots, [2] stage minimum, [3..6] error norm
     double4 *Ug = nullptr;   // ghost states (reference ghost order), allocated on first use
@@ -596,9 +597,16 @@ static int ensure_positivity_buffers(cfd_solver_s *s) {
     HIP_TRY(hipMalloc((void **)&s->Uc, nc * sizeof(double4)));
     HIP_TRY(hipMalloc((void **)&s->acc2, nc * sizeof(double4)));
     HIP_TRY(hipMalloc((void **)&s->pflag, nc));
-    HIP_TRY(hipMalloc((void **)&s->dirty, (size_t)s->dev.ntiles * sizeof(int)));
-    HIP_TRY(hipMemset(s->dirty, 0, (size_t)s->dev.ntiles * sizeof(int)));
+    // marks [ntiles], the list of marked tiles [ntiles] and its two alternating counts
+    const size_t nd = 2 * (size_t)s->dev.ntiles + 2;
+    HIP_TRY(hipMalloc((void **)&s->dirty, nd * sizeof(int)));
+    HIP_TRY(hipMemset(s->dirty, 0, nd * sizeof(int)));
     return 0;
+}
+
+// The fallback's check flags: range test (bit 0) and the parity of the detecting launch (bit 1).
+static int pos_flags(const cfd_solver_s *s) {
+    return (s->cfg.positivity == 1 ? 1 : 0) | (int)((s->pos_seq & 1u) << 1);
 }
 
 // One native step (fused stages). want_err: emit the error norm into scal[3..6].
@@ -655,9 +663,10 @@ static int native_step(cfd_solver_s *s, bool want_err, hipStream_t st) {
         if (pos) {
             a.pflag = s->pflag;
             a.dirty = s->dirty;
-            a.pos_range = (c.positivity == 1);
+            a.pos_range = pos_flags(s);
             HIP_TRY(launch_flux_pos(dm, s->dcfg, c.flux, c.limiter, 1, in, s->grad, a, c.block, st));
             HIP_TRY(launch_flux_pos(dm, s->dcfg, c.flux, c.limiter, 2, in, s->grad, a, c.block, st));
+            s->pos_seq++;
         } else {
             HIP_TRY(launch_flux(dm, s->dcfg, c.flux, c.order, c.limiter, MODE_STAGE, in, s->grad, a, c.block, st));
         }
@@ -749,7 +758,7 @@ int cfd_time_pass(cfd_solver_t *s, int which, int reps, float *ms, void *stream)
         if (int rc = ensure_positivity_buffers(s)) return rc;
         a.pflag = s->pflag;
         a.dirty = s->dirty;
-        a.pos_range = (c.positivity == 1);
+        a.pos_range = pos_flags(s);
     }
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
@@ -770,7 +779,10 @@ int cfd_time_pass(cfd_solver_t *s, int which, int reps, float *ms, void *stream)
         }
     }
     HIP_TRY(hipEventRecord(e1, st));
-    if (which == 1 && pos) HIP_TRY(launch_flux_pos(dm, s->dcfg, c.flux, c.limiter, 2, s->Un, s->grad, a, c.block, st));
+    if (which == 1 && pos) {
+        HIP_TRY(launch_flux_pos(dm, s->dcfg, c.flux, c.limiter, 2, s->Un, s->grad, a, c.block, st));
+        s->pos_seq++;
+    }
     HIP_TRY(hipEventSynchronize(e1));
     HIP_TRY(hipEventElapsedTime(ms, e0, e1));
     (void)hipEventDestroy(e0);

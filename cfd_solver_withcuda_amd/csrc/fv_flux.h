@@ -380,11 +380,15 @@ __global__ void __launch_bounds__(256) CFD_FLUX_ATTR CFD_POS_ATTR flux_tile_kern
                                                         const double *__restrict__ grad,
                                                         StageArgs st) {
     extern __shared__ double smem[];
-    const int ntr = tiles_to_run(m);
-    const int span = PERSIST ? tile_span(ntr, cfg.xcd_remap) : 0;
+    // the recomputing pass of the positivity fallback walks the list of marked tiles
+    constexpr bool DLIST = (POS == 2 && PERSIST);
+    const int par = (st.pos_range >> 1) & 1;
+    const int ntr = DLIST ? st.dirty[2 * m.ntiles + par] : tiles_to_run(m);
+    const int span = DLIST ? ntr : (PERSIST ? tile_span(ntr, cfg.xcd_remap) : 0);
     int v = blockIdx.x;
+    if (DLIST && v >= span) return;
     do {
-    const int tile = virtual_tile(m, v, ntr, cfg.xcd_remap);
+    const int tile = DLIST ? st.dirty[m.ntiles + v] : virtual_tile(m, v, ntr, cfg.xcd_remap);
     if (tile < 0) continue;
     if (PERSIST && v != (int)blockIdx.x) __syncthreads();  // the previous tile's LDS reads are done
     if constexpr (POS == 2) {
@@ -573,8 +577,8 @@ __global__ void __launch_bounds__(256) CFD_FLUX_ATTR CFD_POS_ATTR flux_tile_kern
                 const double dr = fmax(1e-3 * fabs(rhi), 1e-3 * (rhi - rlo));
                 const double dp = fmax(1e-3 * fabs(phi), 1e-3 * (phi - plo));
                 const bool bad = !(rn > 0.0) || !(pn > 0.0) ||
-                                 (st.pos_range && (!(rn >= rlo - dr && rn <= rhi + dr) ||
-                                                   !(pn >= plo - dp && pn <= phi + dp)));
+                                 ((st.pos_range & 1) && (!(rn >= rlo - dr && rn <= rhi + dr) ||
+                                                         !(pn >= plo - dp && pn <= phi + dp)));
                 st.pflag[c] = bad ? 1 : 0;
                 if (bad) {
                     // this tile and the tiles of the cell's face neighbours recompute the stage

@@ -31,11 +31,15 @@ struct StageArgs {
     float *psi;
     // Positivity fallback of native MUSCL stages: the detecting launch writes pflag[cell] (1 = the
     // stage state failed the admissibility check) and marks in dirty[tile] the tiles owning such a
-    // cell or one of its face neighbours; the recomputing launch redoes only marked tiles (with
-    // first-order states on the faces of flagged cells) and clears their marks.
+    // cell or one of its face neighbours; a compaction launch lists the marked tiles in
+    // dirty[ntiles .. 2 ntiles) with their count in dirty[2 ntiles + parity] (and zeroes the other
+    // count, the next stage's: launches alternate between the two, so none is reset while a launch
+    // still reads it); the recomputing launch walks that list with persistent workgroups, redoes
+    // those tiles (with first-order states on the faces of flagged cells) and clears their marks.
+    // One allocation, so that the stage kernels take no further arguments.
     uint8_t *pflag;
     int *dirty;
-    int pos_range;  // 1: the admissibility check includes the neighbourhood range test
+    int pos_range;  // bit 0: the check includes the neighbourhood range test; bit 1: count parity
 };
 
 size_t grad_smem_bytes(const DevMesh &m, int grad_type);

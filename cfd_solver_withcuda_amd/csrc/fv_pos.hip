@@ -9,16 +9,39 @@ namespace cfd {
 
 namespace {
 
+// The marked tiles of a detecting launch, listed (in no particular order: each is recomputed on its
+// own) after the marks; one thread per tile.
+__global__ void __launch_bounds__(256) list_marked_tiles_kernel(int *dirty, int ntiles, int par) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int *count = dirty + 2 * (size_t)ntiles;
+    if (t == 0) count[par ^ 1] = 0;
+    if (t < ntiles && dirty[t]) dirty[ntiles + atomicAdd(&count[par], 1)] = t;
+}
+
+// Pass 1 runs one workgroup per tile; pass 2 runs persistent workgroups over the list of marked
+// tiles (usually few or none), so that it does not launch one workgroup per tile.
+template <int FLUX, int LIM, int PASS, bool FZ>
+void launch_pos_fz(const DevMesh &m, const DevCfg &c, const double4 *Us, const double *grad,
+                   const StageArgs &st, int block, size_t sm, hipStream_t s) {
+    if constexpr (PASS == 1) {
+        hipLaunchKernelGGL((flux_tile_kernel<FLUX, 2, LIM, MODE_STAGE, false, 1, FZ>), dim3(grid_of(m, c)),
+                           dim3(block), sm, s, m, c, Us, grad, st);
+    } else {
+        hipLaunchKernelGGL(list_marked_tiles_kernel, dim3((m.ntiles + 255) / 256), dim3(256), 0, s, st.dirty,
+                           m.ntiles, (st.pos_range >> 1) & 1);
+        auto k = flux_tile_kernel<FLUX, 2, LIM, MODE_STAGE, false, 2, FZ, true>;
+        hipLaunchKernelGGL(k, dim3(grid_persist(k, m, c, block, sm)), dim3(block), sm, s, m, c, Us, grad, st);
+    }
+}
+
 template <int FLUX, int LIM, int PASS>
 void launch_pos_t(const DevMesh &m, const DevCfg &c, const double4 *Us, const double *grad,
                   const StageArgs &st, int block, size_t sm, hipStream_t s) {
     // the freezing code only where a freezing state is set: it costs registers
     if (st.freeze)
-        hipLaunchKernelGGL((flux_tile_kernel<FLUX, 2, LIM, MODE_STAGE, false, PASS, true>), dim3(grid_of(m, c)),
-                           dim3(block), sm, s, m, c, Us, grad, st);
+        launch_pos_fz<FLUX, LIM, PASS, true>(m, c, Us, grad, st, block, sm, s);
     else
-        hipLaunchKernelGGL((flux_tile_kernel<FLUX, 2, LIM, MODE_STAGE, false, PASS, false>), dim3(grid_of(m, c)),
-                           dim3(block), sm, s, m, c, Us, grad, st);
+        launch_pos_fz<FLUX, LIM, PASS, false>(m, c, Us, grad, st, block, sm, s);
 }
 
 template <int PASS>
