@@ -51,7 +51,7 @@ struct cfd_mlp_s {
     float *gW = nullptr, *gb = nullptr;
     std::vector<uint16_t *> Wb, Wt, X, Xt, dZ, dZt;
     std::vector<float *> Z;
-    float *part = nullptr, *gpad = nullptr;
+    float *part = nullptr;
     float *norm = nullptr;  // in_mean[n_in], in_scale[n_in], out_mean[n_out], out_scale[n_out]
     std::vector<float> norm_h;
     float *feat = nullptr, *targ = nullptr, *lrow = nullptr;
@@ -110,8 +110,8 @@ int backward(cfd_mlp_s *m, int rows) {
         // dW_l = dZ_l^T X_l: A = dZt_l [Mp x rows], B = Xt_l [Kp x rows], split over rows
         MHIP(mlp_gemm(MLP_EPI_PARTIAL, m->dZt[l], rows, m->Xt[l], rows, m->Mp[l], m->Kp[l], rows, kSplit,
                       nullptr, nullptr, m->part, nullptr, nullptr, 0, 0, m->Mp[l], m->stream));
-        MHIP(mlp_reduce_partials(m->part, kSplit, m->Mp[l] * m->Kp[l], m->gpad, m->stream));
-        MHIP(mlp_unpad(m->gpad, m->Kp[l], m->Mv[l], m->Kv[l], m->gW + m->woff[l], m->stream));
+        MHIP(mlp_reduce_unpad(m->part, kSplit, m->Mp[l] * m->Kp[l], m->Kp[l], m->Mv[l], m->Kv[l],
+                              m->gW + m->woff[l], m->stream));
         MHIP(mlp_row_sum(m->dZt[l], rows, m->Mv[l], rows, m->gb + m->boff[l], m->stream));
         if (l > 0) {
             // dZ_{l-1} = (dZ_l W_l) * silu'(Z_{l-1}): A = dZ_l [rows x Mp], B = Wt_l [Kp x Mp]
@@ -179,7 +179,6 @@ int cfd_mlp_create(int n_in, int width, int depth, int n_out, int max_rows, unsi
               (e = m->alloc(&m->mb, m->nb)) == hipSuccess && (e = m->alloc(&m->vb, m->nb)) == hipSuccess &&
               (e = m->alloc(&m->gW, m->nw)) == hipSuccess && (e = m->alloc(&m->gb, m->nb)) == hipSuccess &&
               (e = m->alloc(&m->part, (size_t)kSplit * maxMK)) == hipSuccess &&
-              (e = m->alloc(&m->gpad, (size_t)maxMK)) == hipSuccess &&
               (e = m->alloc(&m->norm, 2 * (size_t)(n_in + n_out))) == hipSuccess &&
               (e = m->alloc(&m->lrow, (size_t)R)) == hipSuccess && (e = m->alloc(&m->idx, (size_t)R)) == hipSuccess &&
               (e = m->alloc(&m->d_step, 1)) == hipSuccess;

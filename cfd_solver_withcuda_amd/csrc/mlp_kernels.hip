@@ -115,16 +115,7 @@ __global__ void __launch_bounds__(64) gemm_nt_kernel(const uint16_t *__restrict_
         }
 }
 
-// dW[m][k] = sum_s P[s][m][k] (fixed order); db[m] = sum_n dZt[m][n] (one workgroup per m row).
-__global__ void __launch_bounds__(256) reduce_partials_kernel(const float *__restrict__ P, int S,
-                                                              int MK, float *__restrict__ out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= MK) return;
-    float s = 0.f;
-    for (int q = 0; q < S; q++) s += P[(size_t)q * MK + i];
-    out[i] = s;
-}
-
+// db[m] = sum_n dZt[m][n] (one workgroup per m row).
 __global__ void __launch_bounds__(256) row_sum_kernel(const uint16_t *__restrict__ X, int ld, int n,
                                                       float *__restrict__ out) {
     __shared__ float red[256];
@@ -233,12 +224,16 @@ __global__ void __launch_bounds__(256) pack_weights_kernel(const float *__restri
     Wt[(size_t)k * Mp + m] = h;
 }
 
-// Gathers the valid [M x K] block of a padded [Mp x Kp] fp32 matrix.
-__global__ void __launch_bounds__(256) unpad_kernel(const float *__restrict__ P, int Kp, int M, int K,
-                                                    float *__restrict__ out) {
+// dW[m][k] = sum_s P[s][m][k] over the valid [M x K] block of the padded [Mp x Kp] partials
+// (fixed order), written unpadded.
+__global__ void __launch_bounds__(256) reduce_unpad_kernel(const float *__restrict__ P, int S, int MKp, int Kp,
+                                                           int M, int K, float *__restrict__ out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= M * K) return;
-    out[i] = P[(size_t)(i / K) * Kp + (i % K)];
+    const size_t o = (size_t)(i / K) * Kp + (i % K);
+    float s = 0.f;
+    for (int q = 0; q < S; q++) s += P[(size_t)q * MKp + o];
+    out[i] = s;
 }
 
 // Minibatch indices from a counter-based hash (SplitMix64) of (seed, step, row).
@@ -329,11 +324,6 @@ hipError_t mlp_gemm(int epi, const uint16_t *A, int lda, const uint16_t *B, int 
     return hipGetLastError();
 }
 
-hipError_t mlp_reduce_partials(const float *P, int S, int MK, float *out, hipStream_t s) {
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3((MK + 255) / 256), dim3(256), 0, s, P, S, MK, out);
-    return hipGetLastError();
-}
-
 hipError_t mlp_row_sum(const uint16_t *X, int ld, int rows, int n, float *out, hipStream_t s) {
     hipLaunchKernelGGL(row_sum_kernel, dim3(rows), dim3(256), 0, s, X, ld, n, out);
     return hipGetLastError();
@@ -363,8 +353,8 @@ hipError_t mlp_pack(const float *W, int M, int K, int Mp, int Kp, uint16_t *Wb, 
     return hipGetLastError();
 }
 
-hipError_t mlp_unpad(const float *P, int Kp, int M, int K, float *out, hipStream_t s) {
-    hipLaunchKernelGGL(unpad_kernel, dim3((M * K + 255) / 256), dim3(256), 0, s, P, Kp, M, K, out);
+hipError_t mlp_reduce_unpad(const float *P, int S, int MKp, int Kp, int M, int K, float *out, hipStream_t s) {
+    hipLaunchKernelGGL(reduce_unpad_kernel, dim3((M * K + 255) / 256), dim3(256), 0, s, P, S, MKp, Kp, M, K, out);
     return hipGetLastError();
 }
 

@@ -144,3 +144,35 @@ def predicted_state(mlp: MLP, points, cells, mach, thickness, floor=1e-3, fs=Non
 
 
 __all__ = ["MLP", "features", "cell_centres", "prim_from_cons", "cons_from_prim_arr", "predicted_state"]
+
+
+def _time_training(steps, batch=16384, samples=1 << 20, launch="graph"):
+    """Wall time per training step of the 4 -> 6 x 256 -> 4 network on random samples of a smooth
+    field (5 untimed steps first)."""
+    import time
+
+    rng = np.random.default_rng(0)
+    feat = rng.uniform(0, 1, (samples, 4)).astype(np.float32)
+    targ = np.column_stack([np.sin(3 * feat[:, 0]), feat[:, 1] * feat[:, 2], np.cos(feat[:, 3]),
+                            feat[:, 0] ** 2]).astype(np.float32)
+    m = MLP(4, 256, 6, 4, max_rows=batch, seed=1)
+    m.fit_normalization(feat, targ)
+    m.set_launch(launch)
+    m.train(feat, targ, steps=5, batch=batch, log_every=1000)
+    t = time.perf_counter()
+    m.train(feat, targ, steps=steps, batch=batch, log_every=1 << 30)
+    dt = (time.perf_counter() - t) / steps
+    flops = 6 * (m.nw + m.nb) * batch
+    return dict(ms_per_step=1e3 * dt, tflops=flops / dt / 1e12, batch=batch, launch=launch)
+
+
+if __name__ == "__main__":
+    import argparse
+    import json
+
+    ap = argparse.ArgumentParser(description="time the surrogate's training step on the GPU")
+    ap.add_argument("--time", type=int, default=200, help="timed training steps")
+    ap.add_argument("--batch", type=int, default=16384)
+    ap.add_argument("--launch", default="graph", choices=["graph", "kernels"])
+    a = ap.parse_args()
+    print(json.dumps(_time_training(a.time, a.batch, launch=a.launch)))
