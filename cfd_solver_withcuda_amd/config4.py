@@ -113,11 +113,13 @@ def load_model(path):
     return mlp
 
 
-def evaluate(model, out_file, first=0, count=256, tol=1e-8, max_steps=60000, freeze_warm=True, snap=0.02):
+def evaluate(model, out_file, first=0, count=256, tol=1e-8, max_steps=60000, freeze_warm=True, snap=0.02,
+             positivity=0):
     """freeze_warm: warm-started instances record their limiter factors on the first step and march
     frozen (else they freeze on a stall, as the plain march does). snap: cells predicted within
-    that relative distance of the freestream state start at it exactly (predicted_state)."""
-    sch = scheme()
+    that relative distance of the freestream state start at it exactly (predicted_state).
+    positivity: the fallback of both marches (0 off, 1 sign and range check, 2 sign only)."""
+    sch = scheme(positivity=positivity or None)
     mlp = load_model(model)
     t0 = time.perf_counter()
     ev = held_out_cases(first=first, count=count)
@@ -156,7 +158,7 @@ def evaluate(model, out_file, first=0, count=256, tol=1e-8, max_steps=60000, fre
             d_max.append(float(np.abs(warm_states[k] - plain[k]).max() / np.abs(plain[k]).max()))
             d_l2.append(float(np.linalg.norm(warm_states[k] - plain[k]) / np.linalg.norm(plain[k])))
     out = dict(first=first, count=len(ev), cells=int(sum(len(c.mesh.cells) for c in ev)),
-               scheme={k: sch[k] for k in ("flux", "limiter", "gradient", "cfl", "freeze_stall")},
+               scheme={k: sch.get(k, 0) for k in ("flux", "limiter", "gradient", "cfl", "freeze_stall", "positivity")},
                plain=dict(wall_s=rp["wall_s"], converged=int((rp["status"] == 0).sum()),
                           steps_max=int(rp["steps"].max()), steps_sum=int(rp["steps"].sum())),
                warm=dict(wall_s=warm_wall, march_s=rw["wall_s"], inference_s=t_inf, fallback_check_s=t_check,
@@ -203,12 +205,13 @@ def main(argv=None):
     e.add_argument("--max-steps", type=int, default=60000)
     e.add_argument("--freeze-warm", type=int, default=1)
     e.add_argument("--snap", type=float, default=0.02)
+    e.add_argument("--positivity", type=int, default=0)
     a = p.parse_args(argv)
     if a.cmd == "train":
         train(a.out, n_cases=a.cases, cells=a.cells, chunk=a.chunk, mlp_steps=a.mlp_steps, max_steps=a.max_steps)
     else:
         evaluate(a.model, a.out, first=a.first, count=a.count, max_steps=a.max_steps, freeze_warm=bool(a.freeze_warm),
-                 snap=a.snap)
+                 snap=a.snap, positivity=a.positivity)
     return 0
 
 

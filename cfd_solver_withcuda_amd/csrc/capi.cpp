@@ -597,16 +597,17 @@ static int ensure_positivity_buffers(cfd_solver_s *s) {
     HIP_TRY(hipMalloc((void **)&s->Uc, nc * sizeof(double4)));
     HIP_TRY(hipMalloc((void **)&s->acc2, nc * sizeof(double4)));
     HIP_TRY(hipMalloc((void **)&s->pflag, nc));
-    // marks [ntiles], the list of marked tiles [ntiles] and its two alternating counts
-    const size_t nd = 2 * (size_t)s->dev.ntiles + 2;
+    // marks [ntiles], the list of marked tiles [ntiles] and its counts (two alternate here)
+    const size_t nd = 2 * (size_t)s->dev.ntiles + 4;
     HIP_TRY(hipMalloc((void **)&s->dirty, nd * sizeof(int)));
     HIP_TRY(hipMemset(s->dirty, 0, nd * sizeof(int)));
     return 0;
 }
 
-// The fallback's check flags: range test (bit 0) and the parity of the detecting launch (bit 1).
+// The fallback's check flags: range test (bit 0), the marked-tile count of this detecting launch
+// (bits 1-2) and of the next one (bits 3-4), alternating between two.
 static int pos_flags(const cfd_solver_s *s) {
-    return (s->cfg.positivity == 1 ? 1 : 0) | (int)((s->pos_seq & 1u) << 1);
+    return (s->cfg.positivity == 1 ? 1 : 0) | (int)((s->pos_seq & 1u) << 1) | (int)(((s->pos_seq + 1u) & 1u) << 3);
 }
 
 // One native step (fused stages). want_err: emit the error norm into scal[3..6].

@@ -88,6 +88,11 @@ struct cfd_batch_s {
     double *d_err4 = nullptr;
     // solver state
     double4 *Un = nullptr, *Ua = nullptr, *Ub = nullptr, *acc = nullptr;
+    // positivity fallback: the last stage's output (copied into Un over the marching tiles), the
+    // second RK4 accumulator, per-cell flags, per-tile marks with the list of marked tiles
+    double4 *Uc = nullptr, *acc2 = nullptr;
+    uint8_t *pflag = nullptr;
+    int *dirty = nullptr;
     double *grad = nullptr, *hbuf = nullptr, *err_partial = nullptr;
     DevMesh dev{};
     hipStream_t stream = nullptr;
@@ -109,7 +114,8 @@ struct cfd_batch_s {
         if (stream) (void)hipStreamDestroy(stream);
         void *ps[] = {d_tile_cstart, d_tile_fstart, d_fgeo, d_fw, d_fnx, d_fny, d_fdl, d_fLR, d_fst, d_area, d_inv_area, d_slots, d_tile_inst, d_inst_tile,
                       d_tile_list, d_tile_count, d_inst_off, d_status, d_steps, d_inst_fs, d_err4,
-                      Un, Ua, Ub, acc, grad, hbuf, err_partial, d_fz, d_fz_since, d_fz_best, psi};
+                      Un, Ua, Ub, acc, grad, hbuf, err_partial, d_fz, d_fz_since, d_fz_best, psi,
+                      Uc, acc2, pflag, dirty};
         for (void *p : ps)
             if (p) (void)hipFree(p);
     }
@@ -136,7 +142,10 @@ int check_batch_config(const cfd_config_t *c) {
     if (!(c->freeze_tol >= 0.0) || c->freeze_stall < 0)
         return capi_fail(CFD_E_ARG, "batch config: freeze_tol and freeze_stall must be >= 0");
     if (!c->lts) return capi_fail(CFD_E_ARG, "batch config: batched steady solves use local time stepping (lts = 1)");
-    if (c->positivity) return capi_fail(CFD_E_ARG, "batch config: the positivity fallback exists for single solvers only");
+    if (c->positivity < 0 || c->positivity > 2)
+        return capi_fail(CFD_E_ARG, "batch config: positivity must be 0, 1 or 2");
+    if (c->positivity && (c->order != 2 || c->viscous))
+        return capi_fail(CFD_E_ARG, "batch config: the positivity fallback needs order 2, inviscid");
     for (int k = 1; k <= 5; k++)
         if (c->bc[k] < 1 || c->bc[k] > 10) return capi_fail(CFD_E_ARG, "batch config: bc types must be 1..10");
     if (!(c->p_exit >= 0.0) || !std::isfinite(c->p_exit))
@@ -227,6 +236,16 @@ int build_batch(const std::vector<InstanceInput> &in, const double *fs4, const c
     BALLOC(b->Ua, nc);
     BALLOC(b->Ub, nc);
     if (cfg.rk == 4) BALLOC(b->acc, nc);
+    if (cfg.positivity) {
+        BALLOC(b->Uc, nc);
+        if (cfg.rk == 4) BALLOC(b->acc2, nc);
+        BALLOC(b->pflag, nc);
+        BALLOC(b->dirty, 2 * nt + 4);
+        if ((e = hipMemset(b->dirty, 0, (2 * nt + 4) * sizeof(int))) != hipSuccess) {
+            delete b;
+            return capi_hip_fail(e, "cfd_batch_create alloc");
+        }
+    }
     if (cfg.order >= 2) BALLOC(b->grad, 8 * nc);
     if (cfg.order == 3) BALLOC(b->d_fst, 4 * nfe);
     BALLOC(b->hbuf, nc);
@@ -350,17 +369,25 @@ int batch_reset(cfd_batch_s *b) {
 // solve), so that a batch whose instances have mostly converged launches few empty workgroups.
 int batch_step_launches(cfd_batch_s *b, double tol, int grid_tiles) {
     const cfd_config_t &c = b->cfg;
-    DevMesh dm = b->dev;
-    dm.ntiles = grid_tiles;  // the batch kernels read the marching count from the list, not ntiles
+    const DevMesh &dm = b->dev;
+    DevCfg dc = b->dcfg;
+    dc.grid_tiles = grid_tiles;
     const int nst = (c.rk == 4) ? 4 : 3;
+    // With the positivity fallback no stage writes an array it reads (its recomputation reads the
+    // stage inputs again): the last stage writes Uc, copied into Un over the marching tiles after
+    // the step, and the RK4 accumulator alternates between acc and acc2.
+    const bool pos = c.positivity != 0;
+    double4 *last = pos ? b->Uc : b->Un;
     const double4 *ins3[3] = {b->Un, b->Ua, b->Ub};
-    double4 *outs3[3] = {b->Ua, b->Ub, b->Un};
+    double4 *outs3[3] = {b->Ua, b->Ub, last};
     const double4 *ins4[4] = {b->Un, b->Ua, b->Ub, b->Ua};
-    double4 *outs4[4] = {b->Ua, b->Ub, b->Ua, b->Un};
+    double4 *outs4[4] = {b->Ua, b->Ub, b->Ua, last};
+    double4 *acc_w[4] = {b->acc, pos ? b->acc2 : b->acc, b->acc, b->acc};
+    const double4 *acc_r[4] = {b->acc, b->acc, pos ? b->acc2 : b->acc, b->acc};
     for (int k = 0; k < nst; k++) {
         const double4 *in = (nst == 4) ? ins4[k] : ins3[k];
         double4 *out = (nst == 4) ? outs4[k] : outs3[k];
-        if (c.order >= 2) BHIP(launch_grad(dm, b->dcfg, c.gradient, in, b->grad, b->block, b->stream));
+        if (c.order >= 2) BHIP(launch_grad(dm, dc, c.gradient, in, b->grad, b->block, b->stream));
         StageArgs a{};
         a.rk = nst;
         a.stage = k;
@@ -368,14 +395,24 @@ int batch_step_launches(cfd_batch_s *b, double tol, int grid_tiles) {
         a.last = (k == nst - 1);
         a.Un = b->Un;
         a.Uout = out;
-        a.acc_in = b->acc;
-        a.acc_out = b->acc;
+        a.acc_in = acc_r[k];
+        a.acc_out = acc_w[k];
         a.hbuf = b->hbuf;
         a.err_partial = a.last ? b->err_partial : nullptr;
         a.freeze = b->d_fz;
         a.psi = b->psi;
-        BHIP(launch_flux(dm, b->dcfg, c.flux, c.order, c.limiter, MODE_STAGE, in, b->grad, a, b->block, b->stream));
+        if (pos) {
+            // the stages of one captured step use the marked-tile counts 0 .. nst-1 in turn
+            a.pflag = b->pflag;
+            a.dirty = b->dirty;
+            a.pos_range = (c.positivity == 1 ? 1 : 0) | (k << 1) | (((k + 1) % nst) << 3);
+            BHIP(launch_flux_pos(dm, dc, c.flux, c.limiter, 1, in, b->grad, a, b->block, b->stream));
+            BHIP(launch_flux_pos(dm, dc, c.flux, c.limiter, 2, in, b->grad, a, b->block, b->stream));
+        } else {
+            BHIP(launch_flux(dm, dc, c.flux, c.order, c.limiter, MODE_STAGE, in, b->grad, a, b->block, b->stream));
+        }
     }
+    if (pos) BHIP(launch_batch_commit(dm, grid_tiles, b->Uc, b->Un, b->stream));
     BHIP(launch_batch_status(b->ninst, b->d_inst_tile, b->err_partial, tol, b->d_status, b->d_steps, b->d_err4,
                              b->d_fz, b->d_fz_best, b->d_fz_since, c.freeze_tol, c.freeze_stall, b->stream));
     BHIP(launch_batch_list(b->ninst, b->ntiles, b->d_inst_tile, b->d_tile_inst, b->d_status,
@@ -786,8 +823,9 @@ int cfd_batch_set_cfl(cfd_batch_t *b, double cfl) {
 
 int cfd_batch_set_launch(cfd_batch_t *b, int mode) {
     if (!b || (mode != 0 && mode != 1)) return capi_fail(CFD_E_ARG, "cfd_batch_set_launch: mode must be 0 or 1");
-    if (mode == 1 && (b->cfg.order > 2 || b->cfg.viscous))
-        return capi_fail(CFD_E_ARG, "cfd_batch_set_launch: persistent workgroups need order 1 or 2, inviscid");
+    if (mode == 1 && (b->cfg.order > 2 || b->cfg.viscous || b->cfg.positivity))
+        return capi_fail(CFD_E_ARG, "cfd_batch_set_launch: persistent workgroups need order 1 or 2, inviscid, "
+                                    "without the positivity fallback");
     if (b->dcfg.persist != mode) {
         b->dcfg.persist = mode;
         b->graph_tol = std::nan("");  // the captured step holds the old launches: capture again

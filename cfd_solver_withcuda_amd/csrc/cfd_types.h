@@ -73,6 +73,7 @@ struct DevCfg {
     double inv_pr;   // 1 / Pr
     double p_exit;   // static pressure of back-pressure exits (0: the prescribed state's p)
     int persist;     // tile passes: 0 one workgroup per tile, 1 persistent workgroups (launch only)
+    int grid_tiles;  // > 0: launch grids cover this many tiles (the marching ones; launch only)
 };
 
 }  // namespace cfd

@@ -382,7 +382,7 @@ __global__ void __launch_bounds__(256) CFD_FLUX_ATTR CFD_POS_ATTR flux_tile_kern
     extern __shared__ double smem[];
     // the recomputing pass of the positivity fallback walks the list of marked tiles
     constexpr bool DLIST = (POS == 2 && PERSIST);
-    const int par = (st.pos_range >> 1) & 1;
+    const int par = (st.pos_range >> 1) & 3;
     const int ntr = DLIST ? st.dirty[2 * m.ntiles + par] : tiles_to_run(m);
     const int span = DLIST ? ntr : (PERSIST ? tile_span(ntr, cfg.xcd_remap) : 0);
     int v = blockIdx.x;
@@ -645,7 +645,8 @@ __global__ void __launch_bounds__(256) CFD_FLUX_ATTR CFD_POS_ATTR flux_tile_kern
 }
 
 static inline int grid_of(const DevMesh &m, const DevCfg &c) {
-    return c.xcd_remap ? 8 * ((m.ntiles + 7) / 8) : m.ntiles;
+    const int n = (c.grid_tiles > 0 && c.grid_tiles < m.ntiles) ? c.grid_tiles : m.ntiles;
+    return c.xcd_remap ? 8 * ((n + 7) / 8) : n;
 }
 
 // Grid of a persistent launch: the workgroups the GPU holds at once for this kernel (occupancy

@@ -32,14 +32,14 @@ struct StageArgs {
     // Positivity fallback of native MUSCL stages: the detecting launch writes pflag[cell] (1 = the
     // stage state failed the admissibility check) and marks in dirty[tile] the tiles owning such a
     // cell or one of its face neighbours; a compaction launch lists the marked tiles in
-    // dirty[ntiles .. 2 ntiles) with their count in dirty[2 ntiles + parity] (and zeroes the other
-    // count, the next stage's: launches alternate between the two, so none is reset while a launch
+    // dirty[ntiles .. 2 ntiles) with their count in dirty[2 ntiles + cur] (and zeroes
+    // dirty[2 ntiles + next], the next stage's count, so that no count is reset while a launch
     // still reads it); the recomputing launch walks that list with persistent workgroups, redoes
     // those tiles (with first-order states on the faces of flagged cells) and clears their marks.
-    // One allocation, so that the stage kernels take no further arguments.
+    // One allocation (2 ntiles + 4 ints), so that the stage kernels take no further arguments.
     uint8_t *pflag;
     int *dirty;
-    int pos_range;  // bit 0: the check includes the neighbourhood range test; bit 1: count parity
+    int pos_range;  // bit 0: the check includes the neighbourhood range test; bits 1-2 cur, 3-4 next
 };
 
 size_t grad_smem_bytes(const DevMesh &m, int grad_type);
@@ -79,6 +79,9 @@ hipError_t launch_ghosts(int n, const int *gcell, const double2 *gnrm, const int
 hipError_t launch_batch_status(int ninst, const int *inst_tile, const double *err_partial, double tol,
                                int *status, int *steps, double *err4, int *freeze, double *best,
                                int *since, double freeze_tol, int freeze_stall, hipStream_t s);
+// dst = src over the cells of the tiles still marching (m.tile_list, m.tile_count); grid_tiles
+// bounds the marching count.
+hipError_t launch_batch_commit(const DevMesh &m, int grid_tiles, const double4 *src, double4 *dst, hipStream_t s);
 hipError_t launch_batch_list(int ninst, int ntiles, const int *inst_tile, const int *tile_inst,
                              const int *status, int *inst_off, int *tile_list, int *tile_count,
                              hipStream_t s);

@@ -666,6 +666,24 @@ hipError_t launch_freeze_update(const double *err4, int *freeze, double *best, i
     return hipGetLastError();
 }
 
+// dst = src over the cells of the tiles still marching (one workgroup per list position).
+__global__ void __launch_bounds__(256) batch_commit_kernel(const int *__restrict__ tile_list,
+                                                           const int *__restrict__ tile_count,
+                                                           const int *__restrict__ tile_cstart,
+                                                           const double4 *__restrict__ src,
+                                                           double4 *__restrict__ dst) {
+    if ((int)blockIdx.x >= *tile_count) return;
+    const int t = tile_list[blockIdx.x];
+    for (int c = tile_cstart[t] + threadIdx.x; c < tile_cstart[t + 1]; c += blockDim.x) dst[c] = src[c];
+}
+
+hipError_t launch_batch_commit(const DevMesh &m, int grid_tiles, const double4 *src, double4 *dst, hipStream_t s) {
+    if (grid_tiles <= 0) return hipSuccess;
+    hipLaunchKernelGGL(batch_commit_kernel, dim3(grid_tiles), dim3(128), 0, s, m.tile_list, m.tile_count,
+                       m.tile_cstart, src, dst);
+    return hipGetLastError();
+}
+
 hipError_t launch_batch_list(int ninst, int ntiles, const int *inst_tile, const int *tile_inst,
                              const int *status, int *inst_off, int *tile_list, int *tile_count,
                              hipStream_t s) {

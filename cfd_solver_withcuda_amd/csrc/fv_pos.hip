@@ -11,11 +11,11 @@ namespace {
 
 // The marked tiles of a detecting launch, listed (in no particular order: each is recomputed on its
 // own) after the marks; one thread per tile.
-__global__ void __launch_bounds__(256) list_marked_tiles_kernel(int *dirty, int ntiles, int par) {
+__global__ void __launch_bounds__(256) list_marked_tiles_kernel(int *dirty, int ntiles, int cur, int next) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     int *count = dirty + 2 * (size_t)ntiles;
-    if (t == 0) count[par ^ 1] = 0;
-    if (t < ntiles && dirty[t]) dirty[ntiles + atomicAdd(&count[par], 1)] = t;
+    if (t == 0) count[next] = 0;
+    if (t < ntiles && dirty[t]) dirty[ntiles + atomicAdd(&count[cur], 1)] = t;
 }
 
 // Pass 1 runs one workgroup per tile; pass 2 runs persistent workgroups over the list of marked
@@ -28,7 +28,7 @@ void launch_pos_fz(const DevMesh &m, const DevCfg &c, const double4 *Us, const d
                            dim3(block), sm, s, m, c, Us, grad, st);
     } else {
         hipLaunchKernelGGL(list_marked_tiles_kernel, dim3((m.ntiles + 255) / 256), dim3(256), 0, s, st.dirty,
-                           m.ntiles, (st.pos_range >> 1) & 1);
+                           m.ntiles, (st.pos_range >> 1) & 3, (st.pos_range >> 3) & 3);
         auto k = flux_tile_kernel<FLUX, 2, LIM, MODE_STAGE, false, 2, FZ, true>;
         hipLaunchKernelGGL(k, dim3(grid_persist(k, m, c, block, sm)), dim3(block), sm, s, m, c, Us, grad, st);
     }
