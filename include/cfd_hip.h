@@ -77,14 +77,15 @@ typedef struct {
      * viscous spectral radius of Viscous_Time_Step_1 (src/Time_Step.cpp:103-109). */
     int viscous;
     double mu, prandtl, r_gas;
-    /* Positivity fallback of the native MUSCL stages (order 2, inviscid, single solvers): after
+    /* Positivity fallback of the native MUSCL stages (order 2, inviscid; single and batched solvers): after
      * each stage, a cell whose new state has a non-positive density or pressure, or a density or
      * pressure outside 1e-3 max(|hi|, hi - lo) of the range [lo, hi] of its own and its
      * face neighbours' stage inputs, has the stage recomputed with first-order states on all its
      * faces (an a-posteriori, MOOD-type check; no counterpart in the reference, whose second-order
-     * schemes have no safeguard). 2 applies the positivity test only, without the range test. With
-     * it on, the device buffer of the state alternates between two arrays from step to step (query
-     * cfd_state_device after each step). */
+     * schemes have no safeguard). 2 applies the positivity test only, without the range test. The
+     * range's bounds are the stage inputs rounded to FP32. With it on, a single solver's device
+     * buffer of the state alternates between two arrays from step to step (query cfd_state_device
+     * after each step); a batch copies the last stage's output back over its marching tiles. */
     int positivity;
     /* Static pressure of back-pressure exits (ghost type 9); 0 = the prescribed state's p. */
     double p_exit;
