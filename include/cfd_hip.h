@@ -77,8 +77,9 @@ typedef struct {
      * viscous spectral radius of Viscous_Time_Step_1 (src/Time_Step.cpp:103-109). */
     int viscous;
     double mu, prandtl, r_gas;
-    /* Positivity fallback of the native MUSCL stages (order 2, inviscid; single and batched solvers): after
-     * each stage, a cell whose new state has a non-positive density or pressure, or a density or
+    /* Positivity fallback of the native MUSCL stages (order 2, inviscid; single and batched
+     * solvers): after each stage, a cell whose new state has a non-positive density or pressure,
+     * or a density or
      * pressure outside 1e-3 max(|hi|, hi - lo) of the range [lo, hi] of its own and its
      * face neighbours' stage inputs, has the stage recomputed with first-order states on all its
      * faces (an a-posteriori, MOOD-type check; no counterpart in the reference, whose second-order
