@@ -253,7 +253,11 @@ def warmstart_bench(args):
                                                   / r["mlp"]["train_s"] / 2.5e15},
             "scheme": "HLLC + MUSCL/Van Leer + WLSQ, LTS TVD-RK3 at CFL 1.2, limiter freezing on stall",
             "note": "plain_s / warm_s: wall time until every instance converged or reached the step limit; "
-                    "both_converged: the instances both marches brought to 1e-8, time until the last of them"}
+                    "both_converged: the instances both marches brought to 1e-8, time until the last of them; "
+                    "a scaled-down run of the pipeline that fits the bench's minutes",
+            # the pipeline at its stated size (2,000 training solves, 256 held-out solves of 50k-1M
+            # cells) runs in parts of one GPU session each (python -m cfd_solver_withcuda_amd.config4)
+            "full_size_run": "profiles/warmstart_bump_config4.md"}
 
 
 def sod_bench(args):
