@@ -295,3 +295,27 @@ def test_batched_positivity_fallback_equals_single_solvers(rk, nsteps):
         assert np.all(np.isfinite(Ub[i]))
     with pytest.raises(Exception):
         b.set_launch("persistent")
+
+
+def test_batched_positivity_fallback_with_freezing_equals_single_solvers():
+    """The fallback's kernel variants that carry the frozen limiter factors, batched: instances 0
+    and 2 record their factors on the first step and march frozen, instance 1 evaluates its
+    limiter; each equals the same mesh solved alone in the same mode, bit for bit."""
+    specs = [(0.025, 2, 3.0), (0.03, 5, 2.5), (0.02, 7, 3.2)]
+    meshes = [meshgen.ffs_mesh(h, seed=sd) for h, sd, _ in specs]
+    fs = np.array([freestream(M) for *_, M in specs])
+    cfg = dict(flux="hllc", order=2, limiter="vanleer", gradient="wlsq", cfl=0.4, rk=3, lts=1, roe_bdiss=1,
+               bc=STEP_BCS, positivity=1, freeze_stall=50)
+    print("case batched positivity with freezing", flush=True)
+    b = Batch(meshes, fs, make_config(block=128, **cfg))
+    b.set_freeze([1, 0, 1])
+    nsteps = 20
+    b.solve(tol=-1.0, max_steps=nsteps, check_every=nsteps)
+    Ub = b.get_state()
+    for i, m in enumerate(meshes):
+        s = Solver(Mesh(m.points, m.cells, tile=128, face_budget=256), make_config(block=128, fs=fs[i], **cfg))
+        s.set_state(freestream_state(len(m.cells), fs[i]))
+        if i != 1:
+            s.set_freeze(1)
+        s.step(nsteps)
+        assert np.array_equal(Ub[i], s.get_state()), i
