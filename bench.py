@@ -96,6 +96,8 @@ def parse():
                    help="steps marched from freestream before the warmup, so the timed steps see a "
                         "developed flow (-1: 2000 for bump, 1000 for the step mesh)")
     p.add_argument("--grad-lds", type=int, default=0)
+    p.add_argument("--xcd-remap", type=int, default=16,
+                   help="tile order over the XCDs: 0 block order, 1 one run per XCD, k runs of k tiles dealt in turn")
     p.add_argument("--gradient", default="", choices=["", "gg", "wlsq"],
                    help="override the workload's gradient (bump: wlsq, ffs: gg)")
     p.add_argument("--scheme", default="muscl", choices=["muscl", "weno", "viscous"],
@@ -349,7 +351,7 @@ def steady_bench(args):
                bc=("freestream", "exit", "wall", "wall", "wall"), fs=fs, freeze_stall=300,
                positivity=args.positivity or 0)
     mesh = Mesh(m.points, m.cells, tile=128, renumber=1, face_budget=256)
-    s = Solver(mesh, make_config(roe_bdiss=1, block=128, xcd_remap=1, **cfg))
+    s = Solver(mesh, make_config(roe_bdiss=1, block=128, xcd_remap=args.xcd_remap, **cfg))
     s.set_state(np.tile(cons_from_prim(np.array([fs[0]]), fs[1], fs[2], fs[3]), (mesh.nphys, 1)))
     s.get_state()
     done, ok, err = 0, False, np.full(4, np.nan)
@@ -386,7 +388,7 @@ def ffs_march_bench(args):
     m, cfg, desc = workload("ffs")
     nsteps = args.march_steps
     mesh = Mesh(m.points, m.cells, tile=args.tile, renumber=1, face_budget=args.face_budget)
-    s = Solver(mesh, make_config(roe_bdiss=1, block=args.block, xcd_remap=1, grad_lds=0, positivity=1, **cfg))
+    s = Solver(mesh, make_config(roe_bdiss=1, block=args.block, xcd_remap=args.xcd_remap, grad_lds=0, positivity=1, **cfg))
     fs = cfg["fs"]
     s.set_state(np.tile(cons_from_prim(np.array([fs[0]]), fs[1], fs[2], fs[3]), (mesh.nphys, 1)))
     stream = torch.cuda.current_stream().cuda_stream
@@ -526,7 +528,7 @@ def fv_bench(args, wl, scheme, gradient, sub=False):
         cfg["gradient"] = gradient
     t_setup = time.perf_counter()
     mesh = Mesh(m.points, m.cells, tile=args.tile, renumber=1, face_budget=args.face_budget)
-    mk = lambda **kw: make_config(roe_bdiss=1, block=args.block, xcd_remap=1, grad_lds=args.grad_lds, **kw)
+    mk = lambda **kw: make_config(roe_bdiss=1, block=args.block, xcd_remap=args.xcd_remap, grad_lds=args.grad_lds, **kw)
     s = Solver(mesh, mk(**cfg))
     fs = cfg["fs"]
     U0 = np.tile(cons_from_prim(np.array([fs[0]]), fs[1], fs[2], fs[3]), (mesh.nphys, 1))
@@ -581,7 +583,8 @@ def fv_bench(args, wl, scheme, gradient, sub=False):
 
     t_step = ms_steps / 1e3 / args.steps
     launch_cfg = (f"workload={wl} tile={args.tile} face_budget={args.face_budget} "
-                  f"block={args.block} grad_lds={args.grad_lds}" + (f" gradient={gradient}" if gradient else "")
+                  f"block={args.block} grad_lds={args.grad_lds} xcd_remap={args.xcd_remap}"
+                  + (f" gradient={gradient}" if gradient else "")
                   + (f" scheme={scheme}" if scheme != "muscl" else "") + (" positivity=1" if pos else ""))
     value = F * nres / t_step
     # with the fallback the stage pass is its detecting pass, which also writes one flag byte per cell

@@ -293,6 +293,7 @@ static int check_config(const cfd_mesh_s *m, const cfd_config_t *c) {
     if (c->rk == 1 && !c->ref_mode) return fail(CFD_E_ARG, "config: forward Euler (rk 1) exists in reference mode only");
     if (c->ref_mode && c->rk == 4) return fail(CFD_E_ARG, "config: the reference iteration has no RK4");
     if (c->block != 64 && c->block != 128 && c->block != 256) return fail(CFD_E_ARG, "config: block must be 64, 128 or 256");
+    if (c->xcd_remap < 0 || c->xcd_remap > 4096) return fail(CFD_E_ARG, "config: xcd_remap must be in 0..4096");
     for (int k = 1; k <= 5; k++)
         if (c->bc[k] < 1 || c->bc[k] > 10) return fail(CFD_E_ARG, "config: bc types must be 1..10");
     if (!(c->p_exit >= 0.0) || !std::isfinite(c->p_exit)) return fail(CFD_E_ARG, "config: p_exit must be finite and >= 0");

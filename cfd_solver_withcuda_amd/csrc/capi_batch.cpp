@@ -136,6 +136,7 @@ int check_batch_config(const cfd_config_t *c) {
         return capi_fail(CFD_E_ARG, "batch config: limiter freezing needs order 2");
     if (c->limiter < 0 || c->limiter > 4) return capi_fail(CFD_E_ARG, "batch config: limiter must be 0..4");
     if (c->gradient < 0 || c->gradient > 1) return capi_fail(CFD_E_ARG, "batch config: gradient must be 0 or 1");
+    if (c->xcd_remap < 0 || c->xcd_remap > 4096) return capi_fail(CFD_E_ARG, "batch config: xcd_remap must be in 0..4096");
     if (!(c->cfl > 0.0)) return capi_fail(CFD_E_ARG, "batch config: cfl must be positive");
     if (c->rk != 3 && c->rk != 4) return capi_fail(CFD_E_ARG, "batch config: rk must be 3 or 4");
     if (c->ref_mode) return capi_fail(CFD_E_ARG, "batch config: the reference iteration is not batched");

@@ -63,7 +63,8 @@ struct DevCfg {
     double fs[4];    // prescribed state (rho, u, v, p)
     double cfl;
     int roe_bdiss;
-    int xcd_remap;   // map consecutive tiles onto one XCD
+    int xcd_remap;   // tile order over the XCDs: 0 block order, 1 one contiguous run per XCD,
+                     // k >= 2 runs of k consecutive tiles dealt to the XCDs in turn
     int grad_lds;    // stage the tile's gradients in LDS (1) or read them through L2 (0)
     // laminar viscous terms (second order only): 0 off, 1 constant viscosity mu, 2 Sutherland
     int viscous;

@@ -41,17 +41,26 @@ namespace {
 
 __device__ __forceinline__ int tile_of_block(int b, int ntiles, int remap) {
     if (!remap) return b;
-    // Workgroups are dispatched round-robin over the 8 XCDs (block b -> XCD b % 8); give each
-    // XCD a contiguous run of tiles so that faces shared by neighbouring tiles hit its L2.
-    const int per = (ntiles + 7) / 8;
-    return (b % 8) * per + b / 8;
+    // Workgroups are dispatched round-robin over the 8 XCDs (block b -> XCD b % 8).
+    if (remap == 1) {
+        // each XCD takes one contiguous run of tiles
+        const int per = (ntiles + 7) / 8;
+        return (b % 8) * per + b / 8;
+    }
+    // runs of `remap` consecutive tiles dealt to the XCDs in turn
+    const int j = b / 8;
+    return ((j / remap) * 8 + (b % 8)) * remap + (j % remap);
 }
 
 // Tiles to process: all, or for a batched mesh the list of tiles still marching.
 __device__ __forceinline__ int tiles_to_run(const DevMesh &m) { return m.tile_list ? *m.tile_count : m.ntiles; }
 
 // Virtual workgroups covering n tiles (a multiple of 8 with the XCD-aware order).
-__device__ __forceinline__ int tile_span(int n, int remap) { return remap ? 8 * ((n + 7) / 8) : n; }
+__host__ __device__ __forceinline__ int tile_span(int n, int remap) {
+    if (remap <= 0) return n;
+    if (remap == 1) return 8 * ((n + 7) / 8);
+    return 8 * remap * (((n + remap - 1) / remap + 7) / 8);
+}
 
 // Tile of virtual workgroup v: its XCD-aware position in the tile sequence, or in the list of
 // tiles still marching. -1 when v has no tile. The XCD-aware order is a bijection of [0, span)
@@ -646,7 +655,7 @@ __global__ void __launch_bounds__(256) CFD_FLUX_ATTR CFD_POS_ATTR flux_tile_kern
 
 static inline int grid_of(const DevMesh &m, const DevCfg &c) {
     const int n = (c.grid_tiles > 0 && c.grid_tiles < m.ntiles) ? c.grid_tiles : m.ntiles;
-    return c.xcd_remap ? 8 * ((n + 7) / 8) : n;
+    return tile_span(n, c.xcd_remap);
 }
 
 // Grid of a persistent launch: the workgroups the GPU holds at once for this kernel (occupancy

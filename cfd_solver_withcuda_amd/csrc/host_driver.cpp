@@ -73,7 +73,7 @@ cfd_config_t config(int order, int rk, int ref_mode, int type) {
     c.lts = Local_Time_Stepping ? 1 : 0;
     c.ref_mode = ref_mode;
     c.block = 128;
-    c.xcd_remap = 1;
+    c.xcd_remap = 16;
     c.grad_lds = 0;
     c.zeta = Limiter_Zeta;
     c.zeta1 = Limiter_Zeta1;

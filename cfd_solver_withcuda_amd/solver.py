@@ -31,7 +31,7 @@ REF_BCS = ("inlet_ref", "exit", "wall", "wall", "wall")
 
 def make_config(flux="roe", order=1, limiter="vanleer", gradient="gg", roe_bdiss=0, cfl=0.5,
                 bc=REF_BCS, fs=(1.0, 0.0, 0.0, 1.0), rk=3, lts=0, ref_mode=0, block=256,
-                xcd_remap=1, grad_lds=1, freeze_tol=0.0, freeze_stall=0, zeta=0.5, zeta1=0.5,
+                xcd_remap=16, grad_lds=1, freeze_tol=0.0, freeze_stall=0, zeta=0.5, zeta1=0.5,
                 viscous="off", mu=0.0, prandtl=0.72, r_gas=287.0, positivity=0, p_exit=0.0) -> Config:
     """Scheme settings (cfd_config_t). viscous: "off" (Euler), "constant" (viscosity mu) or
     "sutherland" (Sutherland's law in T = p / (rho r_gas)); laminar viscous terms need order 2 or 3."""

@@ -58,7 +58,9 @@ typedef struct {
     int ref_mode;  /* 1: the reference's iteration (per-stage global dt, delta form, LTS in
                       stage 1 only, exit ghosts from U^n); 0: native fused stages */
     int block;     /* workgroup size (64..1024, power of two; 256 default) */
-    int xcd_remap; /* map consecutive tiles onto one XCD */
+    int xcd_remap; /* order of tiles over the 8 XCDs (workgroups are dealt to them in turn):
+                      0 block order, 1 one contiguous run of tiles per XCD, k = 2..4096 runs of
+                      k consecutive tiles dealt to the XCDs in turn; results do not depend on it */
     int grad_lds;  /* stage the tile's gradients in LDS (1) or read them through L2 (0) */
     /* Limiter freezing for steady marches (order 2, native mode): once Error[0] <= freeze_tol,
      * or once Error[0] has not dropped below 0.9 x its best value for freeze_stall steps, the

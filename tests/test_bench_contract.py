@@ -31,7 +31,7 @@ def test_algorithmic_byte_counts():
 
 
 def test_committed_profiles_carry_the_traffic_of_the_default_launch():
-    launch = "workload={w} tile=128 face_budget=256 block=128 grad_lds=0"
+    launch = "workload={w} tile=128 face_budget=256 block=128 grad_lds=0 xcd_remap=16"
     # the step-mesh workloads time their stages with the positivity fallback by default
     for w, extra in (("bump", ""), ("ffs", " positivity=1"), ("ffs_hllc", " positivity=1")):
         t = bench.measured_traffic((w, "muscl"), launch.format(w=w) + extra)

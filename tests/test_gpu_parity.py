@@ -89,8 +89,11 @@ def test_residual_on_quad_triangle_and_mixed_meshes(orc, tri_frac, jitter, flux,
     assert np.array_equal(dt, dto) and np.array_equal(g, go) and np.array_equal(R, Ro), rel(R, Ro)
 
 
-@pytest.mark.parametrize("tile,renumber,block,remap", [(32, 0, 64, 0), (128, 1, 128, 1), (256, 1, 256, 1)])
+@pytest.mark.parametrize("tile,renumber,block,remap", [(32, 0, 64, 0), (128, 1, 128, 1), (256, 1, 256, 1),
+                                                       (32, 1, 64, 3), (64, 1, 128, 16), (32, 1, 64, 4096)])
 def test_residual_independent_of_tiling(orc, bump, tile, renumber, block, remap):
+    """Tile size, renumbering, block and the tile order over the XCDs (xcd_remap: block order, one
+    run per XCD, runs of k tiles dealt in turn) change nothing: each order visits every tile once."""
     om = orc.mesh(bump.points, bump.cells)
     U = smooth_state(om.arrays().ctr, om.nphys)
     cfg = dict(flux="hllc", order=2, limiter="vanleer", gradient="wlsq", cfl=0.7, bc=BCS_NATIVE, fs=FS)
@@ -100,6 +103,23 @@ def test_residual_independent_of_tiling(orc, bump, tile, renumber, block, remap)
     s.set_state(U)
     R, _, _ = s.residual()
     assert np.array_equal(R, Ro)
+
+
+@pytest.mark.parametrize("remap", [0, 1, 5, 16])
+def test_steps_independent_of_xcd_order(bump, remap):
+    """Fused RK3 stages with local time stepping: the state after 3 steps is the same bit for bit
+    whatever order the tiles are dealt to the XCDs in."""
+    cfg = dict(flux="hllc", order=2, limiter="vanleer", gradient="wlsq", cfl=0.7, bc=BCS_NATIVE, fs=FS,
+               rk=3, lts=1)
+    mesh = Mesh(bump.points, bump.cells, tile=32)
+    U0 = smooth_state(mesh.arrays().ctr, mesh.nphys)
+    out = []
+    for r in (2, remap):
+        s = Solver(mesh, make_config(block=64, xcd_remap=r, **cfg))
+        s.set_state(U0)
+        s.step(3)
+        out.append(s.get_state())
+    assert np.array_equal(out[0], out[1])
 
 
 def test_residual_deterministic(bump):
