@@ -108,6 +108,21 @@ def test_mesh_rejects_bad_input():
         Mesh(np.eye(3), np.array([[0, 1, 7, -1]], np.int32), device=False)  # node out of range
 
 
+def test_batch_rejects_bad_tile_order():
+    """xcd_remap outside 0..4096 is refused before any device work (the tile order's run length
+    enters the workgroup-to-tile arithmetic)."""
+    from cfd_solver_withcuda_amd._lib import CfdError
+    from cfd_solver_withcuda_amd.batch import Batch
+    from cfd_solver_withcuda_amd.solver import make_config
+
+    m = meshgen.ramp_mesh(8, 4, 10.0, seed=1)
+    for bad in (-1, 4097):
+        cfg = make_config(flux="hllc", order=2, limiter="vanleer", gradient="wlsq", rk=3, lts=1,
+                          block=128, xcd_remap=bad)
+        with pytest.raises(CfdError, match="xcd_remap"):
+            Batch([m], np.array([[1.4, 2.0, 0.0, 1.0]]), cfg)
+
+
 def test_host_driver_exports_declared_names():
     """libcfdhost.so defines every function include/cfd_host.hpp declares (the reference's driver
     names, including the dissipation plugin point)."""
