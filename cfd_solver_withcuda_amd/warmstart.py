@@ -14,7 +14,8 @@ their limiter factors on the first step (their shocks are already in place); a f
 error grows tenfold returns to the evaluated limiter. A march frozen at one state converges to the
 fixed point of that frozen scheme, so two marches frozen at different states end at slightly
 different fields. With `limiter="vanalbada", freeze_stall=0` (the differentiable limiter, no
-freezing) both marches share one discrete solution wherever the plain march converges. The plain
+freezing) or `order=1, freeze_stall=0` (first order) both marches share one discrete solution
+wherever the plain march converges. The plain
 FP64 march decides the answer; the surrogate only moves its starting point. `field_tol` continues
 both converged batches, untimed, to a tighter tolerance and compares the fields there too.
 """
@@ -65,12 +66,12 @@ def training_set(cs, states):
 def run(n_train=64, n_eval=32, train_cells=30000, eval_cells=60000, tol=1e-8, train_tol=1e-8,
         max_steps=60000, check_every=100, mlp_steps=3000, mlp_batch=16384, seed=5, cfl=None,
         freeze_warm=True, warm_cfl=None, history=False, log=print, limiter=None, freeze_stall=None,
-        field_tol=None, snap=0.02):
+        field_tol=None, snap=0.02, order=None):
     out = {}
-    over = dict(limiter=limiter, freeze_stall=freeze_stall)
+    over = dict(limiter=limiter, freeze_stall=freeze_stall, order=order)
     sch = scheme(cfl, **over)
     freeze_warm = freeze_warm and sch["freeze_stall"] > 0
-    out["scheme"] = {k: sch[k] for k in ("flux", "limiter", "gradient", "cfl", "freeze_stall")}
+    out["scheme"] = {k: sch[k] for k in ("flux", "order", "limiter", "gradient", "cfl", "freeze_stall")}
     t0 = time.perf_counter()
     tr = cases.bump_family(n_train, seed=seed, cells=train_cells, stream=200)
     b_tr, rt = solve_batch(tr, train_tol, max_steps, check_every, cfl=cfl, **over)

@@ -59,3 +59,39 @@ def test_surrogate_warm_start_beats_freestream():
         assert e_nn < 0.3 * e_fs, i
         assert warm_first[i] < plain_first[i], i
     assert rw["steps"].sum() <= 1.1 * rp["steps"].sum()
+
+
+def test_warm_and_plain_fields_agree_without_freezing():
+    """With a scheme whose steady march converges without limiter freezing (first-order HLLC on
+    this family), the march has one fixed point: the batch started from the surrogate and the batch
+    started from freestream end at the same field. Both run to Error[0] <= 1e-10 and their fields
+    must agree to 1e-8 relative (the oracle's march gives 5e-12 to 7e-12 at 1e-11). With limiter
+    freezing this does not hold: each march converges to the scheme frozen at its own state."""
+    print("case warm start, first order: 8 training bumps of ~3k cells, 4 held-out of ~4k cells", flush=True)
+    sch = dict(SCHEME, order=1, freeze_stall=0)
+    mk = lambda cs: Batch([c.mesh for c in cs], np.array([c.fs for c in cs]), make_config(block=128, **sch))
+    tr = cases.bump_family(8, seed=31, cells=3000, stream=400, **RANGES)
+    bt = mk(tr)
+    rt = bt.solve(tol=1e-8, max_steps=40000, check_every=100)
+    assert rt["converged"], rt["status"]
+    feat, targ = training_set(tr, bt.get_state())
+    mlp = MLP(4, 256, 6, 4, max_rows=4096, seed=5)
+    mlp.fit_normalization(feat, targ)
+    mlp.train(feat, targ, steps=1000, batch=4096, lr=2e-3, lr_final=2e-5, seed=6, log_every=100)
+
+    ev = cases.bump_family(4, seed=32, cells=4000, stream=500, **RANGES)
+    b = mk(ev)
+    b.set_state(freestream_states(ev))
+    rp = b.solve(tol=1e-10, max_steps=100000, check_every=100)
+    assert rp["converged"], rp["status"]
+    plain = b.get_state()
+    b.set_state([predicted_state(mlp, c.mesh.points, c.mesh.cells, c.mach, c.param) for c in ev])
+    rw = b.solve(tol=1e-10, max_steps=100000, check_every=100)
+    assert rw["converged"], rw["status"]
+    warm = b.get_state()
+    for i, (Up, Uw) in enumerate(zip(plain, warm)):
+        d_l2 = np.linalg.norm(Uw - Up) / np.linalg.norm(Up)
+        d_max = np.abs(Uw - Up).max() / np.abs(Up).max()
+        print(f"instance {i}: steps {rp['steps'][i]} plain, {rw['steps'][i]} warm; fields rel L2 {d_l2:.3e}, "
+              f"max {d_max:.3e}", flush=True)
+        assert d_l2 <= 1e-8 and d_max <= 1e-8, i
