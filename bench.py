@@ -257,6 +257,10 @@ def warmstart_bench(args):
             "note": "plain_s / warm_s: wall time until every instance converged or reached the step limit; "
                     "both_converged: the instances both marches brought to 1e-8, time until the last of them; "
                     "a scaled-down run of the pipeline that fits the bench's minutes",
+            "fields_note": "converged_fields_*: the two marches freeze the limiter at different states, so each "
+                           "ends at the fixed point of its own frozen scheme; with a scheme that needs no "
+                           "freezing (first order) the two fields agree to 5e-12 relative at Error[0] 1e-10 "
+                           "(tests/test_gpu_warmstart.py::test_warm_and_plain_fields_agree_without_freezing)",
             # the pipeline at its stated size (2,000 training solves, 256 held-out solves of 50k-1M
             # cells) runs in parts of one GPU session each (python -m cfd_solver_withcuda_amd.config4)
             "full_size_run": "profiles/warmstart_bump_config4.md"}
