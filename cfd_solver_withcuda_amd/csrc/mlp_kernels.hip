@@ -6,7 +6,8 @@
 //
 // GEMM convention: C[r][c] = sum_k A[r][k] * B[c][k] with A [R x K] and B [C x K] both row-major
 // (K contiguous), R, C multiples of 32 and K a multiple of 32. One wave computes a 32 x 32 tile
-// of C as 2 x 2 MFMA 16x16 tiles. Operand lanes of v_mfma_f32_16x16x32_bf16: lane l holds
+// of C as 2 x 2 MFMA 16x16 tiles (four such waves share their operands through LDS where R and C
+// are multiples of 64). Operand lanes of v_mfma_f32_16x16x32_bf16: lane l holds
 // A[l % 16][8 (l / 16) .. +8] and B[l % 16][8 (l / 16) .. +8]; result lane l holds
 // C[4 (l / 16) + v][l % 16], v = 0..3.
 #include <hip/hip_runtime.h>
@@ -38,45 +39,22 @@ __device__ __forceinline__ bf16x8 load8(const uint16_t *p) {
 // Epilogues
 enum { EPI_BIAS_SILU = MLP_EPI_BIAS_SILU, EPI_BIAS = MLP_EPI_BIAS, EPI_DSILU = MLP_EPI_DSILU, EPI_PARTIAL = MLP_EPI_PARTIAL };
 
+// Epilogue of one wave's 32 x 32 tile at (r0, c0) of C (acc as the MFMA result lanes hold it);
+// ks: the split-K range of a PARTIAL launch.
 template <int EPI>
-__global__ void __launch_bounds__(64) gemm_nt_kernel(const uint16_t *__restrict__ A, int lda,
-                                                     const uint16_t *__restrict__ B, int ldb,
-                                                     int R, int C, int K, int ksplit,
-                                                     const float *__restrict__ bias,
-                                                     const float *__restrict__ Zprev,
-                                                     float *__restrict__ Zout,
-                                                     uint16_t *__restrict__ Y, uint16_t *__restrict__ Yt,
-                                                     int ldy, int ldyt, int c_valid) {
-    const int tilesC = C / 32;
-    const int tile = blockIdx.x;
-    const int r0 = (tile / tilesC) * 32, c0 = (tile % tilesC) * 32;
-    const int l = threadIdx.x;
-    const int lr = l % 16, lk = 8 * (l / 16);
-    // split-K: blockIdx.y selects the K range
-    const int kper = K / ksplit;
-    const int kb = blockIdx.y * kper, ke = kb + kper;
-    f32x4 acc[2][2];
-    for (int i = 0; i < 2; i++)
-        for (int j = 0; j < 2; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-    const uint16_t *a0 = A + (size_t)(r0 + lr) * lda + lk;
-    const uint16_t *a1 = A + (size_t)(r0 + 16 + lr) * lda + lk;
-    const uint16_t *b0 = B + (size_t)(c0 + lr) * ldb + lk;
-    const uint16_t *b1 = B + (size_t)(c0 + 16 + lr) * ldb + lk;
-    for (int k = kb; k < ke; k += 32) {
-        const bf16x8 va0 = load8(a0 + k), va1 = load8(a1 + k);
-        const bf16x8 vb0 = load8(b0 + k), vb1 = load8(b1 + k);
-        acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va0, vb0, acc[0][0], 0, 0, 0);
-        acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va0, vb1, acc[0][1], 0, 0, 0);
-        acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va1, vb0, acc[1][0], 0, 0, 0);
-        acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va1, vb1, acc[1][1], 0, 0, 0);
-    }
+__device__ __forceinline__ void gemm_epilogue(const f32x4 (&acc)[2][2], int r0, int c0, int l, int ks, int R,
+                                              int C, const float *__restrict__ bias,
+                                              const float *__restrict__ Zprev, float *__restrict__ Zout,
+                                              uint16_t *__restrict__ Y, uint16_t *__restrict__ Yt, int ldy,
+                                              int ldyt, int c_valid) {
+    const int lr = l % 16;
     for (int i = 0; i < 2; i++)
         for (int j = 0; j < 2; j++) {
             const int c = c0 + 16 * j + lr;
             const int rb = r0 + 16 * i + 4 * (l / 16);
             float v[4] = {acc[i][j][0], acc[i][j][1], acc[i][j][2], acc[i][j][3]};
             if constexpr (EPI == EPI_PARTIAL) {
-                float *P = Zout + (size_t)blockIdx.y * R * C;
+                float *P = Zout + (size_t)ks * R * C;
                 for (int q = 0; q < 4; q++) P[(size_t)(rb + q) * C + c] = v[q];
             } else if constexpr (EPI == EPI_BIAS) {
                 const float bb = (c < c_valid) ? bias[c] : 0.f;
@@ -113,6 +91,106 @@ __global__ void __launch_bounds__(64) gemm_nt_kernel(const uint16_t *__restrict_
                 *reinterpret_cast<uint2 *>(Yt + (size_t)c * ldyt + rb) = pk;
             }
         }
+}
+
+template <int EPI>
+__global__ void __launch_bounds__(64) gemm_nt_kernel(const uint16_t *__restrict__ A, int lda,
+                                                     const uint16_t *__restrict__ B, int ldb,
+                                                     int R, int C, int K, int ksplit,
+                                                     const float *__restrict__ bias,
+                                                     const float *__restrict__ Zprev,
+                                                     float *__restrict__ Zout,
+                                                     uint16_t *__restrict__ Y, uint16_t *__restrict__ Yt,
+                                                     int ldy, int ldyt, int c_valid) {
+    const int tilesC = C / 32;
+    const int tile = blockIdx.x;
+    const int r0 = (tile / tilesC) * 32, c0 = (tile % tilesC) * 32;
+    const int l = threadIdx.x;
+    const int lr = l % 16, lk = 8 * (l / 16);
+    // split-K: blockIdx.y selects the K range
+    const int kper = K / ksplit;
+    const int kb = blockIdx.y * kper, ke = kb + kper;
+    f32x4 acc[2][2];
+    for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const uint16_t *a0 = A + (size_t)(r0 + lr) * lda + lk;
+    const uint16_t *a1 = A + (size_t)(r0 + 16 + lr) * lda + lk;
+    const uint16_t *b0 = B + (size_t)(c0 + lr) * ldb + lk;
+    const uint16_t *b1 = B + (size_t)(c0 + 16 + lr) * ldb + lk;
+    for (int k = kb; k < ke; k += 32) {
+        const bf16x8 va0 = load8(a0 + k), va1 = load8(a1 + k);
+        const bf16x8 vb0 = load8(b0 + k), vb1 = load8(b1 + k);
+        acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va0, vb0, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va0, vb1, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va1, vb0, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va1, vb1, acc[1][1], 0, 0, 0);
+    }
+    gemm_epilogue<EPI>(acc, r0, c0, l, blockIdx.y, R, C, bias, Zprev, Zout, Y, Yt, ldy, ldyt, c_valid);
+}
+
+// The same product for R and C multiples of 64: a workgroup of four waves computes a 64 x 64 block
+// (wave w the 32 x 32 tile (w / 2, w % 2)), staging each 32-wide K slice of its 64 rows of A and B
+// in LDS once (double-buffered, the next slice's global loads in flight during this slice's
+// products) instead of every wave reading its operands from L2: half the L2 reads of the one-wave
+// kernel. Same products in the same order per output, so the same results.
+constexpr int kLdsRow = 40;  // bf16 per staged row: 32 + 8 of padding (conflict-free 16-byte reads)
+
+template <int EPI>
+__global__ void __launch_bounds__(256) gemm_nt64_kernel(const uint16_t *__restrict__ A, int lda,
+                                                        const uint16_t *__restrict__ B, int ldb,
+                                                        int R, int C, int K, int ksplit,
+                                                        const float *__restrict__ bias,
+                                                        const float *__restrict__ Zprev,
+                                                        float *__restrict__ Zout,
+                                                        uint16_t *__restrict__ Y, uint16_t *__restrict__ Yt,
+                                                        int ldy, int ldyt, int c_valid) {
+    __shared__ __attribute__((aligned(16))) uint16_t sA[2][64 * kLdsRow];
+    __shared__ __attribute__((aligned(16))) uint16_t sB[2][64 * kLdsRow];
+    const int tilesC = C / 64;
+    const int R0 = (blockIdx.x / tilesC) * 64, C0 = (blockIdx.x % tilesC) * 64;
+    const int t = threadIdx.x, w = t / 64, l = t % 64;
+    const int lr = l % 16, lk = 8 * (l / 16);
+    const int kper = K / ksplit;
+    const int kb = blockIdx.y * kper, ke = kb + kper;
+    // staging: thread t moves 8 bf16 of row t / 4 of each operand's 64 x 32 slice
+    const int srow = t / 4, scol = 8 * (t % 4);
+    const uint16_t *ga = A + (size_t)(R0 + srow) * lda + scol;
+    const uint16_t *gb = B + (size_t)(C0 + srow) * ldb + scol;
+    const int sofs = srow * kLdsRow + scol;
+    *reinterpret_cast<bf16x8 *>(&sA[0][sofs]) = load8(ga + kb);
+    *reinterpret_cast<bf16x8 *>(&sB[0][sofs]) = load8(gb + kb);
+    __syncthreads();
+    const int fa = (32 * (w / 2) + lr) * kLdsRow + lk;  // this lane's fragment rows in the slice
+    const int fb = (32 * (w % 2) + lr) * kLdsRow + lk;
+    f32x4 acc[2][2];
+    for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    int buf = 0;
+    for (int k = kb; k < ke; k += 32) {
+        const bool more = k + 32 < ke;
+        bf16x8 na, nb;
+        if (more) {
+            na = load8(ga + k + 32);
+            nb = load8(gb + k + 32);
+        }
+        const bf16x8 va0 = *reinterpret_cast<const bf16x8 *>(&sA[buf][fa]);
+        const bf16x8 va1 = *reinterpret_cast<const bf16x8 *>(&sA[buf][fa + 16 * kLdsRow]);
+        const bf16x8 vb0 = *reinterpret_cast<const bf16x8 *>(&sB[buf][fb]);
+        const bf16x8 vb1 = *reinterpret_cast<const bf16x8 *>(&sB[buf][fb + 16 * kLdsRow]);
+        acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va0, vb0, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va0, vb1, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va1, vb0, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va1, vb1, acc[1][1], 0, 0, 0);
+        if (more) {
+            // the other buffer was last read before the previous barrier
+            *reinterpret_cast<bf16x8 *>(&sA[buf ^ 1][sofs]) = na;
+            *reinterpret_cast<bf16x8 *>(&sB[buf ^ 1][sofs]) = nb;
+        }
+        __syncthreads();
+        buf ^= 1;
+    }
+    gemm_epilogue<EPI>(acc, R0 + 32 * (w / 2), C0 + 32 * (w % 2), l, blockIdx.y, R, C, bias, Zprev, Zout, Y, Yt,
+                       ldy, ldyt, c_valid);
 }
 
 // db[m] = sum_n dZt[m][n] (one workgroup per m row).
@@ -302,23 +380,37 @@ hipError_t mlp_advance(int *d_step, hipStream_t s) {
     return hipGetLastError();
 }
 
+namespace {
+template <int EPI>
+void launch_gemm(const uint16_t *A, int lda, const uint16_t *B, int ldb, int R, int C, int K, int ksplit,
+                 const float *bias, const float *Zprev, float *Zout, uint16_t *Y, uint16_t *Yt, int ldy, int ldyt,
+                 int c_valid, hipStream_t s) {
+    if (R % 64 == 0 && C % 64 == 0) {
+        hipLaunchKernelGGL(gemm_nt64_kernel<EPI>, dim3((R / 64) * (C / 64), ksplit), dim3(256), 0, s, A, lda, B, ldb,
+                           R, C, K, ksplit, bias, Zprev, Zout, Y, Yt, ldy, ldyt, c_valid);
+    } else {
+        hipLaunchKernelGGL(gemm_nt_kernel<EPI>, dim3((R / 32) * (C / 32), ksplit), dim3(64), 0, s, A, lda, B, ldb,
+                           R, C, K, ksplit, bias, Zprev, Zout, Y, Yt, ldy, ldyt, c_valid);
+    }
+}
+}  // namespace
+
 hipError_t mlp_gemm(int epi, const uint16_t *A, int lda, const uint16_t *B, int ldb, int R, int C,
                     int K, int ksplit, const float *bias, const float *Zprev, float *Zout,
                     uint16_t *Y, uint16_t *Yt, int ldy, int ldyt, int c_valid, hipStream_t s) {
     if (R % 32 || C % 32 || K % 32 || ksplit < 1 || (K / 32) % ksplit) return hipErrorInvalidValue;
-    dim3 grid((R / 32) * (C / 32), ksplit);
     switch (epi) {
     case EPI_BIAS_SILU:
-        hipLaunchKernelGGL(gemm_nt_kernel<EPI_BIAS_SILU>, grid, dim3(64), 0, s, A, lda, B, ldb, R, C, K, ksplit, bias, Zprev, Zout, Y, Yt, ldy, ldyt, c_valid);
+        launch_gemm<EPI_BIAS_SILU>(A, lda, B, ldb, R, C, K, ksplit, bias, Zprev, Zout, Y, Yt, ldy, ldyt, c_valid, s);
         break;
     case EPI_BIAS:
-        hipLaunchKernelGGL(gemm_nt_kernel<EPI_BIAS>, grid, dim3(64), 0, s, A, lda, B, ldb, R, C, K, ksplit, bias, Zprev, Zout, Y, Yt, ldy, ldyt, c_valid);
+        launch_gemm<EPI_BIAS>(A, lda, B, ldb, R, C, K, ksplit, bias, Zprev, Zout, Y, Yt, ldy, ldyt, c_valid, s);
         break;
     case EPI_DSILU:
-        hipLaunchKernelGGL(gemm_nt_kernel<EPI_DSILU>, grid, dim3(64), 0, s, A, lda, B, ldb, R, C, K, ksplit, bias, Zprev, Zout, Y, Yt, ldy, ldyt, c_valid);
+        launch_gemm<EPI_DSILU>(A, lda, B, ldb, R, C, K, ksplit, bias, Zprev, Zout, Y, Yt, ldy, ldyt, c_valid, s);
         break;
     default:
-        hipLaunchKernelGGL(gemm_nt_kernel<EPI_PARTIAL>, grid, dim3(64), 0, s, A, lda, B, ldb, R, C, K, ksplit, bias, Zprev, Zout, Y, Yt, ldy, ldyt, c_valid);
+        launch_gemm<EPI_PARTIAL>(A, lda, B, ldb, R, C, K, ksplit, bias, Zprev, Zout, Y, Yt, ldy, ldyt, c_valid, s);
         break;
     }
     return hipGetLastError();
