@@ -51,7 +51,8 @@ struct cfd_mlp_s {
     float *gW = nullptr, *gb = nullptr;
     std::vector<uint16_t *> Wb, Wt, X, Xt, dZ, dZt;
     std::vector<float *> Z;
-    float *part = nullptr;
+    float *part = nullptr;          // split-K partials of every layer's weight gradient
+    cfd::MlpLayers tab{};           // shapes and buffers of all layers, for the all-layer launches
     float *norm = nullptr;  // in_mean[n_in], in_scale[n_in], out_mean[n_out], out_scale[n_out]
     std::vector<float> norm_h;
     float *feat = nullptr, *targ = nullptr, *lrow = nullptr;
@@ -85,9 +86,8 @@ struct cfd_mlp_s {
 
 namespace {
 
-int pack_all(cfd_mlp_s *m) {
-    for (int l = 0; l < m->L; l++)
-        MHIP(mlp_pack(m->W + m->woff[l], m->Mv[l], m->Kv[l], m->Mp[l], m->Kp[l], m->Wb[l], m->Wt[l], m->stream));
+int pack_all(cfd_mlp_s *m, int *d_step = nullptr) {
+    MHIP(mlp_pack_layers(m->W, m->tab, d_step, m->stream));
     return 0;
 }
 
@@ -109,10 +109,7 @@ int backward(cfd_mlp_s *m, int rows) {
     for (int l = m->L - 1; l >= 0; l--) {
         // dW_l = dZ_l^T X_l: A = dZt_l [Mp x rows], B = Xt_l [Kp x rows], split over rows
         MHIP(mlp_gemm(MLP_EPI_PARTIAL, m->dZt[l], rows, m->Xt[l], rows, m->Mp[l], m->Kp[l], rows, kSplit,
-                      nullptr, nullptr, m->part, nullptr, nullptr, 0, 0, m->Mp[l], m->stream));
-        MHIP(mlp_reduce_unpad(m->part, kSplit, m->Mp[l] * m->Kp[l], m->Kp[l], m->Mv[l], m->Kv[l],
-                              m->gW + m->woff[l], m->stream));
-        MHIP(mlp_row_sum(m->dZt[l], rows, m->Mv[l], rows, m->gb + m->boff[l], m->stream));
+                      nullptr, nullptr, m->part + m->tab.poff[l], nullptr, nullptr, 0, 0, m->Mp[l], m->stream));
         if (l > 0) {
             // dZ_{l-1} = (dZ_l W_l) * silu'(Z_{l-1}): A = dZ_l [rows x Mp], B = Wt_l [Kp x Mp]
             MHIP(mlp_gemm(MLP_EPI_DSILU, m->dZ[l], m->Mp[l], m->Wt[l], m->Mp[l], rows, m->Kp[l], m->Mp[l], 1,
@@ -120,6 +117,10 @@ int backward(cfd_mlp_s *m, int rows) {
                           m->Mv[l - 1], m->stream));
         }
     }
+    // every layer's split-K partials summed into gW, and every layer's bias gradient, in one
+    // launch each
+    MHIP(mlp_reduce_layers(m->part, kSplit, m->tab, m->gW, m->stream));
+    MHIP(mlp_row_sum_layers(m->tab, rows, m->gb, m->stream));
     return 0;
 }
 
@@ -149,7 +150,8 @@ extern "C" {
 
 int cfd_mlp_create(int n_in, int width, int depth, int n_out, int max_rows, unsigned long long seed,
                    cfd_mlp_t **out) {
-    if (!out || n_in <= 0 || n_in > 32 || width <= 0 || depth <= 0 || n_out <= 0 || n_out > 32 || max_rows <= 0)
+    if (!out || n_in <= 0 || n_in > 32 || width <= 0 || depth <= 0 || depth >= cfd::kMlpMaxLayers || n_out <= 0 ||
+        n_out > 32 || max_rows <= 0)
         return capi_fail(CFD_E_ARG, "cfd_mlp_create: bad arguments");
     *out = nullptr;
     cfd_mlp_s *m = new cfd_mlp_s();
@@ -173,12 +175,17 @@ int cfd_mlp_create(int n_in, int width, int depth, int n_out, int max_rows, unsi
     hipError_t e = hipSuccess;
     const int R = m->max_rows;
     int maxMK = 0;
-    for (int l = 0; l < m->L; l++) maxMK = std::max(maxMK, m->Mp[l] * m->Kp[l]);
+    size_t sumMK = 0;
+    for (int l = 0; l < m->L; l++) {
+        maxMK = std::max(maxMK, m->Mp[l] * m->Kp[l]);
+        m->tab.poff[l] = (long long)(kSplit * sumMK);
+        sumMK += (size_t)m->Mp[l] * m->Kp[l];
+    }
     bool ok = (e = m->alloc(&m->W, m->nw)) == hipSuccess && (e = m->alloc(&m->b, m->nb)) == hipSuccess &&
               (e = m->alloc(&m->mW, m->nw)) == hipSuccess && (e = m->alloc(&m->vW, m->nw)) == hipSuccess &&
               (e = m->alloc(&m->mb, m->nb)) == hipSuccess && (e = m->alloc(&m->vb, m->nb)) == hipSuccess &&
               (e = m->alloc(&m->gW, m->nw)) == hipSuccess && (e = m->alloc(&m->gb, m->nb)) == hipSuccess &&
-              (e = m->alloc(&m->part, (size_t)kSplit * maxMK)) == hipSuccess &&
+              (e = m->alloc(&m->part, (size_t)kSplit * sumMK)) == hipSuccess &&
               (e = m->alloc(&m->norm, 2 * (size_t)(n_in + n_out))) == hipSuccess &&
               (e = m->alloc(&m->lrow, (size_t)R)) == hipSuccess && (e = m->alloc(&m->idx, (size_t)R)) == hipSuccess &&
               (e = m->alloc(&m->d_step, 1)) == hipSuccess;
@@ -197,6 +204,18 @@ int cfd_mlp_create(int n_in, int width, int depth, int n_out, int max_rows, unsi
              (e = m->alloc(&m->Z[l], (size_t)R * m->Mp[l])) == hipSuccess &&
              (e = m->alloc(&m->dZ[l], (size_t)R * m->Mp[l])) == hipSuccess &&
              (e = m->alloc(&m->dZt[l], (size_t)R * m->Mp[l])) == hipSuccess;
+    }
+    m->tab.L = m->L;
+    for (int l = 0; ok && l < m->L; l++) {
+        m->tab.M[l] = m->Mv[l];
+        m->tab.K[l] = m->Kv[l];
+        m->tab.Mp[l] = m->Mp[l];
+        m->tab.Kp[l] = m->Kp[l];
+        m->tab.woff[l] = (long long)m->woff[l];
+        m->tab.boff[l] = (long long)m->boff[l];
+        m->tab.Wb[l] = m->Wb[l];
+        m->tab.Wt[l] = m->Wt[l];
+        m->tab.dZt[l] = m->dZt[l];
     }
     if (ok) e = hipStreamCreate(&m->stream);
     if (!ok || e != hipSuccess) {
@@ -349,11 +368,10 @@ int cfd_mlp_train(cfd_mlp_t *m, const float *feat, const float *targ, int n, int
             MHIP(mlp_loss(m->Z[Lm], m->Mp[Lm], m->targ, m->n_out, m->idx, batch, nrm(m, 2), nrm(m, 3), m->dZ[Lm],
                           m->dZt[Lm], batch, m->lrow, m->stream));
             if (int r = backward(m, batch)) return r;
-            MHIP(mlp_adam_dev(m->W, m->mW, m->vW, m->gW, (int)m->nw, m->sched, m->d_step, b1, b2, eps, m->stream));
-            MHIP(mlp_adam_dev(m->b, m->mb, m->vb, m->gb, (int)m->nb, m->sched, m->d_step, b1, b2, eps, m->stream));
-            if (int r = pack_all(m)) return r;
-            MHIP(mlp_advance(m->d_step, m->stream));
-            return 0;
+            MHIP(mlp_adam2_dev(m->W, m->mW, m->vW, m->gW, (int)m->nw, m->b, m->mb, m->vb, m->gb, (int)m->nb,
+                               m->sched, m->d_step, b1, b2, eps, m->stream));
+            // the repacking launch also advances the step index read by the next replay
+            return pack_all(m, m->d_step);
         };
         MHIP(hipStreamSynchronize(m->stream));
         MHIP(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
@@ -392,8 +410,8 @@ int cfd_mlp_train(cfd_mlp_t *m, const float *feat, const float *targ, int n, int
         if (rc) return rc;
         m->adam_t++;
         const float lrs = sch[3 * (size_t)s], c1 = sch[3 * (size_t)s + 1], c2 = sch[3 * (size_t)s + 2];
-        MHIP(mlp_adam(m->W, m->mW, m->vW, m->gW, (int)m->nw, lrs, b1, b2, eps, c1, c2, m->stream));
-        MHIP(mlp_adam(m->b, m->mb, m->vb, m->gb, (int)m->nb, lrs, b1, b2, eps, c1, c2, m->stream));
+        MHIP(mlp_adam2(m->W, m->mW, m->vW, m->gW, (int)m->nw, m->b, m->mb, m->vb, m->gb, (int)m->nb, lrs, b1, b2,
+                       eps, c1, c2, m->stream));
         rc = pack_all(m);
         if (rc) return rc;
         if ((rc = log_loss(s))) return rc;

@@ -13,6 +13,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "mlp_kernels.h"
 
 namespace cfd {
@@ -193,39 +195,6 @@ __global__ void __launch_bounds__(256) gemm_nt64_kernel(const uint16_t *__restri
                        ldy, ldyt, c_valid);
 }
 
-// db[m] = sum_n dZt[m][n] (one workgroup per m row).
-__global__ void __launch_bounds__(256) row_sum_kernel(const uint16_t *__restrict__ X, int ld, int n,
-                                                      float *__restrict__ out) {
-    __shared__ float red[256];
-    const int r = blockIdx.x, t = threadIdx.x;
-    // eight values per thread and step (one 16-byte load), four steps in flight; n is a multiple
-    // of 512, the partial sums combine in a fixed order
-    const uint16_t *row = X + (size_t)r * ld;
-    float s4[4] = {0.f, 0.f, 0.f, 0.f};
-    int j = 8 * t;
-    for (; j + 3 * 2048 < n; j += 4 * 2048)
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-            const uint4 v = *reinterpret_cast<const uint4 *>(row + j + u * 2048);
-            const uint32_t w[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-            for (int q = 0; q < 4; q++) s4[u] += bf2f((uint16_t)(w[q] & 0xffff)) + bf2f((uint16_t)(w[q] >> 16));
-        }
-    for (; j < n; j += 2048) {
-        const uint4 v = *reinterpret_cast<const uint4 *>(row + j);
-        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-        for (int q = 0; q < 4; q++) s4[0] += bf2f((uint16_t)(w[q] & 0xffff)) + bf2f((uint16_t)(w[q] >> 16));
-    }
-    const float s = (s4[0] + s4[1]) + (s4[2] + s4[3]);
-    red[t] = s;
-    __syncthreads();
-    for (int h = 128; h > 0; h >>= 1) {
-        if (t < h) red[t] += red[t + h];
-        __syncthreads();
-    }
-    if (t == 0) out[r] = red[0];
-}
 
 // Normalised input features (n_in of them) of minibatch rows: row r takes sample idx[r] (or r
 // when idx is null), padded with zeros to K0 columns; row-major X and transposed Xt.
@@ -275,44 +244,8 @@ __global__ void __launch_bounds__(256) loss_kernel(const float *__restrict__ Z, 
     lrow[r] = l / (float)n_out;
 }
 
-// Adam on FP32 master weights; refreshes the bf16 copies W [M][K] and Wt [K][M] (K valid columns
-// of K_pad, M valid rows of M_pad; padding stays zero).
-__global__ void __launch_bounds__(256) adam_kernel(float *__restrict__ W, float *__restrict__ m1,
-                                                   float *__restrict__ m2, const float *__restrict__ g,
-                                                   int n, float lr, float b1, float b2, float eps,
-                                                   float c1, float c2) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float gi = g[i];
-    const float a = b1 * m1[i] + (1.0f - b1) * gi;
-    const float b = b2 * m2[i] + (1.0f - b2) * gi * gi;
-    m1[i] = a;
-    m2[i] = b;
-    W[i] -= lr * (a * c1) / (sqrtf(b * c2) + eps);
-}
 
-__global__ void __launch_bounds__(256) pack_weights_kernel(const float *__restrict__ W, int M, int K,
-                                                           int Mp, int Kp, uint16_t *__restrict__ Wb,
-                                                           uint16_t *__restrict__ Wt) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= Mp * Kp) return;
-    const int m = i / Kp, k = i % Kp;
-    const uint16_t h = (m < M && k < K) ? f2bf(W[(size_t)m * K + k]) : (uint16_t)0;
-    Wb[i] = h;
-    Wt[(size_t)k * Mp + m] = h;
-}
 
-// dW[m][k] = sum_s P[s][m][k] over the valid [M x K] block of the padded [Mp x Kp] partials
-// (fixed order), written unpadded.
-__global__ void __launch_bounds__(256) reduce_unpad_kernel(const float *__restrict__ P, int S, int MKp, int Kp,
-                                                           int M, int K, float *__restrict__ out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= M * K) return;
-    const size_t o = (size_t)(i / K) * Kp + (i % K);
-    float s = 0.f;
-    for (int q = 0; q < S; q++) s += P[(size_t)q * MKp + o];
-    out[i] = s;
-}
 
 // Minibatch indices from a counter-based hash (SplitMix64) of (seed, step, row).
 __global__ void __launch_bounds__(256) sample_kernel(unsigned long long seed, int step, int rows,
@@ -326,9 +259,8 @@ __global__ void __launch_bounds__(256) sample_kernel(unsigned long long seed, in
     idx[r] = (int)(z % (unsigned long long)n);
 }
 
-// The device-side step of a captured training step: the minibatch of step *d_step, and Adam with
-// that step's learning rate and bias corrections sched[3 step .. 3 step + 2]; advance_kernel moves
-// to the next step at the end of the captured step.
+// The minibatch of a captured training step's step *d_step (the step's Adam reads its rates
+// from sched by the same index, and the step's last launch, the weight repacking, advances it).
 __global__ void __launch_bounds__(256) sample_dev_kernel(unsigned long long seed, const int *__restrict__ d_step,
                                                          int rows, int n, int *__restrict__ idx) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -341,15 +273,11 @@ __global__ void __launch_bounds__(256) sample_dev_kernel(unsigned long long seed
     idx[r] = (int)(z % (unsigned long long)n);
 }
 
-__global__ void __launch_bounds__(256) adam_dev_kernel(float *__restrict__ W, float *__restrict__ m1,
-                                                       float *__restrict__ m2, const float *__restrict__ g,
-                                                       int n, const float *__restrict__ sched,
-                                                       const int *__restrict__ d_step, float b1, float b2,
-                                                       float eps) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int step = *d_step;
-    const float lr = sched[3 * step], c1 = sched[3 * step + 1], c2 = sched[3 * step + 2];
+
+// ---- all layers in one launch ---------------------------------------------------------------
+// One Adam element update: moments m1, m2 and W -= lr (m1 c1) / (sqrt(m2 c2) + eps).
+__device__ __forceinline__ void adam_one(float *W, float *m1, float *m2, const float *g, int i, float lr,
+                                         float b1, float b2, float eps, float c1, float c2) {
     const float gi = g[i];
     const float a = b1 * m1[i] + (1.0f - b1) * gi;
     const float b = b2 * m2[i] + (1.0f - b2) * gi * gi;
@@ -358,9 +286,92 @@ __global__ void __launch_bounds__(256) adam_dev_kernel(float *__restrict__ W, fl
     W[i] -= lr * (a * c1) / (sqrtf(b * c2) + eps);
 }
 
-__global__ void __launch_bounds__(64) advance_kernel(int *d_step) {
-    if (threadIdx.x == 0) *d_step += 1;
+// Adam over weights [0, nw) and biases [nw, nw + nb) in one grid; with d_step the step's rates
+// are sched[3 step .. 3 step + 2], else lr, c1, c2.
+__global__ void __launch_bounds__(256) adam2_kernel(float *__restrict__ W, float *__restrict__ mW,
+                                                    float *__restrict__ vW, const float *__restrict__ gW,
+                                                    int nw, float *__restrict__ b, float *__restrict__ mb,
+                                                    float *__restrict__ vb, const float *__restrict__ gb,
+                                                    int nb, float lr, float c1, float c2,
+                                                    const float *__restrict__ sched,
+                                                    const int *__restrict__ d_step, float b1, float b2,
+                                                    float eps) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nw + nb) return;
+    if (d_step) {
+        const int step = *d_step;
+        lr = sched[3 * step];
+        c1 = sched[3 * step + 1];
+        c2 = sched[3 * step + 2];
+    }
+    if (i < nw) adam_one(W, mW, vW, gW, i, lr, b1, b2, eps, c1, c2);
+    else adam_one(b, mb, vb, gb, i - nw, lr, b1, b2, eps, c1, c2);
 }
+
+// W -> Wb [Mp x Kp] and Wt [Kp x Mp] (bf16, zero padding) of layer blockIdx.y; block (0, 0) also
+// advances the captured step's index when d_step is given (nothing in this launch reads it).
+__global__ void __launch_bounds__(256) pack_layers_kernel(const float *__restrict__ W, MlpLayers t, int *d_step) {
+    const int l = blockIdx.y;
+    if (d_step && l == 0 && blockIdx.x == 0 && threadIdx.x == 0) *d_step += 1;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int Mp = t.Mp[l], Kp = t.Kp[l], M = t.M[l], K = t.K[l];
+    if (i >= Mp * Kp) return;
+    const int m = i / Kp, k = i % Kp;
+    const uint16_t h = (m < M && k < K) ? f2bf(W[t.woff[l] + (size_t)m * K + k]) : (uint16_t)0;
+    t.Wb[l][i] = h;
+    t.Wt[l][(size_t)k * Mp + m] = h;
+}
+
+// Split-K partial sums of layer blockIdx.y, in split order, into its unpadded gradient.
+__global__ void __launch_bounds__(256) reduce_layers_kernel(const float *__restrict__ P, int S, MlpLayers t,
+                                                            float *__restrict__ gW) {
+    const int l = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int M = t.M[l], K = t.K[l], Kp = t.Kp[l];
+    if (i >= M * K) return;
+    const size_t MKp = (size_t)t.Mp[l] * Kp;
+    const float *Pl = P + t.poff[l];
+    const size_t o = (size_t)(i / K) * Kp + (i % K);
+    float s = 0.f;
+    for (int q = 0; q < S; q++) s += Pl[(size_t)q * MKp + o];
+    gW[t.woff[l] + i] = s;
+}
+
+// Bias gradients of every layer: block b is row b of the layers' rows in turn; eight values per
+// thread and step (one 16-byte load), four steps in flight (n a multiple of 512), the partial sums
+// combined in a fixed order.
+__global__ void __launch_bounds__(256) row_sum_layers_kernel(MlpLayers t, int n, float *__restrict__ gb) {
+    __shared__ float red[256];
+    int r = blockIdx.x, l = 0;
+    while (l < t.L - 1 && r >= t.M[l]) r -= t.M[l++];
+    const int tt = threadIdx.x;
+    const uint16_t *row = t.dZt[l] + (size_t)r * n;
+    float s4[4] = {0.f, 0.f, 0.f, 0.f};
+    int j = 8 * tt;
+    for (; j + 3 * 2048 < n; j += 4 * 2048)
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const uint4 v = *reinterpret_cast<const uint4 *>(row + j + u * 2048);
+            const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int q = 0; q < 4; q++) s4[u] += bf2f((uint16_t)(w[q] & 0xffff)) + bf2f((uint16_t)(w[q] >> 16));
+        }
+    for (; j < n; j += 2048) {
+        const uint4 v = *reinterpret_cast<const uint4 *>(row + j);
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int q = 0; q < 4; q++) s4[0] += bf2f((uint16_t)(w[q] & 0xffff)) + bf2f((uint16_t)(w[q] >> 16));
+    }
+    const float sum = (s4[0] + s4[1]) + (s4[2] + s4[3]);
+    red[tt] = sum;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (tt < h) red[tt] += red[tt + h];
+        __syncthreads();
+    }
+    if (tt == 0) gb[t.boff[l] + r] = red[0];
+}
+
 
 }  // namespace
 
@@ -369,14 +380,47 @@ hipError_t mlp_sample_dev(unsigned long long seed, const int *d_step, int rows, 
     return hipGetLastError();
 }
 
-hipError_t mlp_adam_dev(float *W, float *m1, float *m2, const float *g, int n, const float *sched,
-                        const int *d_step, float b1, float b2, float eps, hipStream_t s) {
-    hipLaunchKernelGGL(adam_dev_kernel, dim3((n + 255) / 256), dim3(256), 0, s, W, m1, m2, g, n, sched, d_step, b1, b2, eps);
+namespace {
+int max_layer(const MlpLayers &t, bool padded) {
+    int v = 0;
+    for (int l = 0; l < t.L; l++) v = std::max(v, padded ? t.Mp[l] * t.Kp[l] : t.M[l] * t.K[l]);
+    return v;
+}
+}  // namespace
+
+hipError_t mlp_pack_layers(const float *W, const MlpLayers &t, int *d_step, hipStream_t s) {
+    if (t.L < 1 || t.L > kMlpMaxLayers) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(pack_layers_kernel, dim3((max_layer(t, true) + 255) / 256, t.L), dim3(256), 0, s, W, t, d_step);
     return hipGetLastError();
 }
 
-hipError_t mlp_advance(int *d_step, hipStream_t s) {
-    hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(64), 0, s, d_step);
+hipError_t mlp_reduce_layers(const float *P, int S, const MlpLayers &t, float *gW, hipStream_t s) {
+    if (t.L < 1 || t.L > kMlpMaxLayers) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(reduce_layers_kernel, dim3((max_layer(t, false) + 255) / 256, t.L), dim3(256), 0, s, P, S, t, gW);
+    return hipGetLastError();
+}
+
+hipError_t mlp_row_sum_layers(const MlpLayers &t, int n, float *gb, hipStream_t s) {
+    if (t.L < 1 || t.L > kMlpMaxLayers || n % 512) return hipErrorInvalidValue;
+    int rows = 0;
+    for (int l = 0; l < t.L; l++) rows += t.M[l];
+    hipLaunchKernelGGL(row_sum_layers_kernel, dim3(rows), dim3(256), 0, s, t, n, gb);
+    return hipGetLastError();
+}
+
+hipError_t mlp_adam2(float *W, float *mW, float *vW, const float *gW, int nw, float *b, float *mb, float *vb,
+                     const float *gb, int nb, float lr, float b1, float b2, float eps, float c1, float c2,
+                     hipStream_t s) {
+    hipLaunchKernelGGL(adam2_kernel, dim3((nw + nb + 255) / 256), dim3(256), 0, s, W, mW, vW, gW, nw, b, mb, vb, gb, nb,
+                       lr, c1, c2, nullptr, nullptr, b1, b2, eps);
+    return hipGetLastError();
+}
+
+hipError_t mlp_adam2_dev(float *W, float *mW, float *vW, const float *gW, int nw, float *b, float *mb, float *vb,
+                         const float *gb, int nb, const float *sched, const int *d_step, float b1, float b2,
+                         float eps, hipStream_t s) {
+    hipLaunchKernelGGL(adam2_kernel, dim3((nw + nb + 255) / 256), dim3(256), 0, s, W, mW, vW, gW, nw, b, mb, vb, gb, nb,
+                       0.f, 0.f, 0.f, sched, d_step, b1, b2, eps);
     return hipGetLastError();
 }
 
@@ -416,11 +460,6 @@ hipError_t mlp_gemm(int epi, const uint16_t *A, int lda, const uint16_t *B, int 
     return hipGetLastError();
 }
 
-hipError_t mlp_row_sum(const uint16_t *X, int ld, int rows, int n, float *out, hipStream_t s) {
-    hipLaunchKernelGGL(row_sum_kernel, dim3(rows), dim3(256), 0, s, X, ld, n, out);
-    return hipGetLastError();
-}
-
 hipError_t mlp_encode(const float *feat, int n_in, const int *idx, int rows, const float *mean,
                       const float *scale, int K0, uint16_t *X, uint16_t *Xt, int ldxt, hipStream_t s) {
     hipLaunchKernelGGL(encode_kernel, dim3((rows + 255) / 256), dim3(256), 0, s, feat, n_in, idx, rows, mean, scale, K0, X, Xt, ldxt);
@@ -431,22 +470,6 @@ hipError_t mlp_loss(const float *Z, int C, const float *targ, int n_out, const i
                     const float *mean, const float *scale, uint16_t *dZ, uint16_t *dZt, int lddzt,
                     float *lrow, hipStream_t s) {
     hipLaunchKernelGGL(loss_kernel, dim3((rows + 255) / 256), dim3(256), 0, s, Z, C, targ, n_out, idx, rows, mean, scale, dZ, dZt, lddzt, lrow);
-    return hipGetLastError();
-}
-
-hipError_t mlp_adam(float *W, float *m1, float *m2, const float *g, int n, float lr, float b1,
-                    float b2, float eps, float c1, float c2, hipStream_t s) {
-    hipLaunchKernelGGL(adam_kernel, dim3((n + 255) / 256), dim3(256), 0, s, W, m1, m2, g, n, lr, b1, b2, eps, c1, c2);
-    return hipGetLastError();
-}
-
-hipError_t mlp_pack(const float *W, int M, int K, int Mp, int Kp, uint16_t *Wb, uint16_t *Wt, hipStream_t s) {
-    hipLaunchKernelGGL(pack_weights_kernel, dim3((Mp * Kp + 255) / 256), dim3(256), 0, s, W, M, K, Mp, Kp, Wb, Wt);
-    return hipGetLastError();
-}
-
-hipError_t mlp_reduce_unpad(const float *P, int S, int MKp, int Kp, int M, int K, float *out, hipStream_t s) {
-    hipLaunchKernelGGL(reduce_unpad_kernel, dim3((M * K + 255) / 256), dim3(256), 0, s, P, S, MKp, Kp, M, K, out);
     return hipGetLastError();
 }
 

@@ -243,7 +243,7 @@ int cfd_batch_history(const cfd_batch_t *b, int *nchecks, double *e0);
 int cfd_batch_time_steps(cfd_batch_t *b, int nsteps, float *ms);
 
 /* ---- coordinate-MLP surrogate (warm start) ---------------------------------------------------
- * n_in features -> depth SiLU hidden layers of `width` -> n_out outputs; BF16 weights and
+ * n_in features -> depth SiLU hidden layers of `width` (depth 1..15) -> n_out outputs; BF16 weights and
  * activations on MFMA with FP32 accumulation; Adam on FP32 master weights. Parameters are
  * concatenated per layer: W_l row-major [out x in], then b_l. Features and targets are float
  * [n x n_in] / [n x n_out] host arrays, normalised inside as (v - mean) * scale (targets and
