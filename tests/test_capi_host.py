@@ -123,6 +123,18 @@ def test_batch_rejects_bad_tile_order():
             Batch([m], np.array([[1.4, 2.0, 0.0, 1.0]]), cfg)
 
 
+def test_mlp_rejects_bad_shapes():
+    """cfd_mlp_create refuses shapes its kernels do not take (checked before any device work):
+    more than 15 hidden layers (the per-step update treats at most 16 weight layers in one launch),
+    more than 32 inputs or outputs."""
+    from cfd_solver_withcuda_amd._lib import CfdError
+    from cfd_solver_withcuda_amd.surrogate import MLP
+
+    for args in ((4, 32, 16, 4), (33, 32, 2, 4), (4, 32, 2, 33), (4, 32, 0, 4)):
+        with pytest.raises(CfdError, match="cfd_mlp_create"):
+            MLP(*args, max_rows=512)
+
+
 def test_host_driver_exports_declared_names():
     """libcfdhost.so defines every function include/cfd_host.hpp declares (the reference's driver
     names, including the dissipation plugin point)."""
