@@ -59,7 +59,7 @@ struct cfd_mlp_s {
     int *idx = nullptr;
     size_t data_cap = 0;
     int adam_t = 0;
-    int graph_mode = 1;             // training steps: 1 one captured graph per step, 0 launch by launch
+    int graph_mode = 0;             // training steps: 0 launch by launch (faster here), 1 one captured graph per step
     float *sched = nullptr;         // [3 * steps] learning rate and Adam bias corrections per step
     size_t sched_cap = 0;
     int *d_step = nullptr;          // step of the captured graph's next replay

@@ -135,6 +135,87 @@ __global__ void __launch_bounds__(64) gemm_nt_kernel(const uint16_t *__restrict_
 // in LDS once (double-buffered, the next slice's global loads in flight during this slice's
 // products) instead of every wave reading its operands from L2: half the L2 reads of the one-wave
 // kernel. Same products in the same order per output, so the same results.
+// Epilogue of a 64 x 64 block through LDS: the four waves' results (MFMA lane layout) go to a
+// staged [64][65] float array S first, and each output is then written by whole 16-byte stores of
+// consecutive elements: the fp32 rows of Z (or of the split-K partial), the bf16 rows of Y and the
+// bf16 rows of Yt (columns of the block). Same values as gemm_epilogue.
+constexpr int kStageRow = 65;
+
+template <int EPI>
+__device__ __forceinline__ void gemm64_epilogue(const f32x4 (&acc)[2][2], float *S, int R0, int C0, int w, int l,
+                                                int t, int ks, int R, int C, const float *__restrict__ bias,
+                                                const float *__restrict__ Zprev, float *__restrict__ Zout,
+                                                uint16_t *__restrict__ Y, uint16_t *__restrict__ Yt, int ldy,
+                                                int ldyt, int c_valid) {
+    // 1. results into S (the loop's last barrier ended every read of the operand slices)
+    for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++) {
+            const int c = 32 * (w % 2) + 16 * j + l % 16;
+            const int r = 32 * (w / 2) + 16 * i + 4 * (l / 16);
+            float bb = 0.f;
+            bool cv = true;
+            if constexpr (EPI == EPI_BIAS_SILU || EPI == EPI_BIAS) {
+                cv = C0 + c < c_valid;
+                bb = bias[cv ? C0 + c : 0];
+            }
+            for (int q = 0; q < 4; q++) {
+                float v = acc[i][j][q];
+                if constexpr (EPI == EPI_BIAS_SILU || EPI == EPI_BIAS) v = cv ? v + bb : 0.f;
+                S[(r + q) * kStageRow + c] = v;
+            }
+        }
+    __syncthreads();
+    if constexpr (EPI == EPI_PARTIAL || EPI == EPI_BIAS || EPI == EPI_BIAS_SILU) {
+        // 2. fp32 rows: four consecutive values per thread and pass
+        float *out = (EPI == EPI_PARTIAL) ? Zout + (size_t)ks * R * C : Zout;
+        for (int e = t; e < 64 * 16; e += 256) {
+            const int r = e / 16, c = 4 * (e % 16);
+            float *sr = S + r * kStageRow + c;
+            *reinterpret_cast<float4 *>(out + (size_t)(R0 + r) * C + C0 + c) = make_float4(sr[0], sr[1], sr[2], sr[3]);
+            if constexpr (EPI == EPI_BIAS_SILU) {
+                // the activation replaces the pre-activation in S (one SiLU per element)
+                for (int q = 0; q < 4; q++) {
+                    const float x = sr[q];
+                    sr[q] = (C0 + c + q < c_valid) ? x * sigmoidf_(x) : 0.f;
+                }
+            }
+        }
+        if constexpr (EPI == EPI_BIAS_SILU) __syncthreads();
+    }
+    if constexpr (EPI == EPI_DSILU) {
+        // 2. dZprev = acc * silu'(Zprev) in place in S, read by rows with Zprev (padding columns
+        // of Zprev hold zero; their results are selected to zero)
+        for (int e = t; e < 64 * 16; e += 256) {
+            const int r = e / 16, c = 4 * (e % 16);
+            const float4 z4 = *reinterpret_cast<const float4 *>(Zprev + (size_t)(R0 + r) * C + C0 + c);
+            const float zz[4] = {z4.x, z4.y, z4.z, z4.w};
+            float *sr = S + r * kStageRow + c;
+            for (int q = 0; q < 4; q++) {
+                const float z = zz[q];
+                const float sg = sigmoidf_(z);
+                sr[q] = (C0 + c + q < c_valid) ? sr[q] * (sg * (1.0f + z * (1.0f - sg))) : 0.f;
+            }
+        }
+        __syncthreads();
+    }
+    if constexpr (EPI == EPI_BIAS_SILU || EPI == EPI_DSILU) {
+        // 3. bf16 rows of Y and of Yt: eight consecutive values per thread and pass
+        auto val = [&](int r, int c) -> uint16_t { return f2bf(S[r * kStageRow + c]); };
+        for (int e = t; e < 64 * 8; e += 256) {
+            const int r = e / 8, c = 8 * (e % 8);
+            uint32_t pk[4];
+            for (int q = 0; q < 4; q++) pk[q] = (uint32_t)val(r, c + 2 * q) | ((uint32_t)val(r, c + 2 * q + 1) << 16);
+            *reinterpret_cast<uint4 *>(Y + (size_t)(R0 + r) * ldy + C0 + c) = make_uint4(pk[0], pk[1], pk[2], pk[3]);
+        }
+        for (int e = t; e < 64 * 8; e += 256) {
+            const int c = e / 8, r = 8 * (e % 8);
+            uint32_t pk[4];
+            for (int q = 0; q < 4; q++) pk[q] = (uint32_t)val(r + 2 * q, c) | ((uint32_t)val(r + 2 * q + 1, c) << 16);
+            *reinterpret_cast<uint4 *>(Yt + (size_t)(C0 + c) * ldyt + R0 + r) = make_uint4(pk[0], pk[1], pk[2], pk[3]);
+        }
+    }
+}
+
 constexpr int kLdsRow = 40;  // bf16 per staged row: 32 + 8 of padding (conflict-free 16-byte reads)
 
 template <int EPI>
@@ -146,8 +227,10 @@ __global__ void __launch_bounds__(256) gemm_nt64_kernel(const uint16_t *__restri
                                                         float *__restrict__ Zout,
                                                         uint16_t *__restrict__ Y, uint16_t *__restrict__ Yt,
                                                         int ldy, int ldyt, int c_valid) {
-    __shared__ __attribute__((aligned(16))) uint16_t sA[2][64 * kLdsRow];
-    __shared__ __attribute__((aligned(16))) uint16_t sB[2][64 * kLdsRow];
+    // the K slices' operands; after the loop the same LDS stages the 64 x 64 result
+    __shared__ __attribute__((aligned(16))) uint16_t sbuf[4 * 64 * kLdsRow];
+    uint16_t(*sA)[64 * kLdsRow] = reinterpret_cast<uint16_t(*)[64 * kLdsRow]>(sbuf);
+    uint16_t(*sB)[64 * kLdsRow] = reinterpret_cast<uint16_t(*)[64 * kLdsRow]>(sbuf + 2 * 64 * kLdsRow);
     const int tilesC = C / 64;
     const int R0 = (blockIdx.x / tilesC) * 64, C0 = (blockIdx.x % tilesC) * 64;
     const int t = threadIdx.x, w = t / 64, l = t % 64;
@@ -191,8 +274,8 @@ __global__ void __launch_bounds__(256) gemm_nt64_kernel(const uint16_t *__restri
         __syncthreads();
         buf ^= 1;
     }
-    gemm_epilogue<EPI>(acc, R0 + 32 * (w / 2), C0 + 32 * (w % 2), l, blockIdx.y, R, C, bias, Zprev, Zout, Y, Yt,
-                       ldy, ldyt, c_valid);
+    gemm64_epilogue<EPI>(acc, reinterpret_cast<float *>(sbuf), R0, C0, w, l, t, blockIdx.y, R, C, bias, Zprev, Zout,
+                         Y, Yt, ldy, ldyt, c_valid);
 }
 
 

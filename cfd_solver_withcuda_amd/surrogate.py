@@ -79,7 +79,8 @@ class MLP:
         return hist
 
     def set_launch(self, mode):
-        """Training steps: "graph" (one captured step, replayed; default) or "kernels"."""
+        """Training steps: "kernels" (launched one by one; default) or "graph" (one captured step,
+        replayed). Results are identical."""
         m = {"kernels": 0, "graph": 1}[mode] if isinstance(mode, str) else int(mode)
         check(lib().cfd_mlp_set_launch(self.h, m), "cfd_mlp_set_launch")
 
@@ -146,7 +147,7 @@ def predicted_state(mlp: MLP, points, cells, mach, thickness, floor=1e-3, fs=Non
 __all__ = ["MLP", "features", "cell_centres", "prim_from_cons", "cons_from_prim_arr", "predicted_state"]
 
 
-def _time_training(steps, batch=16384, samples=1 << 20, launch="graph"):
+def _time_training(steps, batch=16384, samples=1 << 20, launch="kernels"):
     """Wall time per training step of the 4 -> 6 x 256 -> 4 network on random samples of a smooth
     field (5 untimed steps first)."""
     import time
@@ -173,6 +174,6 @@ if __name__ == "__main__":
     ap = argparse.ArgumentParser(description="time the surrogate's training step on the GPU")
     ap.add_argument("--time", type=int, default=200, help="timed training steps")
     ap.add_argument("--batch", type=int, default=16384)
-    ap.add_argument("--launch", default="graph", choices=["graph", "kernels"])
+    ap.add_argument("--launch", default="kernels", choices=["graph", "kernels"])
     a = ap.parse_args()
     print(json.dumps(_time_training(a.time, a.batch, launch=a.launch)))

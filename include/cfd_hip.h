@@ -267,9 +267,9 @@ int cfd_mlp_gradients(cfd_mlp_t *m, const float *feat, const float *targ, int n,
  * (may be NULL) records the minibatch loss of every log_every-th step. */
 int cfd_mlp_train(cfd_mlp_t *m, const float *feat, const float *targ, int n, int steps, int batch,
                   float lr, float lr_final, unsigned long long seed, float *loss_hist, int log_every);
-/* How cfd_mlp_train launches a step: 1 (default) one training step captured in a hipGraph and
- * replayed (step index, learning rate and bias corrections read from device memory), 0 kernel by
- * kernel. Results are identical. */
+/* How cfd_mlp_train launches a step: 0 (default) kernel by kernel, 1 one training step captured
+ * in a hipGraph and replayed (step index, learning rate and bias corrections read from device
+ * memory; for callers that replay it themselves). Results are identical. */
 int cfd_mlp_set_launch(cfd_mlp_t *m, int mode);
 /* Predictions out[n x n_out] (physical units) for feat[n x n_in]. */
 int cfd_mlp_predict(cfd_mlp_t *m, const float *feat, int n, float *out);
