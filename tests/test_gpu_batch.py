@@ -250,6 +250,30 @@ def test_persistent_launch_equals_tile_launch_as_instances_converge():
         assert np.array_equal(U0[i], U1[i]), i
 
 
+def test_tile_order_over_xcds_changes_nothing_as_instances_converge():
+    """The order in which tiles are dealt to the XCDs (xcd_remap: one contiguous run per XCD, runs
+    of 3 or of 16 tiles in turn) changes neither which tiles a step computes nor its results, also
+    once converged instances have left the compacted tile list and the launch grids have shrunk:
+    32 uneven ramps, first order to Error[0] <= 1e-4, tile and persistent launches."""
+    from cfd_solver_withcuda_amd import cases
+
+    cs = cases.ramp_batch(32, seed=43, min_cells=1000, max_cells=60000)
+    meshes, fs = [c.mesh for c in cs], np.array([c.fs for c in cs])
+    print(f"case tile orders, 32 ramps, {sum(len(m.cells) for m in meshes)} cells", flush=True)
+    runs = []
+    for remap, mode in ((1, "tiles"), (3, "tiles"), (16, "tiles"), (3, "persistent")):
+        b = Batch(meshes, fs, make_config(block=128, xcd_remap=remap, **kw(order=1)))
+        b.set_launch(mode)
+        r = b.solve(tol=1e-4, max_steps=3000, check_every=50)
+        runs.append((r, b.get_state()))
+    r0, U0 = runs[0]
+    assert len(set(r0["steps"].tolist())) > 4  # instances converge at different steps
+    for r, U in runs[1:]:
+        assert np.array_equal(r0["steps"], r["steps"]) and np.array_equal(r0["err"], r["err"])
+        for i in range(len(cs)):
+            assert np.array_equal(U0[i], U[i]), i
+
+
 def test_persistent_launch_rk4_and_limits():
     """RK4 stages under persistent workgroups equal the tile launch; order 3 and viscous batches
     refuse persistent workgroups, and unknown modes are refused."""
