@@ -577,7 +577,7 @@ def fv_bench(args, wl, scheme, gradient, sub=False):
     contract = residual_contract(mesh.nphys + mesh.nghost, F, cfg["gradient"], ms_grad + ms_resid)
     contract_other = None
     if ffs and not sub:
-        # the same residual with the other gradient (SURVEY's roofline run names both)
+        # the same residual with the other gradient (the step-mesh roofline run is defined for both)
         other = "gg" if cfg["gradient"] == "wlsq" else "wlsq"
         s.configure(mk(**dict(cfg, gradient=other)))
         g2 = s.time_pass("grad", args.kernel_reps, stream=stream) / args.kernel_reps
