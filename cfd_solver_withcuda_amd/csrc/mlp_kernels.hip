@@ -290,11 +290,18 @@ __global__ void __launch_bounds__(256) encode_kernel(const float *__restrict__ f
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= rows) return;
     const int s = idx ? idx[r] : r;
-    for (int k = 0; k < K0; k++) {
-        uint16_t h = 0;
-        if (k < n_in) h = f2bf((feat[(size_t)s * n_in + k] - mean[k]) * scale[k]);
-        X[(size_t)r * K0 + k] = h;
-        Xt[(size_t)k * ldxt + r] = h;
+    // the row of X eight values (16 bytes) at a time; the columns of Xt one value per row, so
+    // that neighbouring rows' threads write neighbouring bytes (K0 is a multiple of 32)
+    for (int k0 = 0; k0 < K0; k0 += 8) {
+        uint32_t pk[4];
+        for (int q = 0; q < 8; q++) {
+            const int k = k0 + q;
+            uint16_t h = 0;
+            if (k < n_in) h = f2bf((feat[(size_t)s * n_in + k] - mean[k]) * scale[k]);
+            Xt[(size_t)k * ldxt + r] = h;
+            pk[q / 2] = (q % 2) ? (pk[q / 2] | ((uint32_t)h << 16)) : (uint32_t)h;
+        }
+        *reinterpret_cast<uint4 *>(X + (size_t)r * K0 + k0) = make_uint4(pk[0], pk[1], pk[2], pk[3]);
     }
 }
 
@@ -312,17 +319,23 @@ __global__ void __launch_bounds__(256) loss_kernel(const float *__restrict__ Z, 
     const int s = idx ? idx[r] : r;
     const float inv = 2.0f / (float)(rows * n_out);
     float l = 0.f;
-    for (int c = 0; c < C; c++) {
-        float d = 0.f;
-        if (c < n_out) {
-            const float t = (targ[(size_t)s * n_out + c] - mean[c]) * scale[c];
-            const float e = Z[(size_t)r * C + c] - t;
-            l += e * e;
-            d = inv * e;
+    // the row of dZ eight values (16 bytes) at a time (C is a multiple of 32), dZt as encode's Xt
+    for (int c0 = 0; c0 < C; c0 += 8) {
+        uint32_t pk[4];
+        for (int q = 0; q < 8; q++) {
+            const int c = c0 + q;
+            float d = 0.f;
+            if (c < n_out) {
+                const float t = (targ[(size_t)s * n_out + c] - mean[c]) * scale[c];
+                const float e = Z[(size_t)r * C + c] - t;
+                l += e * e;
+                d = inv * e;
+            }
+            const uint16_t h = f2bf(d);
+            dZt[(size_t)c * lddzt + r] = h;
+            pk[q / 2] = (q % 2) ? (pk[q / 2] | ((uint32_t)h << 16)) : (uint32_t)h;
         }
-        const uint16_t h = f2bf(d);
-        dZ[(size_t)r * C + c] = h;
-        dZt[(size_t)c * lddzt + r] = h;
+        *reinterpret_cast<uint4 *>(dZ + (size_t)r * C + c0) = make_uint4(pk[0], pk[1], pk[2], pk[3]);
     }
     lrow[r] = l / (float)n_out;
 }
@@ -545,6 +558,7 @@ hipError_t mlp_gemm(int epi, const uint16_t *A, int lda, const uint16_t *B, int 
 
 hipError_t mlp_encode(const float *feat, int n_in, const int *idx, int rows, const float *mean,
                       const float *scale, int K0, uint16_t *X, uint16_t *Xt, int ldxt, hipStream_t s) {
+    if (K0 % 8) return hipErrorInvalidValue;  // the kernel writes X rows 8 values at a time
     hipLaunchKernelGGL(encode_kernel, dim3((rows + 255) / 256), dim3(256), 0, s, feat, n_in, idx, rows, mean, scale, K0, X, Xt, ldxt);
     return hipGetLastError();
 }
@@ -552,6 +566,7 @@ hipError_t mlp_encode(const float *feat, int n_in, const int *idx, int rows, con
 hipError_t mlp_loss(const float *Z, int C, const float *targ, int n_out, const int *idx, int rows,
                     const float *mean, const float *scale, uint16_t *dZ, uint16_t *dZt, int lddzt,
                     float *lrow, hipStream_t s) {
+    if (C % 8) return hipErrorInvalidValue;  // the kernel writes dZ rows 8 values at a time
     hipLaunchKernelGGL(loss_kernel, dim3((rows + 255) / 256), dim3(256), 0, s, Z, C, targ, n_out, idx, rows, mean, scale, dZ, dZt, lddzt, lrow);
     return hipGetLastError();
 }
