@@ -50,7 +50,8 @@ struct cfd_mlp_s {
     float *W = nullptr, *b = nullptr, *mW = nullptr, *vW = nullptr, *mb = nullptr, *vb = nullptr;
     float *gW = nullptr, *gb = nullptr;
     std::vector<uint16_t *> Wb, Wt, X, Xt, dZ, dZt;
-    std::vector<float *> Z;
+    std::vector<uint16_t *> Zh;     // bf16 pre-activations of the hidden layers
+    float *Zo = nullptr;            // fp32 outputs of the last layer
     float *part = nullptr;          // split-K partials of every layer's weight gradient
     cfd::MlpLayers tab{};           // shapes and buffers of all layers, for the all-layer launches
     float *norm = nullptr;  // in_mean[n_in], in_scale[n_in], out_mean[n_out], out_scale[n_out]
@@ -92,12 +93,13 @@ int pack_all(cfd_mlp_s *m, int *d_step = nullptr) {
 }
 
 // Forward over `rows` (padded) minibatch rows already encoded in X[0]; leaves the output layer's
-// fp32 result in Z[L-1] ([rows x Mp[L-1]]).
+// fp32 result in Zo ([rows x Mp[L-1]]) and the hidden layers' bf16 pre-activations in Zh.
 int forward(cfd_mlp_s *m, int rows) {
     for (int l = 0; l < m->L; l++) {
         const bool last = (l == m->L - 1);
         MHIP(mlp_gemm(last ? MLP_EPI_BIAS : MLP_EPI_BIAS_SILU, m->X[l], m->Kp[l], m->Wb[l], m->Kp[l], rows,
-                      m->Mp[l], m->Kp[l], 1, m->b + m->boff[l], nullptr, m->Z[l],
+                      m->Mp[l], m->Kp[l], 1, m->b + m->boff[l], nullptr,
+                      last ? static_cast<void *>(m->Zo) : static_cast<void *>(m->Zh[l]),
                       last ? nullptr : m->X[l + 1], last ? nullptr : m->Xt[l + 1], m->Mp[l], rows,
                       m->Mv[l], m->stream));
     }
@@ -113,7 +115,7 @@ int backward(cfd_mlp_s *m, int rows) {
         if (l > 0) {
             // dZ_{l-1} = (dZ_l W_l) * silu'(Z_{l-1}): A = dZ_l [rows x Mp], B = Wt_l [Kp x Mp]
             MHIP(mlp_gemm(MLP_EPI_DSILU, m->dZ[l], m->Mp[l], m->Wt[l], m->Mp[l], rows, m->Kp[l], m->Mp[l], 1,
-                          nullptr, m->Z[l - 1], nullptr, m->dZ[l - 1], m->dZt[l - 1], m->Kp[l], rows,
+                          nullptr, m->Zh[l - 1], nullptr, m->dZ[l - 1], m->dZt[l - 1], m->Kp[l], rows,
                           m->Mv[l - 1], m->stream));
         }
     }
@@ -193,7 +195,7 @@ int cfd_mlp_create(int n_in, int width, int depth, int n_out, int max_rows, unsi
     m->Wt.resize(m->L);
     m->X.resize(m->L);
     m->Xt.resize(m->L);
-    m->Z.resize(m->L);
+    m->Zh.resize(m->L, nullptr);
     m->dZ.resize(m->L);
     m->dZt.resize(m->L);
     for (int l = 0; ok && l < m->L; l++) {
@@ -201,7 +203,8 @@ int cfd_mlp_create(int n_in, int width, int depth, int n_out, int max_rows, unsi
         ok = (e = m->alloc(&m->Wb[l], mk)) == hipSuccess && (e = m->alloc(&m->Wt[l], mk)) == hipSuccess &&
              (e = m->alloc(&m->X[l], (size_t)R * m->Kp[l])) == hipSuccess &&
              (e = m->alloc(&m->Xt[l], (size_t)R * m->Kp[l])) == hipSuccess &&
-             (e = m->alloc(&m->Z[l], (size_t)R * m->Mp[l])) == hipSuccess &&
+             (e = (l < m->L - 1 ? m->alloc(&m->Zh[l], (size_t)R * m->Mp[l])
+                                : m->alloc(&m->Zo, (size_t)R * m->Mp[l]))) == hipSuccess &&
              (e = m->alloc(&m->dZ[l], (size_t)R * m->Mp[l])) == hipSuccess &&
              (e = m->alloc(&m->dZt[l], (size_t)R * m->Mp[l])) == hipSuccess;
     }
@@ -302,7 +305,7 @@ int cfd_mlp_gradients(cfd_mlp_t *m, const float *feat, const float *targ, int n,
     rc = forward(m, n);
     if (rc) return rc;
     const int Lm = m->L - 1;
-    MHIP(mlp_loss(m->Z[Lm], m->Mp[Lm], m->targ, m->n_out, nullptr, n, nrm(m, 2), nrm(m, 3), m->dZ[Lm], m->dZt[Lm], n,
+    MHIP(mlp_loss(m->Zo, m->Mp[Lm], m->targ, m->n_out, nullptr, n, nrm(m, 2), nrm(m, 3), m->dZ[Lm], m->dZt[Lm], n,
                   m->lrow, m->stream));
     rc = backward(m, n);
     if (rc) return rc;
@@ -365,7 +368,7 @@ int cfd_mlp_train(cfd_mlp_t *m, const float *feat, const float *targ, int n, int
             MHIP(mlp_encode(m->feat, m->n_in, m->idx, batch, nrm(m, 0), nrm(m, 1), m->Kp[0], m->X[0], m->Xt[0],
                             batch, m->stream));
             if (int r = forward(m, batch)) return r;
-            MHIP(mlp_loss(m->Z[Lm], m->Mp[Lm], m->targ, m->n_out, m->idx, batch, nrm(m, 2), nrm(m, 3), m->dZ[Lm],
+            MHIP(mlp_loss(m->Zo, m->Mp[Lm], m->targ, m->n_out, m->idx, batch, nrm(m, 2), nrm(m, 3), m->dZ[Lm],
                           m->dZt[Lm], batch, m->lrow, m->stream));
             if (int r = backward(m, batch)) return r;
             MHIP(mlp_adam2_dev(m->W, m->mW, m->vW, m->gW, (int)m->nw, m->b, m->mb, m->vb, m->gb, (int)m->nb,
@@ -404,7 +407,7 @@ int cfd_mlp_train(cfd_mlp_t *m, const float *feat, const float *targ, int n, int
                         m->stream));
         rc = forward(m, batch);
         if (rc) return rc;
-        MHIP(mlp_loss(m->Z[Lm], m->Mp[Lm], m->targ, m->n_out, m->idx, batch, nrm(m, 2), nrm(m, 3), m->dZ[Lm],
+        MHIP(mlp_loss(m->Zo, m->Mp[Lm], m->targ, m->n_out, m->idx, batch, nrm(m, 2), nrm(m, 3), m->dZ[Lm],
                       m->dZt[Lm], batch, m->lrow, m->stream));
         rc = backward(m, batch);
         if (rc) return rc;
@@ -442,7 +445,7 @@ int cfd_mlp_predict(cfd_mlp_t *m, const float *feat, int n, float *out) {
                         m->stream));
         rc = forward(m, rows);
         if (rc) return rc;
-        MHIP(hipMemcpyAsync(z.data(), m->Z[Lm], (size_t)rows * C * sizeof(float), hipMemcpyDeviceToHost, m->stream));
+        MHIP(hipMemcpyAsync(z.data(), m->Zo, (size_t)rows * C * sizeof(float), hipMemcpyDeviceToHost, m->stream));
         MHIP(hipStreamSynchronize(m->stream));
         for (int r = 0; r < cnt; r++)
             for (int k = 0; k < m->n_out; k++)
@@ -465,7 +468,7 @@ int cfd_mlp_time_step(cfd_mlp_t *m, int batch, int reps, float *ms) {
                         m->stream));
         int rc = forward(m, batch);
         if (rc) return rc;
-        MHIP(mlp_loss(m->Z[Lm], m->Mp[Lm], m->targ, m->n_out, m->idx, batch, nrm(m, 2), nrm(m, 3), m->dZ[Lm],
+        MHIP(mlp_loss(m->Zo, m->Mp[Lm], m->targ, m->n_out, m->idx, batch, nrm(m, 2), nrm(m, 3), m->dZ[Lm],
                       m->dZt[Lm], batch, m->lrow, m->stream));
         rc = backward(m, batch);
         if (rc) return rc;

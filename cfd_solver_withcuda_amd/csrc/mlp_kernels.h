@@ -19,13 +19,14 @@ struct MlpLayers {
 };
 
 // C[r][c] = sum_k A[r][k] B[c][k] (bf16 in, fp32 accumulate) with the epilogue `epi`:
-//   BIAS_SILU: Zout = C + bias (fp32, ld C), Y = silu(Zout) (bf16, ld ldy), Yt = Y^T (ld ldyt)
-//   BIAS:      Zout = C + bias
-//   DSILU:     Y = C * silu'(Zprev) (bf16), Yt = Y^T
-//   PARTIAL:   Zout[blockIdx.y] = C over the blockIdx.y-th of ksplit K ranges
+//   BIAS_SILU: Zout = C + bias (bf16 pre-activations, ld C), Y = silu(C + bias) (bf16, ld ldy),
+//              Yt = Y^T (ld ldyt)
+//   BIAS:      Zout = C + bias (fp32)
+//   DSILU:     Y = C * silu'(Zprev) (Zprev the bf16 pre-activations; Y bf16), Yt = Y^T
+//   PARTIAL:   Zout[blockIdx.y] = C over the blockIdx.y-th of ksplit K ranges (fp32)
 // Columns c >= c_valid are written as zero. R, C, K multiples of 32; (K/32) % ksplit == 0.
 hipError_t mlp_gemm(int epi, const uint16_t *A, int lda, const uint16_t *B, int ldb, int R, int C,
-                    int K, int ksplit, const float *bias, const float *Zprev, float *Zout,
+                    int K, int ksplit, const float *bias, const void *Zprev, void *Zout,
                     uint16_t *Y, uint16_t *Yt, int ldy, int ldyt, int c_valid, hipStream_t s);
 hipError_t mlp_encode(const float *feat, int n_in, const int *idx, int rows, const float *mean,
                       const float *scale, int K0, uint16_t *X, uint16_t *Xt, int ldxt, hipStream_t s);

@@ -46,7 +46,7 @@ enum { EPI_BIAS_SILU = MLP_EPI_BIAS_SILU, EPI_BIAS = MLP_EPI_BIAS, EPI_DSILU = M
 template <int EPI>
 __device__ __forceinline__ void gemm_epilogue(const f32x4 (&acc)[2][2], int r0, int c0, int l, int ks, int R,
                                               int C, const float *__restrict__ bias,
-                                              const float *__restrict__ Zprev, float *__restrict__ Zout,
+                                              const void *__restrict__ Zprev, void *__restrict__ Zout,
                                               uint16_t *__restrict__ Y, uint16_t *__restrict__ Yt, int ldy,
                                               int ldyt, int c_valid) {
     const int lr = l % 16;
@@ -56,17 +56,19 @@ __device__ __forceinline__ void gemm_epilogue(const f32x4 (&acc)[2][2], int r0, 
             const int rb = r0 + 16 * i + 4 * (l / 16);
             float v[4] = {acc[i][j][0], acc[i][j][1], acc[i][j][2], acc[i][j][3]};
             if constexpr (EPI == EPI_PARTIAL) {
-                float *P = Zout + (size_t)ks * R * C;
+                float *P = static_cast<float *>(Zout) + (size_t)ks * R * C;
                 for (int q = 0; q < 4; q++) P[(size_t)(rb + q) * C + c] = v[q];
             } else if constexpr (EPI == EPI_BIAS) {
+                float *Z = static_cast<float *>(Zout);
                 const float bb = (c < c_valid) ? bias[c] : 0.f;
-                for (int q = 0; q < 4; q++) Zout[(size_t)(rb + q) * C + c] = (c < c_valid) ? v[q] + bb : 0.f;
+                for (int q = 0; q < 4; q++) Z[(size_t)(rb + q) * C + c] = (c < c_valid) ? v[q] + bb : 0.f;
             } else if constexpr (EPI == EPI_BIAS_SILU) {
+                uint16_t *Zh = static_cast<uint16_t *>(Zout);
                 const float bb = (c < c_valid) ? bias[c] : 0.f;
                 uint16_t ys[4];
                 for (int q = 0; q < 4; q++) {
                     const float z = (c < c_valid) ? v[q] + bb : 0.f;
-                    Zout[(size_t)(rb + q) * C + c] = z;
+                    Zh[(size_t)(rb + q) * C + c] = f2bf(z);
                     ys[q] = (c < c_valid) ? f2bf(z * sigmoidf_(z)) : (uint16_t)0;
                     Y[(size_t)(rb + q) * ldy + c] = ys[q];
                 }
@@ -75,12 +77,13 @@ __device__ __forceinline__ void gemm_epilogue(const f32x4 (&acc)[2][2], int r0, 
                 pk.x = (uint32_t)ys[0] | ((uint32_t)ys[1] << 16);
                 pk.y = (uint32_t)ys[2] | ((uint32_t)ys[3] << 16);
                 *reinterpret_cast<uint2 *>(Yt + (size_t)c * ldyt + rb) = pk;
-            } else {  // EPI_DSILU: dZprev = acc * silu'(Zprev)
+            } else {  // EPI_DSILU: dZprev = acc * silu'(Zprev), Zprev the bf16 pre-activations
+                const uint16_t *Zh = static_cast<const uint16_t *>(Zprev);
                 uint16_t ys[4];
                 for (int q = 0; q < 4; q++) {
                     float d = 0.f;
                     if (c < c_valid) {
-                        const float z = Zprev[(size_t)(rb + q) * C + c];
+                        const float z = bf2f(Zh[(size_t)(rb + q) * C + c]);
                         const float s = sigmoidf_(z);
                         d = v[q] * (s * (1.0f + z * (1.0f - s)));
                     }
@@ -100,8 +103,8 @@ __global__ void __launch_bounds__(64) gemm_nt_kernel(const uint16_t *__restrict_
                                                      const uint16_t *__restrict__ B, int ldb,
                                                      int R, int C, int K, int ksplit,
                                                      const float *__restrict__ bias,
-                                                     const float *__restrict__ Zprev,
-                                                     float *__restrict__ Zout,
+                                                     const void *__restrict__ Zprev,
+                                                     void *__restrict__ Zout,
                                                      uint16_t *__restrict__ Y, uint16_t *__restrict__ Yt,
                                                      int ldy, int ldyt, int c_valid) {
     const int tilesC = C / 32;
@@ -144,7 +147,7 @@ constexpr int kStageRow = 65;
 template <int EPI>
 __device__ __forceinline__ void gemm64_epilogue(const f32x4 (&acc)[2][2], float *S, int R0, int C0, int w, int l,
                                                 int t, int ks, int R, int C, const float *__restrict__ bias,
-                                                const float *__restrict__ Zprev, float *__restrict__ Zout,
+                                                const void *__restrict__ Zprev, void *__restrict__ Zout,
                                                 uint16_t *__restrict__ Y, uint16_t *__restrict__ Yt, int ldy,
                                                 int ldyt, int c_valid) {
     // 1. results into S (the loop's last barrier ended every read of the operand slices)
@@ -167,11 +170,19 @@ __device__ __forceinline__ void gemm64_epilogue(const f32x4 (&acc)[2][2], float 
     __syncthreads();
     if constexpr (EPI == EPI_PARTIAL || EPI == EPI_BIAS || EPI == EPI_BIAS_SILU) {
         // 2. fp32 rows: four consecutive values per thread and pass
-        float *out = (EPI == EPI_PARTIAL) ? Zout + (size_t)ks * R * C : Zout;
         for (int e = t; e < 64 * 16; e += 256) {
             const int r = e / 16, c = 4 * (e % 16);
             float *sr = S + r * kStageRow + c;
-            *reinterpret_cast<float4 *>(out + (size_t)(R0 + r) * C + C0 + c) = make_float4(sr[0], sr[1], sr[2], sr[3]);
+            const size_t o = (size_t)(R0 + r) * C + C0 + c;
+            if constexpr (EPI == EPI_BIAS_SILU) {
+                // bf16 pre-activations (what the SiLU' product reads back)
+                const uint32_t lo = (uint32_t)f2bf(sr[0]) | ((uint32_t)f2bf(sr[1]) << 16);
+                const uint32_t hi = (uint32_t)f2bf(sr[2]) | ((uint32_t)f2bf(sr[3]) << 16);
+                *reinterpret_cast<uint2 *>(static_cast<uint16_t *>(Zout) + o) = make_uint2(lo, hi);
+            } else {
+                float *out = static_cast<float *>(Zout) + ((EPI == EPI_PARTIAL) ? (size_t)ks * R * C : 0);
+                *reinterpret_cast<float4 *>(out + o) = make_float4(sr[0], sr[1], sr[2], sr[3]);
+            }
             if constexpr (EPI == EPI_BIAS_SILU) {
                 // the activation replaces the pre-activation in S (one SiLU per element)
                 for (int q = 0; q < 4; q++) {
@@ -183,12 +194,14 @@ __device__ __forceinline__ void gemm64_epilogue(const f32x4 (&acc)[2][2], float 
         if constexpr (EPI == EPI_BIAS_SILU) __syncthreads();
     }
     if constexpr (EPI == EPI_DSILU) {
-        // 2. dZprev = acc * silu'(Zprev) in place in S, read by rows with Zprev (padding columns
-        // of Zprev hold zero; their results are selected to zero)
+        // 2. dZprev = acc * silu'(Zprev) in place in S, read by rows with the bf16 Zprev (padding
+        // columns of Zprev hold zero; their results are selected to zero)
         for (int e = t; e < 64 * 16; e += 256) {
             const int r = e / 16, c = 4 * (e % 16);
-            const float4 z4 = *reinterpret_cast<const float4 *>(Zprev + (size_t)(R0 + r) * C + C0 + c);
-            const float zz[4] = {z4.x, z4.y, z4.z, z4.w};
+            const uint2 z2 = *reinterpret_cast<const uint2 *>(static_cast<const uint16_t *>(Zprev) +
+                                                              (size_t)(R0 + r) * C + C0 + c);
+            const float zz[4] = {bf2f((uint16_t)(z2.x & 0xffff)), bf2f((uint16_t)(z2.x >> 16)),
+                                 bf2f((uint16_t)(z2.y & 0xffff)), bf2f((uint16_t)(z2.y >> 16))};
             float *sr = S + r * kStageRow + c;
             for (int q = 0; q < 4; q++) {
                 const float z = zz[q];
@@ -223,8 +236,8 @@ __global__ void __launch_bounds__(256) gemm_nt64_kernel(const uint16_t *__restri
                                                         const uint16_t *__restrict__ B, int ldb,
                                                         int R, int C, int K, int ksplit,
                                                         const float *__restrict__ bias,
-                                                        const float *__restrict__ Zprev,
-                                                        float *__restrict__ Zout,
+                                                        const void *__restrict__ Zprev,
+                                                        void *__restrict__ Zout,
                                                         uint16_t *__restrict__ Y, uint16_t *__restrict__ Yt,
                                                         int ldy, int ldyt, int c_valid) {
     // the K slices' operands; after the loop the same LDS stages the 64 x 64 result
@@ -523,7 +536,7 @@ hipError_t mlp_adam2_dev(float *W, float *mW, float *vW, const float *gW, int nw
 namespace {
 template <int EPI>
 void launch_gemm(const uint16_t *A, int lda, const uint16_t *B, int ldb, int R, int C, int K, int ksplit,
-                 const float *bias, const float *Zprev, float *Zout, uint16_t *Y, uint16_t *Yt, int ldy, int ldyt,
+                 const float *bias, const void *Zprev, void *Zout, uint16_t *Y, uint16_t *Yt, int ldy, int ldyt,
                  int c_valid, hipStream_t s) {
     if (R % 64 == 0 && C % 64 == 0) {
         hipLaunchKernelGGL(gemm_nt64_kernel<EPI>, dim3((R / 64) * (C / 64), ksplit), dim3(256), 0, s, A, lda, B, ldb,
@@ -536,7 +549,7 @@ void launch_gemm(const uint16_t *A, int lda, const uint16_t *B, int ldb, int R, 
 }  // namespace
 
 hipError_t mlp_gemm(int epi, const uint16_t *A, int lda, const uint16_t *B, int ldb, int R, int C,
-                    int K, int ksplit, const float *bias, const float *Zprev, float *Zout,
+                    int K, int ksplit, const float *bias, const void *Zprev, void *Zout,
                     uint16_t *Y, uint16_t *Yt, int ldy, int ldyt, int c_valid, hipStream_t s) {
     if (R % 32 || C % 32 || K % 32 || ksplit < 1 || (K / 32) % ksplit) return hipErrorInvalidValue;
     switch (epi) {

@@ -6,9 +6,9 @@ by finite differences and by PyTorch autograd (tests/test_oracle_mlp.py).
 Network: x -> depth hidden layers z = W x + b, y = silu(z) -> output z = W x + b. Parameters are
 concatenated per layer as in the C-ABI: W_l row-major [out x in], then b_l. The loss is the mean
 squared error of the normalised outputs, (v - mean) * scale. With bf16=True the values the
-device stores in BF16 (inputs, weights, hidden activations, output gradients and hidden
-pre-activation gradients) are rounded to BF16 (round to nearest even) at the same points; all
-arithmetic stays FP64.
+device stores in BF16 (inputs, weights, hidden activations, hidden pre-activations as the
+backward pass reads them, output gradients and hidden pre-activation gradients) are rounded to
+BF16 (round to nearest even) at the same points; all arithmetic stays FP64.
 """
 from __future__ import annotations
 
@@ -78,7 +78,7 @@ class MLP:
             gW[l] = dZ.T @ xs[l]
             gb[l] = dZ.sum(axis=0)
             if l > 0:
-                dZ = q((dZ @ q(Ws[l])) * dsilu(zs[l - 1]))
+                dZ = q((dZ @ q(Ws[l])) * dsilu(q(zs[l - 1])))
         return loss, np.concatenate([g.ravel() for g in gW]), np.concatenate(gb)
 
     def init(self, seed=0):
