@@ -37,7 +37,8 @@ namespace {
 
 int pad32(int n) { return (n + 31) / 32 * 32; }
 constexpr int kRowAlign = 512;  // minibatch rows: multiple of 32 * split-K factor
-constexpr int kSplit = 16;      // split-K factor of the weight-gradient GEMMs
+constexpr int kSplit = 16;      // split-K factor of the weight-gradient GEMMs (rows a multiple of 512)
+constexpr int kSplitWide = 32;  // ... where the rows are a multiple of 1024 (more workgroups in flight)
 
 }  // namespace
 
@@ -108,9 +109,10 @@ int forward(cfd_mlp_s *m, int rows) {
 
 // Backward from dZ[L-1]/dZt[L-1] (set by the loss kernel); weight and bias gradients into gW, gb.
 int backward(cfd_mlp_s *m, int rows) {
+    const int split = (rows % (32 * kSplitWide) == 0) ? kSplitWide : kSplit;
     for (int l = m->L - 1; l >= 0; l--) {
         // dW_l = dZ_l^T X_l: A = dZt_l [Mp x rows], B = Xt_l [Kp x rows], split over rows
-        MHIP(mlp_gemm(MLP_EPI_PARTIAL, m->dZt[l], rows, m->Xt[l], rows, m->Mp[l], m->Kp[l], rows, kSplit,
+        MHIP(mlp_gemm(MLP_EPI_PARTIAL, m->dZt[l], rows, m->Xt[l], rows, m->Mp[l], m->Kp[l], rows, split,
                       nullptr, nullptr, m->part + m->tab.poff[l], nullptr, nullptr, 0, 0, m->Mp[l], m->stream));
         if (l > 0) {
             // dZ_{l-1} = (dZ_l W_l) * silu'(Z_{l-1}): A = dZ_l [rows x Mp], B = Wt_l [Kp x Mp]
@@ -121,7 +123,7 @@ int backward(cfd_mlp_s *m, int rows) {
     }
     // every layer's split-K partials summed into gW, and every layer's bias gradient, in one
     // launch each
-    MHIP(mlp_reduce_layers(m->part, kSplit, m->tab, m->gW, m->stream));
+    MHIP(mlp_reduce_layers(m->part, split, m->tab, m->gW, m->stream));
     MHIP(mlp_row_sum_layers(m->tab, rows, m->gb, m->stream));
     return 0;
 }
@@ -180,14 +182,14 @@ int cfd_mlp_create(int n_in, int width, int depth, int n_out, int max_rows, unsi
     size_t sumMK = 0;
     for (int l = 0; l < m->L; l++) {
         maxMK = std::max(maxMK, m->Mp[l] * m->Kp[l]);
-        m->tab.poff[l] = (long long)(kSplit * sumMK);
+        m->tab.poff[l] = (long long)(kSplitWide * sumMK);
         sumMK += (size_t)m->Mp[l] * m->Kp[l];
     }
     bool ok = (e = m->alloc(&m->W, m->nw)) == hipSuccess && (e = m->alloc(&m->b, m->nb)) == hipSuccess &&
               (e = m->alloc(&m->mW, m->nw)) == hipSuccess && (e = m->alloc(&m->vW, m->nw)) == hipSuccess &&
               (e = m->alloc(&m->mb, m->nb)) == hipSuccess && (e = m->alloc(&m->vb, m->nb)) == hipSuccess &&
               (e = m->alloc(&m->gW, m->nw)) == hipSuccess && (e = m->alloc(&m->gb, m->nb)) == hipSuccess &&
-              (e = m->alloc(&m->part, (size_t)kSplit * sumMK)) == hipSuccess &&
+              (e = m->alloc(&m->part, (size_t)kSplitWide * sumMK)) == hipSuccess &&
               (e = m->alloc(&m->norm, 2 * (size_t)(n_in + n_out))) == hipSuccess &&
               (e = m->alloc(&m->lrow, (size_t)R)) == hipSuccess && (e = m->alloc(&m->idx, (size_t)R)) == hipSuccess &&
               (e = m->alloc(&m->d_step, 1)) == hipSuccess;
